@@ -1,0 +1,216 @@
+/*
+ * tqf.h - C ABI of the MI355X Monte Carlo path-simulation engine.
+ *
+ * The reference (google/tf-quant-finance) is pure Python on TensorFlow and
+ * has no FFI; its operator API is a handful of Python functions.  Each entry
+ * point below replaces the TF ops behind one of those functions and names
+ * the reference function (file:line) it serves.  The Python package `tqf`
+ * binds these symbols with ctypes and keeps the reference's signatures.
+ *
+ * Conventions
+ *  - Every pointer argument marked "device" is HBM memory owned by the
+ *    caller (torch allocates it); the library allocates no device memory.
+ *    Work buffers are passed in; *_workspace_size() gives their size.
+ *  - `stream` is a hipStream_t (torch.cuda.current_stream().cuda_stream).
+ *    Launches are asynchronous on it; nothing synchronises the host.
+ *  - Return value: 0 ok; TQF_EINVAL (< 0) for an invalid argument (Python
+ *    raises ValueError), TQF_ENOTIMPL for an unsupported option
+ *    (NotImplementedError); > 0 is a hipError_t (RuntimeError).
+ *    tqf_last_error() returns the calling thread's message.  Nothing aborts.
+ *  - dtype: TQF_F32 or TQF_F64 selects the arithmetic of a call; all real
+ *    device arrays of that call use it unless stated otherwise.
+ */
+#ifndef TQF_H_
+#define TQF_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TQF_OK 0
+#define TQF_EINVAL (-1)
+#define TQF_ENOTIMPL (-2)
+
+#define TQF_F32 0
+#define TQF_F64 1
+
+/* ---- library ----------------------------------------------------------- */
+int tqf_version(void);
+const char* tqf_last_error(void);
+
+/* ---- Sobol ------------------------------------------------------------- */
+
+/* Host.  Joe-Kuo direction numbers m[dim][32] (uint32, low 32 bits) from the
+ * parsed data file: poly[k] = 2^s + 2a + 1 and minit[k*18 + i] = m_{i+1} of
+ * line k.  Replaces tff/math/random_ops/sobol/sobol_impl.py:171-197
+ * (_compute_direction_numbers). */
+int tqf_sobol_direction_numbers(const int64_t* poly, const int64_t* minit,
+                                int32_t n_poly, int32_t dim, uint32_t* m_out);
+
+/* Device.  Integer or real Sobol points, row-major out[i*out_stride + d].
+ *   v        device [dim][32] generating columns already aligned to `nd`
+ *            digits (column j = m_j << (nd - 1 - j)).
+ *   index_i  = first_index + i, or seq_indices[i] when seq_indices != NULL.
+ *   shift    device [dim] digital shift (XOR) or NULL.
+ *   out_kind 0: uint32 integers; 1: float32 cast(int) * 2^-nd;
+ *            2: float64 cast(int) * 2^-nd.  tent != 0 applies the tent map.
+ * Replaces sobol_impl.py:39-167 (sobol.sample: first_index = skip + 1,
+ * nd = ceil(log2(skip + N + 1))) and tff/math/qmc/digital_net.py:205-419
+ * (digital_net_sample / qmc.sobol_sample: first_index = 0). */
+int tqf_sobol_points(const uint32_t* v, int32_t dim, int32_t nd, int64_t n,
+                     int64_t first_index, const int64_t* seq_indices,
+                     const uint32_t* shift, int32_t out_kind, int32_t tent,
+                     void* out, int64_t out_stride, void* stream);
+
+/* Device.  lo[d][256] = XOR of v[d][b] over the set bits b < 8 of the table
+ * index: the per-lane part of a point, used by the fused steppers. */
+int tqf_sobol_lo_table(const uint32_t* v, int32_t dim, uint32_t* lo,
+                       void* stream);
+
+/* Device.  Normal draws from Sobol points, z = sqrt(2) erfinv(2u - 1),
+ * out[i*dim + d] in `dtype`.  mode 0: the reference's transform of
+ * sobol.sample (u = cast(int) 2^-nd, tff/math/random_ops/
+ * multivariate_normal.py:356-424); mode 1: the engine's shifted transform
+ * (u = (top24 + 1/2) 2^-24 for float32, (int + 1/2) 2^-32 for float64). */
+int tqf_sobol_normals(const uint32_t* v, int32_t dim, int32_t nd, int64_t n,
+                      int64_t first_index, const uint32_t* shift,
+                      int32_t mode, int32_t dtype, void* out, void* stream);
+
+/* ---- Philox normals ---------------------------------------------------- */
+
+/* Device.  out[p*n_draws + j] = normal j of path p of the engine's Philox
+ * stream (Philox4x32-10 counter (p_lo, j/4, stream_id, p_hi), key = seed,
+ * Box-Muller pairs).  antithetic != 0: rows p and p + n_paths/2 are (z, -z).
+ * Replaces tf.random.normal / stateless_normal at
+ * tff/math/random_ops/multivariate_normal.py:248-311. */
+int tqf_philox_normals(uint64_t seed, uint32_t stream_id, int64_t n_paths,
+                       int64_t n_draws, int32_t antithetic, int32_t dtype,
+                       void* out, void* stream);
+
+/* ---- Fused Euler-Maruyama steppers ------------------------------------- */
+
+#define TQF_MODEL_GBM 0       /* dX = mu X dt + sigma X dW (Euler on S)     */
+#define TQF_MODEL_CONST 1     /* dX = a dt + b dW, constant a, b (log GBM)  */
+#define TQF_MODEL_HESTON_FT 2 /* (log S, v), full truncation, drift r      */
+#define TQF_MODEL_LOCAL_VOL 3 /* log S, sigma(t, S) from a bilinear table   */
+#define TQF_MODEL_HESTON_QE 4 /* (log S, v), Andersen QE (HestonModel)      */
+
+#define TQF_RNG_DRAWS 0            /* injected normals                      */
+#define TQF_RNG_PHILOX 1
+#define TQF_RNG_PHILOX_ANTITHETIC 2
+#define TQF_RNG_SOBOL 3            /* reference transform (mode 0 above)    */
+#define TQF_RNG_SOBOL_SHIFTED 4    /* engine transform (mode 1 above)       */
+
+#define TQF_OUT_PATH_MAJOR 0  /* out[p][slot][d]  (reference layout)        */
+#define TQF_OUT_STEP_MAJOR 1  /* out[slot][d][p]  (coalesced path store)    */
+#define TQF_OUT_PAYOFF 2      /* European calls reduced in the kernel       */
+
+typedef struct tqf_model {
+  int32_t kind;
+  int32_t dim;
+  double p[8];  /* GBM: mu, sigma.  CONST: a, b.
+                   HESTON_FT: r, kappa, theta, xi, rho, variant
+                   (variant 0: full truncation; 1: HestonModel's own
+                   coefficients, raw v drift and sqrt|v| vol).
+                   LOCAL_VOL: drift rate (r - q).
+                   HESTON_QE: theta (per-step values in step_coef). */
+  const void* lv_vol;   /* LOCAL_VOL: device [nt][nx] sigma table          */
+  const void* lv_t;     /* device [nt] table times (increasing)            */
+  double lv_x0;         /* log-spot of table column 0                      */
+  double lv_dx;         /* uniform log-spot spacing                        */
+  int32_t lv_nt;
+  int32_t lv_nx;
+  const void* step_coef;  /* HESTON_QE: device [n_steps][12] per-step
+                             constants: 0 e = exp(-kappa dt), 1 c1, 2 c2
+                             (s^2 = v c1 + c2), 3..7 k0..k4 of the log-spot
+                             update, 8 theta, 9 update flag (0 when
+                             dt <= tolerance), 10-11 unused */
+} tqf_model;
+
+typedef struct tqf_grid {
+  const void* dt;       /* device [n_steps] times[i+1] - times[i]          */
+  const void* sqrt_dt;  /* device [n_steps]                                */
+  const void* t_next;   /* device [n_steps] times[i+1]                     */
+  const int32_t* slot;  /* device [n_steps] slot written after step i, -1  */
+  int32_t n_steps;      /* steps the loop executes                         */
+  int32_t n_slots;      /* requested times k                               */
+  int32_t init_slot;    /* slot holding the initial state, or -1           */
+  int32_t n_rng_steps;  /* steps the draws are laid out for (>= n_steps)   */
+} tqf_grid;
+
+typedef struct tqf_rng {
+  int32_t kind;
+  uint32_t stream_id;
+  uint64_t seed;
+  const void* draws;           /* DRAWS: element (p, i, k) at
+                                  draws[p*stride_path + i*stride_step + k]  */
+  int64_t stride_path;
+  int64_t stride_step;
+  const uint32_t* sobol_v;     /* SOBOL*: device [ndims][32]                */
+  const uint32_t* sobol_lo;    /* SOBOL*: device [ndims][256]               */
+  const uint32_t* sobol_shift; /* SOBOL*: device [ndims] or NULL            */
+  int64_t sobol_first_index;   /* sequence index of path 0                  */
+  int32_t sobol_nd;
+  int32_t sobol_ndims;
+} tqf_rng;
+
+typedef struct tqf_output {
+  int32_t mode;
+  int32_t n_strikes;         /* PAYOFF                                      */
+  void* paths;               /* PATH_MAJOR / STEP_MAJOR                     */
+  const double* strikes;     /* PAYOFF: device [n_strikes] (calls)          */
+  const double* discount;    /* PAYOFF: device [n_slots] discount factors   */
+  double* workspace;         /* PAYOFF: tqf_euler_workspace_size() bytes    */
+  double* prices;            /* PAYOFF: device [n_slots][n_strikes]         */
+  double* stderrs;           /* PAYOFF: device [n_slots][n_strikes]         */
+  double* sums;              /* PAYOFF: device [n_slots][n_strikes][2]
+                                (sum, sum of squares) of raw payoffs, or NULL */
+} tqf_output;
+
+/* Workspace bytes for TQF_OUT_PAYOFF. */
+int64_t tqf_euler_workspace_size(int64_t n_paths, int32_t n_slots,
+                                 int32_t n_strikes);
+
+/* Device.  Fused Euler-Maruyama: draws, steps, records and (optionally)
+ * payoffs in one kernel with the state in registers.  x0: device
+ * [n_paths][dim] (x0_stride_path = dim) or [dim] (x0_stride_path = 0).
+ * Replaces tff/models/euler_sampling.py:335-537 (_sample, _while_loop,
+ * _euler_step) for the known coefficient functions of
+ * GeometricBrownianMotion (univariate...py:127-153), HestonModel
+ * (heston_model.py:143-173, with truncation and drift r) and
+ * LocalVolatilityModel (local_volatility_model.py:461-476). */
+int tqf_euler(const tqf_model* model, const tqf_grid* grid,
+              const tqf_rng* rng, const tqf_output* out, int64_t n_paths,
+              const void* x0, int64_t x0_stride_path, int32_t dtype,
+              void* stream);
+
+/* ---- Exact geometric Brownian motion ----------------------------------- */
+
+typedef struct tqf_gbm {
+  int32_t dim;              /* assets, 1..16                                 */
+  const void* drift;        /* device [dim] mu_i - sigma_i^2 / 2             */
+  const void* vol;          /* device [dim] sigma_i                          */
+  const void* chol;         /* device [dim][dim] lower Cholesky factor of the
+                               correlation matrix, or NULL (independent)     */
+  const void* s0;           /* device [dim] or [n_paths][dim]                */
+  int64_t s0_stride_path;   /* 0 or dim                                      */
+} tqf_gbm;
+
+/* Device.  S_i(t) = S0_i exp(sum (mu_i - sigma_i^2/2) dt + sqrt(dt) sigma_i
+ * (L z)_i) over the grid, recorded at the grid's slots into `out`
+ * (TQF_OUT_PATH_MAJOR [N][k][dim] or TQF_OUT_STEP_MAJOR [k][dim][N]).
+ * Replaces GeometricBrownianMotion._sample_paths (tff/models/
+ * geometric_brownian_motion/univariate_geometric_brownian_motion.py:261-317)
+ * and MultivariateGeometricBrownianMotion._sample_paths
+ * (multivariate_geometric_brownian_motion.py:229-282). */
+int tqf_gbm_exact(const tqf_gbm* gbm, const tqf_grid* grid, const tqf_rng* rng,
+                  int32_t out_mode, void* out, int64_t n_paths, int32_t dtype,
+                  void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* TQF_H_ */

@@ -1,0 +1,26 @@
+"""Oracle: uniform -> normal transforms.
+
+  * quasi draws: z = erfinv((u - 0.5) * 2) * sqrt(2)
+    (tff/math/random_ops/multivariate_normal.py:420), evaluated here in
+    float64 with scipy.special.erfinv.
+  * the engine's shifted-Sobol transform: 24-bit uniform u = (m + 1/2) 2^-24
+    from the top 24 bits m of a 32-bit point, q = 2u - 1 formed exactly,
+    z = sqrt(2) erfinv(q).
+
+Pinned by scipy's independent special functions (erfinv/ndtri agree) and by
+symmetry z(u) = -z(1-u); the device float32 transform is compared with a
+stated ULP-style tolerance, the float64 one at 1e-13 relative.
+"""
+import numpy as np
+from scipy import special
+
+
+def quasi_normals(u):
+  u = np.asarray(u, np.float64)
+  return special.erfinv((u - 0.5) * 2) * np.sqrt(2.0)
+
+
+def shifted_sobol_normals(ints_u32):
+  m = (np.asarray(ints_u32, np.uint64) >> np.uint64(8)).astype(np.float64)
+  q = (2 * m + 1 - 2.0**24) * 2.0**-24
+  return special.erfinv(q) * np.sqrt(2.0)

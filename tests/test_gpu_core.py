@@ -1,0 +1,334 @@
+"""GPU parity tests: Sobol, Philox, fused Euler / exact GBM / Heston QE
+kernels against the CPU oracle through the C ABI.
+
+Tolerances: integers bit-exact; float64 paths 1e-12 relative (scale-aware),
+float32 paths 2e-5 relative, with identical injected draws (the north-star
+bar).  Streams generated in the kernels (Philox/Box-Muller, Sobol + inverse
+CDF) are compared with the oracle's streams at the precision of the device
+transcendentals, stated per test.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import load_golden
+from oracle import closed_form, euler as oeuler, gbm as ogbm
+from oracle import heston as oheston, normal as onormal, rng as orng
+from oracle import sobol as osobol
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+  import tqf
+  from tqf.engine import euler as engine
+  from tqf.engine import pricing
+  from tqf.math import qmc
+  from tqf.math.random import sobol as tsobol
+  from tqf.math.random import multivariate_normal as mvn
+
+
+def _np(t):
+  return t.detach().cpu().numpy()
+
+
+def _close(got, want, rtol, scale=None):
+  got, want = np.asarray(got), np.asarray(want)
+  scale = np.max(np.abs(want)) if scale is None else scale
+  err = np.max(np.abs(got - want) / (np.abs(want) + 1e-3 * scale + 1e-300))
+  assert err <= rtol, 'max scaled rel err %.3e > %.1e' % (err, rtol)
+
+
+# ---------------------------------------------------------------- Sobol ----
+
+@pytest.mark.parametrize('dim,n,skip', [
+    (1, 1, 0), (2, 31, 0), (5, 1000, 17), (1, 3, 2**31 - 5), (2, 2**20, 0),
+    (730, 2**13, 0), (730, 4096, 12345), (37, 777, 2**20 - 3)])
+def test_sobol_integers_bit_exact(dim, n, skip):
+  print('sobol ints dim=%d n=%d skip=%d' % (dim, n, skip), flush=True)
+  got, nd = tsobol.sample_int(dim, n, skip)
+  want, nd_o = osobol.sample_int(dim, n, skip)
+  assert nd == nd_o
+  np.testing.assert_array_equal(_np(got).view(np.uint32).astype(np.uint64),
+                                want)
+
+
+def test_sobol_sample_goldens_and_dtypes():
+  g = load_golden('sobol_golden.json')
+  e = g['random_ops_dim2_first5']
+  for dt in (np.float16, np.float32, np.float64):
+    got = tqf.math.random.sobol.sample(2, 5, dtype=dt)
+    np.testing.assert_allclose(_np(got).astype(np.float64), e['expected'],
+                               rtol=1e-6)
+  e = g['random_ops_large_skip']
+  np.testing.assert_allclose(_np(tqf.math.random.sobol.sample(1, 3, e['skip'])),
+                             e['expected'])
+  s = _np(tqf.math.random.sobol.sample(5, 31))
+  assert sorted(map(tuple, s.tolist())) == sorted(
+      map(tuple, g['random_ops_dim5_31_as_set']['expected']))
+  a = _np(tqf.math.random.sobol.sample(10, 67))
+  b = _np(tqf.math.random.sobol.sample(10, 50, skip=17))
+  np.testing.assert_array_equal(a[17:], b)
+  # float32 rounding of cast(int) / 2^nd matches the oracle at nd = 31
+  got = _np(tqf.math.random.sobol.sample(3, 1000, skip=2**31 - 1002))
+  np.testing.assert_array_equal(got, osobol.sample(3, 1000, 2**31 - 1002))
+
+
+def test_sobol_validate_args():
+  with pytest.raises(tqf.errors.InvalidArgumentError):
+    tqf.math.random.sobol.sample(1, 4, skip=2**31 - 5, validate_args=True)
+
+
+def test_qmc_goldens_shift_indices_tent():
+  g = load_golden('sobol_golden.json')
+  np.testing.assert_allclose(_np(qmc.sobol_sample(5, 29)),
+                             g['qmc_sobol_29x5']['expected'])
+  e = g['qmc_sequence_indices']
+  np.testing.assert_allclose(
+      _np(qmc.sobol_sample(5, 29, sequence_indices=e['indices'])),
+      e['expected'])
+  np.testing.assert_allclose(
+      _np(qmc.sobol_sample(6, 8, apply_tent_transform=True)),
+      g['qmc_tent_6x8']['expected'])
+  np.testing.assert_allclose(_np(qmc.sobol_sample(6, 8, dtype=np.float64)),
+                             g['qmc_dtype_6x8']['expected'])
+  gm = qmc.sobol_generating_matrices(5, 31, 5)
+  np.testing.assert_array_equal(gm, g['qmc_generating_matrices']['expected'])
+  # digital shift: XOR with random 20-digit words
+  shift = qmc.random_digital_shift(9, 20, seed=11)
+  assert np.all(shift >= 2**19) and np.all(shift < 2**20)
+  gm = qmc.sobol_generating_matrices(9, 5000, 20)
+  got = _np(qmc.digital_net_sample(gm, 5000, 20, digital_shift=shift,
+                                   dtype=np.float64))
+  want = osobol.digital_net_sample(gm, 5000, 20, digital_shift=shift,
+                                   dtype=np.float64)
+  np.testing.assert_array_equal(got, want)
+
+
+# ---------------------------------------------------------------- Philox ---
+
+@pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-13),
+                                       (torch.float32, 2e-5)])
+def test_philox_normals_match_oracle(dtype, tol):
+  npd = np.float64 if dtype == torch.float64 else np.float32
+  got = _np(mvn.philox_normals(3000, 7, 1234, dtype, stream_id=3))
+  want = orng.box_muller(orng.raw_words(3000, 7, 1234, stream=3), npd)
+  assert np.max(np.abs(got - want)) <= tol * 10
+  got = _np(mvn.philox_normals(64, 5, 99, dtype, antithetic=True))
+  np.testing.assert_array_equal(got[:32], -got[32:])
+  big = _np(mvn.philox_normals(2**18, 8, 7, dtype)).ravel()
+  assert abs(big.mean()) < 5 / np.sqrt(big.size)
+  assert abs(big.var() - 1) < 5 * np.sqrt(2 / big.size)
+
+
+def test_mv_normal_sample_sobol_layout():
+  # SOBOL through mv_normal_sample: erfinv((u - .5) * 2) sqrt 2 of sobol.sample
+  z = _np(tqf.math.random.mv_normal_sample([1000], mean=np.zeros(6),
+                                           random_type=mvn.RandomType.SOBOL,
+                                           skip=5, dtype=np.float64))
+  u = osobol.sample(6, 1000, skip=5, dtype=np.float64)
+  np.testing.assert_allclose(z, onormal.quasi_normals(u), rtol=1e-13,
+                             atol=1e-13)
+
+
+# ---------------------------------------------------------------- Euler ----
+
+def _models():
+  return {
+      'gbm': (('gbm', 0.05, 0.2), oeuler.gbm_fns(0.05, 0.2), [100.0]),
+      'const': (('const', 0.05 - 0.02, 0.2), oeuler.gbm_log_fns(0.05, 0.2),
+                [np.log(100.0)]),
+      'heston_ft': (('heston', 0.03, 2.0, 0.04, 0.3, -0.7, 0.0),
+                    oeuler.heston_ft_fns(2.0, 0.04, 0.3, -0.7, 0.03),
+                    [np.log(100.), 0.04]),
+  }
+
+
+def _heston_ref_fns(kappa, theta, xi, rho):
+  def drift(t, x):
+    return oheston.drift_vol(x, kappa, theta, xi, rho)[0]
+
+  def vol(t, x):
+    return oheston.drift_vol(x, kappa, theta, xi, rho)[1]
+  return drift, vol
+
+
+@pytest.mark.parametrize('name', ['gbm', 'const', 'heston_ft', 'heston_ref'])
+@pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12),
+                                       (torch.float32, 2e-5)])
+def test_euler_injected_draws_parity(name, dtype, tol):
+  print('euler parity', name, dtype, flush=True)
+  npd = np.float64 if dtype == torch.float64 else np.float32
+  if name == 'heston_ref':
+    spec = ('heston', 0.0, 2.0, 0.04, 0.3, -0.7, 1.0)
+    fns, x0 = _heston_ref_fns(2.0, 0.04, 0.3, -0.7), [np.log(100.), 0.04]
+  else:
+    spec, fns, x0 = _models()[name]
+  d = engine.model_dim(spec)
+  times = [0.25, 0.5, 1.0]
+  rs = np.random.RandomState(7)
+  from oracle import grid as ogrid
+  t, mask, _ = ogrid.prepare_grid(times, npd, num_time_steps=50)
+  z = rs.standard_normal((4000, t.shape[0] - 1, d)).astype(npd)
+  want = oeuler.sample(d, fns[0], fns[1], np.asarray(times, npd),
+                       num_samples=4000, initial_state=np.asarray(x0, npd),
+                       normal_draws=z, num_time_steps=50, dtype=npd)
+  drift = engine.tag(lambda t, x: None, spec)
+  vol = engine.tag(lambda t, x: None, spec)
+  got = tqf.models.euler_sampling.sample(
+      d, drift, vol, np.asarray(times, npd), num_time_steps=50,
+      initial_state=np.asarray(x0, npd), normal_draws=z, dtype=dtype)
+  assert got.shape == want.shape
+  for c in range(d):
+    _close(_np(got)[..., c], want[..., c], tol)
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_euler_sobol_stream_parity(dtype):
+  npd = np.float64 if dtype == torch.float64 else np.float32
+  n, steps, skip = 2048, 40, 3
+  spec, fns, x0 = _models()['heston_ft']
+  u = osobol.sample(steps * 2, n, skip=skip, dtype=npd)
+  z = onormal.quasi_normals(u.astype(np.float64)).reshape(n, steps, 2)
+  want = oeuler.sample(2, fns[0], fns[1], np.asarray([1.0], npd),
+                       num_samples=n, initial_state=np.asarray(x0, npd),
+                       normal_draws=z.astype(npd), num_time_steps=steps,
+                       dtype=npd)
+  drift = engine.tag(lambda t, x: None, spec)
+  got = tqf.models.euler_sampling.sample(
+      2, drift, drift, np.asarray([1.0], npd), num_time_steps=steps,
+      num_samples=n, initial_state=np.asarray(x0, npd),
+      random_type=mvn.RandomType.SOBOL, skip=skip, dtype=dtype)
+  # float32: the device erfinv is within 3e-7 relative of scipy's
+  _close(_np(got), want, 1e-11 if dtype == torch.float64 else 5e-5)
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_euler_philox_stream_parity(dtype):
+  npd = np.float64 if dtype == torch.float64 else np.float32
+  n, steps = 2048, 30
+  spec, fns, x0 = _models()['gbm']
+  key = mvn.seed_to_key(42)
+  z = orng.normals(n, steps, key, dtype=npd).reshape(n, steps, 1)
+  want = oeuler.sample(1, fns[0], fns[1], np.asarray([1.0], npd),
+                       num_samples=n, initial_state=np.asarray(x0, npd),
+                       normal_draws=z.astype(npd), num_time_steps=steps,
+                       dtype=npd)
+  drift = engine.tag(lambda t, x: None, spec)
+  got = tqf.models.euler_sampling.sample(
+      1, drift, drift, np.asarray([1.0], npd), num_time_steps=steps,
+      num_samples=n, initial_state=np.asarray(x0, npd),
+      random_type=mvn.RandomType.PSEUDO, seed=42, dtype=dtype)
+  _close(_np(got), want, 1e-11 if dtype == torch.float64 else 1e-4)
+
+
+def test_euler_payoff_mode_matches_recorded_paths():
+  n = 50000
+  spec = ('heston', 0.03, 2.0, 0.04, 0.3, -0.7, 0.0)
+  all_times, keep = pricing.heston_grid_for([0.5, 1.0], 100, 'reference',
+                                            torch.float64)
+  rng = engine.RngSpec('philox', seed=5)
+  x0 = np.array([np.log(100.), 0.04])
+  paths = engine.run(spec, all_times, keep, 2, x0, n, rng, torch.float64,
+                     mode='paths')
+  strikes = np.linspace(80, 120, 70)  # more than one 64-lane chunk
+  res = engine.run(spec, all_times, keep, 2, x0, n, rng, torch.float64,
+                   mode='payoff', strikes=strikes,
+                   discount=np.exp(-0.03 * np.array([0.5, 1.0])))
+  s = np.exp(_np(paths)[:, :, 0])
+  pay = np.maximum(s[:, :, None] - strikes[None, None, :], 0)
+  disc = np.exp(-0.03 * np.array([0.5, 1.0]))[:, None]
+  np.testing.assert_allclose(_np(res['prices']), disc * pay.mean(0),
+                             rtol=1e-12)
+  np.testing.assert_allclose(
+      _np(res['stderrs']), disc * pay.std(0, ddof=1) / np.sqrt(n), rtol=1e-9)
+  # store mode is the transpose of path mode
+  store = engine.run(spec, all_times, keep, 2, x0, n, rng, torch.float64,
+                     mode='store')
+  np.testing.assert_array_equal(_np(store).transpose(2, 0, 1), _np(paths))
+
+
+def test_gbm_euler_prices_within_3se_of_black_scholes():
+  # Plain method: 1-asset GBM Euler, float64, 1e5 paths, 100 steps
+  strikes = np.linspace(80, 120, 8)
+  all_times, keep = pricing.heston_grid_for([1.0], 100, 'reference',
+                                            torch.float64)
+  res = pricing.european_calls(('gbm', 0.05, 0.2), [100.0], strikes,
+                               all_times, keep, 100000,
+                               engine.RngSpec('philox', seed=42),
+                               discount_rate=0.05, dtype=torch.float64)
+  p, se = _np(res['prices'])[0], _np(res['stderrs'])[0]
+  bs = closed_form.bs_price(100, strikes, 1.0, 0.05, 0.2)
+  assert np.all(np.abs(p - bs) <= 3 * se), (p - bs) / se
+
+
+def test_generic_callables_match_fused_kernel():
+  times = [0.5, 1.0]
+  rs = np.random.RandomState(3)
+  z = rs.standard_normal((1000, 20, 2))
+  model = tqf.models.HestonModel(2.0, 0.04, 0.3, -0.7, dtype=np.float64)
+  fd, fv = model.euler_ft_fns(0.03)
+  fused = tqf.models.euler_sampling.sample(
+      2, fd, fv, times, num_time_steps=20, initial_state=[np.log(100), .04],
+      normal_draws=z, dtype=torch.float64)
+  plain_d = lambda t, x: fd(t, x)
+  plain_v = lambda t, x: fv(t, x)
+  generic = tqf.models.euler_sampling.sample(
+      2, plain_d, plain_v, times, num_time_steps=20,
+      initial_state=[np.log(100), .04], normal_draws=z, dtype=torch.float64)
+  _close(_np(generic), _np(fused), 1e-12)
+
+
+# ---------------------------------------------------------------- GBM exact
+
+@pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12),
+                                       (torch.float32, 2e-5)])
+def test_gbm_exact_parity(dtype, tol):
+  npd = np.float64 if dtype == torch.float64 else np.float32
+  rs = np.random.RandomState(11)
+  times = np.array([0.1, 0.4, 1.0, 2.0])
+  z = rs.standard_normal((5000, 4, 1)).astype(npd)
+  p = tqf.models.GeometricBrownianMotion(0.05, 0.3, dtype=dtype)
+  got = p.sample_paths(times, initial_state=100.0, normal_draws=z)
+  want = ogbm.sample_paths(times, 0.05, 0.3, 100.0, z, dtype=npd)
+  _close(_np(got), want, tol)
+  corr = np.array([[1, .3, .1], [.3, 1, .2], [.1, .2, 1]])
+  z = rs.standard_normal((5000, 4, 3)).astype(npd)
+  mp = tqf.models.MultivariateGeometricBrownianMotion(
+      3, means=[0.01, 0.02, 0.03], volatilities=[0.1, 0.2, 0.3],
+      corr_matrix=corr, dtype=dtype)
+  got = mp.sample_paths(times, initial_state=[1., 2., 3.], normal_draws=z)
+  want = ogbm.mv_sample_paths(times, [0.01, 0.02, 0.03], [0.1, 0.2, 0.3],
+                              corr, [1., 2., 3.], z, dtype=npd)
+  _close(_np(got), want, tol)
+
+
+def test_gbm_exact_moments_sobol_and_philox():
+  p = tqf.models.GeometricBrownianMotion(0.05, 0.2, dtype=torch.float64)
+  for rt in (mvn.RandomType.SOBOL, mvn.RandomType.PSEUDO,
+             mvn.RandomType.PSEUDO_ANTITHETIC):
+    s = _np(p.sample_paths([1.0], initial_state=1.0, num_samples=100000,
+                           random_type=rt, seed=3))[:, 0, 0]
+    lg = np.log(s)
+    assert abs(lg.mean() - 0.03) < 3 * lg.std() / np.sqrt(lg.size)
+    assert abs(lg.var() - 0.04) < 3 * 0.04 * np.sqrt(2 / (lg.size - 1))
+
+
+# ---------------------------------------------------------------- Heston QE
+
+@pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12),
+                                       (torch.float32, 2e-5)])
+def test_heston_qe_parity(dtype, tol):
+  npd = np.float64 if dtype == torch.float64 else np.float32
+  rs = np.random.RandomState(13)
+  times = np.array([0.3, 1.0])
+  from oracle import grid as ogrid
+  all_t, _ = ogrid.heston_prepare_grid(times, npd, time_step=0.05)
+  z = rs.standard_normal((4000, all_t.shape[0] - 1, 2)).astype(npd)
+  model = tqf.models.HestonModel(1.2, 0.05, 0.9, -0.6, dtype=dtype)
+  got = _np(model.sample_paths(times, [np.log(100.), 0.03], time_step=0.05,
+                               normal_draws=z))
+  want = oheston.sample_paths_qe(times, np.log(100.), 0.03, 1.2, 0.05, 0.9,
+                                 -0.6, z, time_step=0.05, dtype=npd)
+  _close(got[..., 0], want[..., 0], tol)
+  # the variance has exact zeros on the psi > 1.5 branch: absolute scale
+  _close(got[..., 1], want[..., 1], tol, scale=1e3 * np.max(want[..., 1]))

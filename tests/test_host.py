@@ -1,0 +1,162 @@
+"""CPU tests of the product's host logic and of the C ABI library itself
+(loads, exports every declared symbol, host-only entry points).  No GPU
+compute is called here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT
+from oracle import grid as ogrid
+from oracle import heston as oheston
+from oracle import sobol as osobol
+from tqf import _native
+from tqf.math.random import sobol as tsobol
+from tqf.models import utils
+from tqf.models.heston import qe_step_coefficients
+
+
+def _header_symbols():
+  text = open(os.path.join(ROOT, 'include', 'tqf.h')).read()
+  return sorted(set(re.findall(r'\b(tqf_[a-z0-9_]+)\s*\(', text)))
+
+
+def test_library_exports_every_declared_symbol():
+  lib = _native.lib()
+  syms = _header_symbols()
+  assert len(syms) >= 8
+  for s in syms:
+    assert hasattr(lib, s), s
+  # the ctypes table binds exactly the declared entry points
+  assert sorted(_native.exported_symbols()) == syms
+  assert lib.tqf_version() >= 1
+
+
+def test_error_path_maps_to_value_error():
+  m = np.zeros((2, 32), np.uint32)
+  poly = np.zeros(1, np.int64)
+  with pytest.raises(ValueError, match='Dimension must be greater'):
+    _native.call('tqf_sobol_direction_numbers',
+                 poly.ctypes.data_as(ctypes.c_void_p),
+                 poly.ctypes.data_as(ctypes.c_void_p), ctypes.c_int32(1),
+                 ctypes.c_int32(0), m.ctypes.data_as(ctypes.c_void_p))
+
+
+def test_native_direction_numbers_match_oracle():
+  dim = 1200
+  got = tsobol.direction_numbers(dim)
+  want = osobol.direction_numbers(dim)
+  want = np.array(want, dtype=object)
+  want = np.vectorize(lambda v: v & 0xFFFFFFFF)(want).astype(np.uint64)
+  np.testing.assert_array_equal(got.astype(np.uint64), want)
+
+
+def test_generating_columns_match_qmc_matrices():
+  g = osobol.sobol_generating_matrices(7, 500, 9)
+  v = tsobol.generating_columns(7, 9, g.shape[1])
+  np.testing.assert_array_equal(v[:, :g.shape[1]].astype(np.int64), g)
+
+
+@pytest.mark.parametrize('kw', [
+    dict(times=[1.0], num_time_steps=100),
+    dict(times=[0.25, 0.5, 0.75, 1.0], num_time_steps=365),
+    dict(times=[0.1, 0.5, 1.0], time_step=0.1),
+    dict(times=[0.3, 0.7], time_step=0.05),
+    dict(times=[0.33, 0.5], times_grid=np.linspace(0, 0.5, 11)),
+])
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+def test_prepare_grid_matches_oracle(kw, dtype):
+  kw = dict(kw)
+  times = kw.pop('times')
+  ts = kw.get('time_step')
+  if ts is None and kw.get('num_time_steps'):
+    ts = np.asarray(times, dtype)[-1] / dtype(kw['num_time_steps'])
+  a = utils.prepare_grid(times=times, time_step=ts, dtype=dtype,
+                         num_time_steps=kw.get('num_time_steps'),
+                         times_grid=kw.get('times_grid'))
+  b = ogrid.prepare_grid(times, dtype, time_step=kw.get('time_step'),
+                         num_time_steps=kw.get('num_time_steps'),
+                         times_grid=kw.get('times_grid'))
+  np.testing.assert_array_equal(a[0], b[0])
+  np.testing.assert_array_equal(a[1], b[1])
+  np.testing.assert_array_equal(a[2], b[2])
+
+
+def test_heston_grid_matches_oracle():
+  a = utils.heston_grid([0.5, 1.0], 0.1, None, [], np.float64)
+  b = ogrid.heston_prepare_grid([0.5, 1.0], np.float64, time_step=0.1)
+  np.testing.assert_array_equal(a[0], b[0])
+  np.testing.assert_array_equal(a[1], b[1])
+  a = utils.heston_grid([0.25, 1.0], 0.1, 10, [], np.float64)
+  b = ogrid.heston_prepare_grid([0.25, 1.0], np.float64, num_time_steps=10)
+  np.testing.assert_array_equal(a[0], b[0])
+
+
+def _simulate_records(keep_mask, k, initial_written=None):
+  """Replays the reference loop with state = step counter."""
+  steps = len(keep_mask) - 1
+  out = [None] * k
+  record = k != 1
+  written = int(keep_mask[0]) if initial_written is None else initial_written
+  out[0] = -1  # TensorArray slot 0 / the plain result start as the state
+  i = 0
+  while i < steps and written < k:
+    if record:
+      out[written] = i
+    else:
+      out[0] = i
+    written += int(keep_mask[i + 1])
+    i += 1
+  return out, i
+
+
+@pytest.mark.parametrize('mask,k', [
+    ([False, False, True, False, True], 2),
+    ([True, False, True, True], 3),
+    ([False, True, False, False], 1),
+    ([False, False, False, True, False, True, False], 2),
+    ([True, False, False], 1),
+])
+def test_record_plan_replays_reference_loop(mask, k):
+  slots, init_slot, executed, filled = utils.record_plan(np.array(mask), k)
+  want, n_exec = _simulate_records(mask, k)
+  assert executed == n_exec
+  got = [None] * k
+  if init_slot >= 0:
+    got[init_slot] = -1
+  for i, s in enumerate(slots):
+    if s >= 0:
+      got[s] = i
+  assert got == want
+
+
+def test_qe_step_coefficients_reproduce_oracle_moments():
+  t = np.array([0.0, 0.1, 0.25, 0.25, 0.6])
+  c = qe_step_coefficients(t, 1.5, 0.05, 0.6, -0.4, 1e-6, torch.float64)
+  v = 0.07
+  for i, dt in enumerate(np.diff(t)):
+    e = np.exp(-1.5 * dt)
+    m = 0.05 + (v - 0.05) * e
+    s2 = v * 0.36 * e / 1.5 * (1 - e) + 0.05 * 0.36 / 2 / 1.5 * (1 - e)**2
+    np.testing.assert_allclose(c[i, 8] + (v - c[i, 8]) * c[i, 0], m)
+    np.testing.assert_allclose(v * c[i, 1] + c[i, 2], s2, atol=1e-18)
+    assert c[i, 9] == (1.0 if dt > 1e-6 else 0.0)
+  # log-spot update agrees with the oracle's for a pair of variances
+  z = 0.3
+  for i, dt in enumerate(np.diff(t)):
+    if dt == 0:
+      continue
+    want = oheston.update_log_spot(1.5, 0.05, 0.6, -0.4, v, 0.06, 4.6, dt, z)
+    got = (4.6 + c[i, 3] + c[i, 4] * v + c[i, 5] * 0.06 +
+           np.sqrt(c[i, 6] * v + c[i, 7] * 0.06) * z)
+    np.testing.assert_allclose(got, want, rtol=1e-14)
+
+
+def test_product_fails_loudly_without_gpu():
+  if torch.cuda.is_available():
+    pytest.skip('a GPU is present')
+  with pytest.raises(RuntimeError, match='ROCm GPU'):
+    tsobol.sample(2, 5)

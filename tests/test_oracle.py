@@ -1,0 +1,320 @@
+"""Pins the CPU oracle against the reference's golden vectors and against
+closed forms / independent constructions (no GPU needed)."""
+import numpy as np
+import pytest
+from scipy import special, stats
+
+from conftest import load_golden
+from oracle import closed_form, euler, gbm, grid, heston, lsm, normal, rng, sobol
+
+
+# ---------------------------------------------------------------- Sobol ----
+
+def test_sobol_random_ops_goldens():
+  g = load_golden('sobol_golden.json')
+  e = g['random_ops_dim2_first5']
+  for dt in (np.float16, np.float32, np.float64):
+    got = sobol.sample(e['dim'], e['num_results'], dtype=dt)
+    np.testing.assert_allclose(got, e['expected'], rtol=1e-6)
+  e = g['random_ops_dim5_31_as_set']
+  got = sobol.sample(5, 31)
+  assert sorted(map(tuple, got.tolist())) == sorted(map(tuple, e['expected']))
+  e = g['random_ops_large_skip']
+  np.testing.assert_allclose(sobol.sample(1, 3, skip=e['skip']), e['expected'])
+
+
+def test_sobol_skip_consistency():
+  a = sobol.sample(10, 50 + 17)
+  b = sobol.sample(10, 50, skip=17)
+  np.testing.assert_allclose(a[17:], b)
+
+
+def test_sobol_qmc_goldens():
+  g = load_golden('sobol_golden.json')
+  np.testing.assert_allclose(sobol.sobol_sample(5, 29),
+                             g['qmc_sobol_29x5']['expected'], rtol=1e-6)
+  e = g['qmc_sequence_indices']
+  np.testing.assert_allclose(
+      sobol.sobol_sample(5, 29, sequence_indices=e['indices']), e['expected'])
+  np.testing.assert_allclose(
+      sobol.sobol_sample(6, 8, apply_tent_transform=True),
+      g['qmc_tent_6x8']['expected'])
+  for dt in (np.float32, np.float64):
+    np.testing.assert_allclose(sobol.sobol_sample(6, 8, dtype=dt),
+                               g['qmc_dtype_6x8']['expected'])
+  for key in ('qmc_generating_matrices', 'qmc_generating_matrices_doc'):
+    e = g[key]
+    np.testing.assert_array_equal(
+        sobol.sobol_generating_matrices(e['dim'], e['num_results'],
+                                        e['num_digits']), e['expected'])
+  e = g['digital_net_dtype_5x6']
+  gm = sobol.sobol_generating_matrices(5, 6, 3)
+  np.testing.assert_allclose(sobol.digital_net_sample(gm, 6, 3), e['expected'])
+
+
+def test_sobol_gray_code_cross_check():
+  # An independent Gray-code (Antonov-Saleev) generator visits natural index
+  # g(n) = n ^ (n >> 1) at step n.
+  g = sobol.gray_code_ints(40, 1000, nd=10)
+  nat, nd = sobol.sample_int(40, 1023)
+  assert nd == 10
+  n = np.arange(1, 1001)
+  np.testing.assert_array_equal(g, nat[(n ^ (n >> 1)) - 1])
+
+
+def test_sobol_net_property_and_index_agreement():
+  # every aligned 2^m block of a dimension is a permutation of j / 2^m
+  ints, nd = sobol.sample_int(30, 2**10 - 1)
+  full = np.concatenate([np.zeros((1, 30), np.uint64), ints])  # index 0..1023
+  for m in (4, 7):
+    for b in range(0, 1024, 2**m):
+      blk = full[b:b + 2**m] >> np.uint64(nd - m)
+      for d in range(30):
+        assert np.array_equal(np.sort(blk[:, d]), np.arange(2**m))
+  # random_ops index k equals qmc index k for k >= 1 (same digits)
+  q = sobol.digital_net_sample_int(sobol.sobol_generating_matrices(30, 1024, 10),
+                                   1024, 10)
+  np.testing.assert_array_equal(q[1:], ints.astype(np.int64))
+
+
+# ---------------------------------------------------------------- Philox ---
+
+def test_philox_known_answers():
+  kat = load_golden('philox_kat.json')
+  for v in kat['vectors']:
+    c = np.array([int(x, 16) for x in v['counter']], np.uint64)
+    k = np.array([int(x, 16) for x in v['key']], np.uint64)
+    got = rng.philox4x32_10(c, k)
+    assert [int(x) for x in got] == [int(x, 16) for x in v['expected']]
+
+
+def test_box_muller_distribution():
+  z = rng.normals(100000, 4, seed=42)
+  assert abs(z.mean()) < 4 / np.sqrt(z.size)
+  assert abs(z.var() - 1) < 4 * np.sqrt(2 / z.size)
+  assert stats.kstest(z.ravel(), 'norm').pvalue > 1e-3
+  # columns independent
+  c = np.corrcoef(z.T)
+  assert np.max(np.abs(c - np.eye(4))) < 0.02
+  # closed form on one counter
+  w = rng.raw_words(1, 2, seed=5)
+  u = (float(w[0, 0]) + 1) * 2.0**-32
+  t = float(w[0, 1]) * 2.0**-32
+  want = np.sqrt(-2 * np.log(u)) * np.array([np.cos(2 * np.pi * t),
+                                             np.sin(2 * np.pi * t)])
+  np.testing.assert_allclose(rng.normals(1, 2, seed=5)[0], want, rtol=1e-14)
+  anti = rng.normals(8, 3, seed=1, antithetic=True)
+  np.testing.assert_array_equal(anti[:4], -anti[4:])
+
+
+def test_normal_transforms():
+  u = np.array([0.1, 0.25, 0.5, 0.75, 0.9])
+  np.testing.assert_allclose(normal.quasi_normals(u), special.ndtri(u),
+                             rtol=1e-13, atol=1e-15)
+  ints = np.array([0, 2**31, 2**32 - 1, 12345678], np.uint64)
+  z = normal.shifted_sobol_normals(ints)
+  m = (ints >> np.uint64(8)).astype(np.float64)
+  np.testing.assert_allclose(z, special.ndtri((m + 0.5) / 2**24), rtol=1e-12)
+
+
+# ---------------------------------------------------------------- grids ----
+
+def test_grid_from_num_times():
+  t, mask, idx = grid.prepare_grid([1.0], np.float64, num_time_steps=100)
+  assert t.shape == (101,) and t[0] == 0 and t[-1] == 1.0
+  np.testing.assert_allclose(np.diff(t), 0.01, rtol=1e-12)
+  assert mask.sum() == 1 and mask[-1]
+  t, mask, idx = grid.prepare_grid([0.25, 0.5, 0.75, 1.0], np.float64,
+                                   num_time_steps=365)
+  assert t.shape == (366,)
+  np.testing.assert_allclose(t[idx], [0.25, 0.5, 0.75, 1.0])
+  assert np.all(np.diff(t) > 0)
+
+
+def test_grid_from_time_step_dedup():
+  t, mask, idx = grid.prepare_grid([0.1, 0.5, 1.0], np.float64, time_step=0.1)
+  assert np.all(np.diff(t) > 1e-10)
+  np.testing.assert_allclose(t[idx], [0.1, 0.5, 1.0], atol=1e-12)
+  assert mask.sum() == 3
+
+
+# ---------------------------------------------------------------- Euler ----
+
+def test_euler_gbm_product_form():
+  rs = np.random.RandomState(0)
+  n, steps = 64, 50
+  z = rs.standard_normal((n, steps, 1))
+  drift, vol = euler.gbm_fns(0.05, 0.2)
+  out = euler.sample(1, drift, vol, [1.0], num_samples=n, initial_state=[100.],
+                     normal_draws=z, num_time_steps=steps)
+  t, _, _ = grid.prepare_grid([1.0], np.float64, num_time_steps=steps)
+  dt = np.diff(t)
+  prod = 100 * np.prod(1 + 0.05 * dt + 0.2 * np.sqrt(dt) * z[:, :, 0], axis=1)
+  np.testing.assert_allclose(out[:, 0, 0], prod, rtol=1e-12)
+
+
+def test_euler_wiener_is_cumsum_and_records():
+  rs = np.random.RandomState(1)
+  times = [0.1, 0.35, 1.0]
+  drift = lambda t, x: np.zeros_like(x)
+  vol = lambda t, x: np.ones(x.shape + (1,))
+  t, mask, idx = grid.prepare_grid(times, np.float64, num_time_steps=20)
+  z = rs.standard_normal((5, t.shape[0] - 1, 1))
+  out = euler.sample(1, drift, vol, times, num_samples=5, normal_draws=z,
+                     num_time_steps=20)
+  w = np.concatenate([np.zeros((5, 1)),
+                      np.cumsum(z[:, :, 0] * np.sqrt(np.diff(t)), 1)], 1)
+  np.testing.assert_allclose(out[:, :, 0], w[:, idx], rtol=1e-12, atol=1e-15)
+
+
+# ---------------------------------------------------------------- Heston ---
+
+def test_heston_coefficient_goldens():
+  g = load_golden('heston_coefficients.json')
+  d, v = heston.drift_vol(np.array([g['state']]), g['kappa'], g['theta'],
+                          g['volvol'], g['rho'])
+  np.testing.assert_allclose(d[0], g['drift'], rtol=1e-6)
+  np.testing.assert_allclose(v[0], g['vol'], rtol=1e-6, atol=1e-8)
+  e = g['expected_total_variance']
+  got = heston.expected_total_variance(e['T'], e['v0'], e['kappa'], e['theta'])
+  want = ((e['v0'] - e['theta']) * (1 - np.exp(-e['kappa'] * e['T'])) /
+          e['kappa'] + e['theta'] * e['T'])
+  np.testing.assert_allclose(got, want)
+
+
+def test_heston_semi_analytic_reduces_to_black_scholes():
+  # the price moves linearly in xi (the rho*xi term) as xi -> 0
+  bs = closed_form.bs_price(100, 90, 1.0, 0.03, 0.2)
+  e3 = heston.european_call(100, 90, 1.0, 0.03, 0.04, 2.0, 0.04, 1e-3, -0.7)
+  e4 = heston.european_call(100, 90, 1.0, 0.03, 0.04, 2.0, 0.04, 1e-4, -0.7)
+  e5 = heston.european_call(100, 90, 1.0, 0.03, 0.04, 2.0, 0.04, 1e-5, -0.7)
+  np.testing.assert_allclose(e5, bs, rtol=3e-6)
+  np.testing.assert_allclose((e3 - bs) / (e4 - bs), 10.0, rtol=0.02)
+  # put-call parity holds for the integral with stochastic vol
+  c = heston.european_call(100, 110, 0.5, 0.03, 0.04, 2.0, 0.05, 0.5, -0.7)
+  assert 0 < c < 100
+
+
+def test_heston_qe_matches_conditional_mean():
+  rs = np.random.RandomState(2)
+  n = 200000
+  z = rs.standard_normal((n, 1, 2))
+  out = heston.sample_paths_qe([0.5], np.log(100.), 0.04, 1.5, 0.05, 0.6,
+                               -0.5, z, num_time_steps=1)
+  # E[v_T | v_0] = theta + (v0 - theta) e^{-kappa T}
+  want = 0.05 + (0.04 - 0.05) * np.exp(-1.5 * 0.5)
+  v = out[:, -1, 1]
+  assert abs(v.mean() - want) < 4 * v.std() / np.sqrt(n)
+
+
+def test_heston_qe_small_volvol_is_gbm():
+  rs = np.random.RandomState(3)
+  n = 100000
+  z = rs.standard_normal((n, 10, 2))
+  out = heston.sample_paths_qe([1.0], np.log(100.), 0.04, 1.0, 0.04, 1e-3,
+                               0.0, z, num_time_steps=10)
+  s = np.exp(out[:, -1, 0])
+  assert abs(s.mean() - 100.) < 4 * s.std() / np.sqrt(n)
+  v = out[:, -1, 1]
+  np.testing.assert_allclose(v, 0.04, rtol=2e-2)
+  assert abs(v.mean() - 0.04) < 4 * v.std() / np.sqrt(n)
+
+
+# ---------------------------------------------------------------- GBM ------
+
+def test_gbm_exact_single_step_and_moments():
+  rs = np.random.RandomState(4)
+  z = rs.standard_normal((100000, 2, 1))
+  out = gbm.sample_paths([0.5, 1.0], 0.05, 0.2, 100.0, z)
+  np.testing.assert_allclose(
+      out[:, 0, 0], 100 * np.exp((0.05 - 0.02) * 0.5 + 0.2 * np.sqrt(.5) *
+                                 z[:, 0, 0]), rtol=1e-13)
+  lg = np.log(out[:, 1, 0] / 100)
+  assert abs(lg.mean() - 0.03) < 3 * lg.std() / np.sqrt(lg.size)
+  assert abs(lg.var() - 0.04) < 3 * 0.04 * np.sqrt(2 / (lg.size - 1))
+
+
+def test_mv_gbm_correlation():
+  rs = np.random.RandomState(5)
+  corr = np.array([[1, .5], [.5, 1]])
+  z = rs.standard_normal((100000, 1, 2))
+  out = gbm.mv_sample_paths([1.0], [0.0, 0.0], [0.2, 0.3], corr, [1., 1.], z)
+  lg = np.log(out[:, 0])
+  assert abs(np.corrcoef(lg.T)[0, 1] - 0.5) < 0.01
+
+
+# ---------------------------------------------------------------- LSM ------
+
+def test_lsm_longstaff_schwartz_table():
+  g = load_golden('lsm_ls2001.json')
+  s = np.array(g['samples'])[..., None]
+  df = np.exp(-np.cumsum(g['interest_rates']))
+  tol = g['tolerance']
+  basis = lsm.make_polynomial_basis(2)
+  put = lsm.make_basket_put_payoff([1.1])
+  for dt in (np.float32, np.float64):
+    np.testing.assert_allclose(
+        lsm.least_square_mc(s, [3], put, basis, discount_factors=[df[-1]],
+                            dtype=dt), [g['european_put']], atol=tol, rtol=tol)
+    np.testing.assert_allclose(
+        lsm.least_square_mc(s, [1, 2, 3], put, basis, discount_factors=df,
+                            dtype=dt), [g['american_put']], atol=tol, rtol=tol)
+    np.testing.assert_allclose(
+        lsm.least_square_mc(s, [1, 2, 3], put, basis, discount_factors=df,
+                            num_calibration_samples=4, dtype=dt),
+        [g['american_put_calib4']], atol=tol, rtol=tol)
+  put2 = lsm.make_basket_put_payoff([1.1, 1.2])
+  df2 = np.exp(-np.cumsum(g['batch_rates'], -1))[None]
+  np.testing.assert_allclose(
+      lsm.least_square_mc(s, [1, 2, 3], put2, basis, discount_factors=df2),
+      g['batch_payoff'], atol=tol, rtol=tol)
+  np.testing.assert_allclose(
+      lsm.least_square_mc(np.stack([s, s + 0.1]), [1, 2, 3], put2, basis,
+                          discount_factors=df2),
+      g['batch_samples'], atol=tol, rtol=tol)
+
+
+def test_lsm_duplicate_asset_invariant_and_single_date():
+  g = load_golden('lsm_ls2001.json')
+  s = np.array(g['samples'])[..., None]
+  df = np.exp(-np.cumsum(g['interest_rates']))
+  basis = lsm.make_polynomial_basis(10)
+  put = lsm.make_basket_put_payoff([1.1, 1.2, 1.3])
+  a = lsm.least_square_mc(np.concatenate([s, s], -1), [1, 2, 3], put, basis,
+                          discount_factors=df)
+  b = lsm.least_square_mc(s, [1, 2, 3], put, basis, discount_factors=df)
+  np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-4)
+  # one exercise date = discounted European mean
+  e = lsm.least_square_mc(s, [3], lsm.make_basket_put_payoff([1.1]),
+                          lsm.make_polynomial_basis(2),
+                          discount_factors=[df[-1]])
+  np.testing.assert_allclose(
+      e, df[-1] * np.maximum(1.1 - s[:, 3, 0], 0).mean(), rtol=1e-12)
+
+
+def test_polynomial_basis_order():
+  x = np.array([[1.0], [2.0], [3.0], [4.0]])[..., None]  # [4, 1, 1]
+  b = lsm.make_polynomial_basis(2)(x, 0)
+  np.testing.assert_allclose(b[0], [[1, 1, 1, 1], [-1.5, -.5, .5, 1.5],
+                                    [2.25, .25, .25, 2.25]])
+
+
+# ---------------------------------------------------------------- closed ---
+
+def test_black_scholes_parity_and_greeks():
+  c = closed_form.bs_price(100, 95, 0.7, 0.03, 0.25)
+  p = closed_form.bs_price(100, 95, 0.7, 0.03, 0.25, is_call=False)
+  np.testing.assert_allclose(c - p, 100 - 95 * np.exp(-0.03 * 0.7))
+  h = 1e-5
+  fd = (closed_form.bs_price(100 + h, 95, .7, .03, .25) -
+        closed_form.bs_price(100 - h, 95, .7, .03, .25)) / (2 * h)
+  np.testing.assert_allclose(closed_form.bs_delta(100, 95, .7, .03, .25), fd,
+                             rtol=1e-7)
+  fd = (closed_form.bs_price(100, 95, .7, .03, .25 + h) -
+        closed_form.bs_price(100, 95, .7, .03, .25 - h)) / (2 * h)
+  np.testing.assert_allclose(closed_form.bs_vega(100, 95, .7, .03, .25), fd,
+                             rtol=1e-7)
+  fd = (closed_form.bs_price(100, 95, .7, .03 + h, .25) -
+        closed_form.bs_price(100, 95, .7, .03 - h, .25)) / (2 * h)
+  np.testing.assert_allclose(closed_form.bs_rho(100, 95, .7, .03, .25), fd,
+                             rtol=1e-6)

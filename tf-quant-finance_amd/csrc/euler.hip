@@ -1,0 +1,451 @@
+// Fused Euler-Maruyama steppers.
+//
+// One thread owns one path for the whole time loop: the state lives in
+// VGPRs, normals are produced in registers (Philox/Box-Muller, Sobol + an
+// in-register inverse normal CDF, or read from injected draws), and the
+// requested times are either recorded or priced on the spot.  This is the
+// reference's `_while_loop` / `_euler_step` (tff/models/euler_sampling.py:
+// 405-464, 513-537) with the same update order
+//     t = times[i+1];  dw = z_i * sqrt_dt[i];
+//     x' = (x + dt[i] * a(t, x)) + b(t, x) . dw
+// and the same record rule (write at `written_count`, advance on
+// keep_mask[i+1]; the host turns that rule into `slot[i]`).
+//
+// Sobol points: the sequence index of a path splits into a low byte
+// (thread-dependent, looked up in a [dim][256] table that stays in L2) and
+// high bits that at most two values share across a 256-thread block; the
+// high-bit XOR for the block is built in LDS once per chunk of steps.
+//
+// Payoffs (TQF_OUT_PAYOFF): at a recorded time every lane of a wave holds
+// one spot.  Lane s takes strike s and walks the 64 spots of the wave with
+// v_readlane (scalar operands), so the wave produces the 64 strike sums of
+// its 64 paths without any cross-lane reduction tree.  Per-wave sums go to
+// LDS, the block combines its 4 waves in float64 and one finalize kernel
+// sums the blocks in a fixed order (deterministic).
+#include "common.h"
+#include "normals.h"
+
+using namespace tqf;
+
+namespace {
+
+constexpr int kBlock = kPathBlock;
+constexpr int kWaves = kBlock / kWave;
+constexpr int kMaxLvLds = 8192;   // table entries staged in LDS
+
+template <typename R>
+struct EulerArgs {
+  // model
+  R p[8];
+  const R* lv_vol;
+  const R* lv_t;
+  const R* coef;  // HESTON_QE per-step constants [n_steps][12]
+  R lv_x0, lv_inv_dx;
+  int lv_nt, lv_nx, lv_in_lds;
+  // grid
+  const R* dt;
+  const R* sqrt_dt;
+  const R* t_next;
+  const int* slot;
+  int n_steps, n_slots, init_slot;
+  RngArgs<R> rng;
+  // output
+  int mode, n_strikes;
+  R* paths;
+  const double* strikes;
+  double* partials;
+  int64_t n_paths;
+  const R* x0;
+  int64_t x0_stride;
+  int lds_pay_off;  // byte offset of the payoff area in dynamic LDS
+};
+
+template <int MODEL>
+struct ModelDim {
+  static constexpr int D =
+      (MODEL == TQF_MODEL_HESTON_FT || MODEL == TQF_MODEL_HESTON_QE) ? 2 : 1;
+};
+
+// ---- local volatility lookup -----------------------------------------------
+template <typename R>
+__device__ __forceinline__ R lv_sigma(const EulerArgs<R>& a, const R* tab,
+                                      int row, R wt, R x) {
+  R fx = (x - a.lv_x0) * a.lv_inv_dx;
+  fx = fx < (R)0 ? (R)0 : fx;
+  const R fmax = (R)(a.lv_nx - 1);
+  fx = fx > fmax ? fmax : fx;
+  int j = (int)fx;
+  j = j > a.lv_nx - 2 ? a.lv_nx - 2 : j;
+  const R wx = fx - (R)j;
+  const R* r0 = tab + (int64_t)row * a.lv_nx;
+  const R* r1 = row + 1 < a.lv_nt ? r0 + a.lv_nx : r0;
+  const R s0 = r0[j] + wx * (r0[j + 1] - r0[j]);
+  const R s1 = r1[j] + wx * (r1[j + 1] - r1[j]);
+  return s0 + wt * (s1 - s0);
+}
+
+// ---- per-step update -------------------------------------------------------
+template <typename R, int MODEL>
+struct Stepper {
+  static constexpr int D = ModelDim<MODEL>::D;
+  __device__ __forceinline__ static void step(const EulerArgs<R>& a, R* x,
+                                              const R* z, R dt, R sdt,
+                                              const R* tab, int lv_row,
+                                              R lv_w, int i) {
+    if (MODEL == TQF_MODEL_GBM) {
+      const R drift = a.p[0] * x[0];
+      const R vol = a.p[1] * x[0];
+      x[0] = (x[0] + dt * drift) + vol * (z[0] * sdt);
+    } else if (MODEL == TQF_MODEL_CONST) {
+      x[0] = (x[0] + dt * a.p[0]) + a.p[1] * (z[0] * sdt);
+    } else if (MODEL == TQF_MODEL_HESTON_FT) {
+      // p: r, kappa, theta, xi*rho, xi*sqrt(1-rho^2), variant.
+      // variant 0: full truncation, v+ = max(v, 0) in drift and vol.
+      // variant 1: HestonModel's own callables (heston_model.py:143-173):
+      //            raw v in the drift, sqrt(|v|) in the vol.
+      const bool ft = a.p[5] == (R)0;
+      const R vpos = x[1] > (R)0 ? x[1] : (R)0;
+      const R vp = ft ? vpos : x[1];
+      const R sq = fast_sqrt(ft ? vpos : (x[1] < (R)0 ? -x[1] : x[1]));
+      const R dw0 = z[0] * sdt, dw1 = z[1] * sdt;
+      const R x0n = (x[0] + dt * (a.p[0] - (R)0.5 * vp)) + sq * dw0;
+      const R x1n = (x[1] + dt * (a.p[1] * (a.p[2] - vp))) +
+                    ((a.p[3] * sq) * dw0 + (a.p[4] * sq) * dw1);
+      x[0] = x0n;
+      x[1] = x1n;
+    } else if (MODEL == TQF_MODEL_HESTON_QE) {
+      // Andersen QE (heston_model.py:522-572), psi_c = 1.5, gamma = 1/2.
+      const R* c = a.coef + (int64_t)i * 12;
+      if (c[9] != (R)0) {
+        const R v = x[1];
+        const R m = c[8] + (v - c[8]) * c[0];
+        const R s2 = v * c[1] + c[2];
+        const R psi = s2 / (m * m);
+        R vn;
+        if (psi < (R)1.5) {
+          const R pinv = (R)2 / psi;
+          const R b2 = pinv - (R)1 + fast_sqrt(pinv * (pinv - (R)1));
+          const R am = m / ((R)1 + b2);
+          const R t = fast_sqrt(b2) + z[0];
+          vn = am * t * t;
+        } else {
+          const R p = (psi - (R)1) / (psi + (R)1);
+          const R beta = ((R)1 - p) / m;
+          const R u = (R)0.5 * ((R)1 + erf(z[0] * (R)0.70710678118654752));
+          vn = u > p ? (log((R)1 - p) - log((R)1 - u)) / beta : (R)0;
+        }
+        x[0] = x[0] + c[3] + c[4] * v + c[5] * vn +
+               fast_sqrt(c[6] * v + c[7] * vn) * z[1];
+        x[1] = vn;
+      }
+    } else {  // LOCAL_VOL: drift (r - q) - sigma^2/2, vol sigma(t, e^x)
+      const R s = lv_sigma(a, tab, lv_row, lv_w, x[0]);
+      x[0] = (x[0] + dt * (a.p[0] - (R)0.5 * s * s)) + s * (z[0] * sdt);
+    }
+  }
+};
+
+// Payoff of one recorded slot for the calling wave (see file comment).
+template <typename R, int MODEL>
+__device__ void wave_payoff(const EulerArgs<R>& a, R* lds_pay, int slot,
+                            R spot, int n_active) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int c = 0; c < a.n_strikes; c += kWave) {
+    const int ks = c + lane;
+    const R k = ks < a.n_strikes ? (R)a.strikes[ks] : (R)0;
+    R s1 = 0, s2 = 0;
+    for (int j = 0; j < n_active; ++j) {
+      const R sj = read_lane(spot, j);
+      R pay = sj - k;
+      pay = pay > (R)0 ? pay : (R)0;
+      s1 += pay;
+      s2 += pay * pay;
+    }
+    if (ks < a.n_strikes) {
+      R* o = lds_pay + ((int64_t)(wave * a.n_slots + slot) * a.n_strikes + ks) * 2;
+      o[0] = s1;
+      o[1] = s2;
+    }
+  }
+}
+
+template <typename R, int MODEL>
+__device__ __forceinline__ void record(const EulerArgs<R>& a, R* lds_pay,
+                                       int slot, const R* x, int64_t path,
+                                       bool active, int n_active) {
+  constexpr int D = ModelDim<MODEL>::D;
+  if (a.mode == TQF_OUT_PAYOFF) {
+    const R spot = MODEL == TQF_MODEL_GBM ? x[0] : fast_exp(x[0]);
+    wave_payoff<R, MODEL>(a, lds_pay, slot, spot, n_active);
+  } else if (active) {
+    if (a.mode == TQF_OUT_PATH_MAJOR) {
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        a.paths[(path * a.n_slots + slot) * D + c] = x[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        a.paths[((int64_t)slot * D + c) * a.n_paths + path] = x[c];
+    }
+  }
+}
+
+template <typename R, int MODEL, int RNG>
+__global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
+  constexpr int D = ModelDim<MODEL>::D;
+  constexpr bool kSobol = RNG == TQF_RNG_SOBOL || RNG == TQF_RNG_SOBOL_SHIFTED;
+  extern __shared__ __align__(16) unsigned char lds[];
+  uint32_t* hx = (uint32_t*)lds;
+  R* lds_pay = (R*)(lds + a.lds_pay_off);
+
+  const int64_t path = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool active = path < a.n_paths;
+  const int64_t pc = active ? path : a.n_paths - 1;
+  const int64_t wave_base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63);
+  const int64_t rem = a.n_paths - wave_base;
+  const int n_active = rem >= kWave ? kWave : (rem > 0 ? (int)rem : 0);
+
+  // Local-vol table: stage in LDS when small.
+  const R* tab = a.lv_vol;
+  if (MODEL == TQF_MODEL_LOCAL_VOL && a.lv_in_lds) {
+    R* t = (R*)(lds + (kSobol ? sobol_lds_words(D) * 4 : 0));
+    for (int e = threadIdx.x; e < a.lv_nt * a.lv_nx; e += kBlock) t[e] = a.lv_vol[e];
+    tab = t;
+    __syncthreads();
+  }
+
+  R x[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) x[c] = a.x0[pc * a.x0_stride + c];
+
+  Normals<R, RNG> nrm;
+  nrm.init(a.rng, pc, D);
+
+  if (a.init_slot >= 0) record<R, MODEL>(a, lds_pay, a.init_slot, x, path, active, n_active);
+
+  int lv_row = 0;
+  const int n_dims = a.n_steps * D;
+  for (int i = 0; i < a.n_steps; ++i) {
+    nrm.begin_step(a.rng, hx, i, n_dims);
+    const R dt = a.dt[i];
+    const R sdt = a.sqrt_dt[i];
+    R z[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) z[k] = nrm.get(a.rng, hx, i, k);
+    R lv_w = 0;
+    if (MODEL == TQF_MODEL_LOCAL_VOL) {
+      const R t = a.t_next[i];
+      while (lv_row + 1 < a.lv_nt && a.lv_t[lv_row + 1] <= t) ++lv_row;
+      if (lv_row + 1 < a.lv_nt) {
+        const R t0 = a.lv_t[lv_row], t1 = a.lv_t[lv_row + 1];
+        lv_w = t <= t0 ? (R)0 : (t - t0) / (t1 - t0);
+      }
+    }
+    Stepper<R, MODEL>::step(a, x, z, dt, sdt, tab, lv_row, lv_w, i);
+    const int s = a.slot[i];
+    if (s >= 0) record<R, MODEL>(a, lds_pay, s, x, path, active, n_active);
+  }
+
+  if (a.mode == TQF_OUT_PAYOFF) {
+    __syncthreads();
+    const int n_out = a.n_slots * a.n_strikes;
+    for (int o = threadIdx.x; o < 2 * n_out; o += kBlock) {
+      double acc = 0;
+      for (int w = 0; w < kWaves; ++w) {
+        // waves with no active path never wrote; they hold zeros
+        const int64_t wb = (int64_t)blockIdx.x * kBlock + w * kWave;
+        if (wb < a.n_paths) acc += (double)lds_pay[(int64_t)w * 2 * n_out + o];
+      }
+      a.partials[(int64_t)blockIdx.x * 2 * n_out + o] = acc;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+payoff_finalize_kernel(const double* __restrict__ partials, int64_t n_blocks,
+                       int n_out, int n_strikes, int64_t n_paths,
+                       const double* __restrict__ discount,
+                       double* __restrict__ prices,
+                       double* __restrict__ stderrs, double* __restrict__ sums) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n_out) return;
+  double s1 = 0, s2 = 0;
+  for (int64_t b = 0; b < n_blocks; ++b) {
+    s1 += partials[(b * n_out + o) * 2];
+    s2 += partials[(b * n_out + o) * 2 + 1];
+  }
+  const double n = (double)n_paths;
+  const double mean = s1 / n;
+  double var = n > 1 ? (s2 - n * mean * mean) / (n - 1) : 0.0;
+  var = var > 0 ? var : 0.0;
+  const double df = discount ? discount[o / n_strikes] : 1.0;
+  if (prices) prices[o] = df * mean;
+  if (stderrs) stderrs[o] = df * sqrt(var / n);
+  if (sums) {
+    sums[2 * o] = s1;
+    sums[2 * o + 1] = s2;
+  }
+}
+
+template <typename R>
+int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
+                 const tqf_output* out, int64_t n_paths, const void* x0,
+                 int64_t x0_stride, hipStream_t stream) {
+  EulerArgs<R> a{};
+  for (int i = 0; i < 8; ++i) a.p[i] = (R)m->p[i];
+  const int D =
+      (m->kind == TQF_MODEL_HESTON_FT || m->kind == TQF_MODEL_HESTON_QE) ? 2 : 1;
+  if (m->kind == TQF_MODEL_HESTON_FT) {
+    const double xi = m->p[3], rho = m->p[4];
+    a.p[3] = (R)(xi * rho);
+    a.p[4] = (R)(xi * std::sqrt(1.0 - rho * rho));
+  }
+  a.lv_vol = (const R*)m->lv_vol;
+  a.coef = (const R*)m->step_coef;
+  a.lv_t = (const R*)m->lv_t;
+  a.lv_x0 = (R)m->lv_x0;
+  a.lv_inv_dx = m->lv_dx > 0 ? (R)(1.0 / m->lv_dx) : (R)0;
+  a.lv_nt = m->lv_nt;
+  a.lv_nx = m->lv_nx;
+  a.dt = (const R*)g->dt;
+  a.sqrt_dt = (const R*)g->sqrt_dt;
+  a.t_next = (const R*)g->t_next;
+  a.slot = g->slot;
+  a.n_steps = g->n_steps;
+  a.n_slots = g->n_slots;
+  a.init_slot = g->init_slot;
+  fill_rng_args(a.rng, r, n_paths);
+  a.mode = out->mode;
+  a.n_strikes = out->n_strikes;
+  a.paths = (R*)out->paths;
+  a.strikes = out->strikes;
+  a.partials = out->workspace;
+  a.n_paths = n_paths;
+  a.x0 = (const R*)x0;
+  a.x0_stride = x0_stride;
+
+  const bool sobol = r->kind == TQF_RNG_SOBOL || r->kind == TQF_RNG_SOBOL_SHIFTED;
+  size_t lds = sobol ? (size_t)sobol_lds_words(D) * 4 : 0;
+  if (m->kind == TQF_MODEL_LOCAL_VOL) {
+    const int64_t ne = (int64_t)m->lv_nt * m->lv_nx;
+    a.lv_in_lds = ne <= kMaxLvLds;
+    if (a.lv_in_lds) lds += (size_t)ne * sizeof(R);
+  }
+  lds = (lds + 15) & ~(size_t)15;
+  a.lds_pay_off = (int)lds;
+  if (out->mode == TQF_OUT_PAYOFF)
+    lds += (size_t)kWaves * g->n_slots * out->n_strikes * 2 * sizeof(R);
+  if (lds == 0) lds = 16;
+
+  const unsigned n_blocks = (unsigned)((n_paths + kBlock - 1) / kBlock);
+  void (*kern)(EulerArgs<R>) = nullptr;
+#define TQF_PICK(MODEL, RNG) \
+  if (m->kind == MODEL && r->kind == RNG) kern = euler_kernel<R, MODEL, RNG>;
+#define TQF_PICK_RNG(MODEL)                       \
+  TQF_PICK(MODEL, TQF_RNG_DRAWS)                  \
+  TQF_PICK(MODEL, TQF_RNG_PHILOX)                 \
+  TQF_PICK(MODEL, TQF_RNG_PHILOX_ANTITHETIC)      \
+  TQF_PICK(MODEL, TQF_RNG_SOBOL)                  \
+  TQF_PICK(MODEL, TQF_RNG_SOBOL_SHIFTED)
+  TQF_PICK_RNG(TQF_MODEL_GBM)
+  TQF_PICK_RNG(TQF_MODEL_CONST)
+  TQF_PICK_RNG(TQF_MODEL_HESTON_FT)
+  TQF_PICK_RNG(TQF_MODEL_LOCAL_VOL)
+  TQF_PICK_RNG(TQF_MODEL_HESTON_QE)
+#undef TQF_PICK_RNG
+#undef TQF_PICK
+  if (!kern) return set_error(TQF_ENOTIMPL, "unsupported model/rng combination");
+  hipLaunchKernelGGL(kern, dim3(n_blocks), dim3(kBlock), lds, stream, a);
+  int rc = check_launch("tqf_euler");
+  if (rc) return rc;
+  if (out->mode == TQF_OUT_PAYOFF) {
+    const int n_out = g->n_slots * out->n_strikes;
+    hipLaunchKernelGGL(payoff_finalize_kernel, dim3((n_out + 255) / 256),
+                       dim3(256), 0, stream, out->workspace, (int64_t)n_blocks,
+                       n_out, out->n_strikes, n_paths, out->discount,
+                       out->prices, out->stderrs, out->sums);
+    rc = check_launch("tqf_euler finalize");
+  }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int64_t tqf_euler_workspace_size(int64_t n_paths, int32_t n_slots,
+                                            int32_t n_strikes) {
+  const int64_t n_blocks = (n_paths + kBlock - 1) / kBlock;
+  return n_blocks * (int64_t)n_slots * n_strikes * 2 * (int64_t)sizeof(double);
+}
+
+extern "C" int tqf_euler(const tqf_model* m, const tqf_grid* g,
+                         const tqf_rng* r, const tqf_output* out,
+                         int64_t n_paths, const void* x0,
+                         int64_t x0_stride_path, int32_t dtype, void* stream) {
+  clear_error();
+  TQF_REQUIRE(m && g && r && out, "null descriptor");
+  TQF_REQUIRE(dtype == TQF_F32 || dtype == TQF_F64, "bad dtype %d", dtype);
+  TQF_REQUIRE(n_paths >= 1, "num_samples must be positive, got %lld",
+              (long long)n_paths);
+  TQF_REQUIRE(m->kind >= TQF_MODEL_GBM && m->kind <= TQF_MODEL_HESTON_QE,
+              "unknown model kind %d", m->kind);
+  const int D =
+      (m->kind == TQF_MODEL_HESTON_FT || m->kind == TQF_MODEL_HESTON_QE) ? 2 : 1;
+  if (m->kind == TQF_MODEL_HESTON_QE) {
+    TQF_REQUIRE(m->step_coef != nullptr || g->n_steps == 0,
+                "Heston QE needs per-step constants");
+  }
+  TQF_REQUIRE(m->dim == D, "model dim %d does not match kind (%d)", m->dim, D);
+  TQF_REQUIRE(g->n_steps >= 0 && g->n_slots >= 1, "bad grid sizes");
+  TQF_REQUIRE(g->n_rng_steps >= g->n_steps, "n_rng_steps < n_steps");
+  TQF_REQUIRE(g->init_slot >= -1 && g->init_slot < g->n_slots, "bad init_slot");
+  TQF_REQUIRE(g->n_steps == 0 || (g->dt && g->sqrt_dt && g->slot),
+              "null grid arrays");
+  TQF_REQUIRE(x0 != nullptr, "null initial state");
+  TQF_REQUIRE(x0_stride_path == 0 || x0_stride_path == D,
+              "x0_stride_path must be 0 or dim");
+  TQF_REQUIRE(r->kind >= TQF_RNG_DRAWS && r->kind <= TQF_RNG_SOBOL_SHIFTED,
+              "unknown rng kind %d", r->kind);
+  if (r->kind == TQF_RNG_DRAWS) {
+    TQF_REQUIRE(r->draws != nullptr, "null normal_draws");
+  }
+  if (r->kind == TQF_RNG_PHILOX_ANTITHETIC) {
+    TQF_REQUIRE(n_paths % 2 == 0,
+                "First dimension of `sample_shape` should be even for "
+                "PSEUDO_ANTITHETIC random type");
+  }
+  if (r->kind == TQF_RNG_SOBOL || r->kind == TQF_RNG_SOBOL_SHIFTED) {
+    TQF_REQUIRE(r->sobol_v && r->sobol_lo, "null Sobol tables");
+    TQF_REQUIRE((int64_t)r->sobol_ndims >= (int64_t)g->n_steps * D,
+                "Sobol tables hold %d dimensions, the grid needs %lld",
+                r->sobol_ndims, (long long)g->n_steps * D);
+    TQF_REQUIRE(r->sobol_first_index >= 0 &&
+                    r->sobol_first_index + n_paths <= (int64_t)0xffffffffll + 1,
+                "Sobol index range exceeds 32 bits");
+    TQF_REQUIRE(r->sobol_nd >= 1 && r->sobol_nd <= 32, "bad Sobol digits");
+    TQF_REQUIRE(r->kind == TQF_RNG_SOBOL || r->sobol_nd == 32,
+                "the shifted transform needs nd = 32");
+  }
+  if (m->kind == TQF_MODEL_LOCAL_VOL) {
+    TQF_REQUIRE(m->lv_vol && m->lv_t && m->lv_nt >= 1 && m->lv_nx >= 2 &&
+                    m->lv_dx > 0,
+                "bad local volatility table");
+    TQF_REQUIRE(g->t_next != nullptr, "null t_next");
+  }
+  TQF_REQUIRE(out->mode >= TQF_OUT_PATH_MAJOR && out->mode <= TQF_OUT_PAYOFF,
+              "unknown output mode %d", out->mode);
+  if (out->mode == TQF_OUT_PAYOFF) {
+    TQF_REQUIRE(out->n_strikes >= 1 && out->strikes && out->workspace,
+                "payoff mode needs strikes and a workspace");
+    const int64_t lds = (int64_t)kWaves * g->n_slots * out->n_strikes * 2 *
+                        dtype_size(dtype);
+    TQF_REQUIRE(lds <= 48 * 1024,
+                "payoff mode supports n_slots * n_strikes <= %d for this dtype",
+                (int)(48 * 1024 / (kWaves * 2 * dtype_size(dtype))));
+  } else {
+    TQF_REQUIRE(out->paths != nullptr, "null output paths");
+  }
+  const hipStream_t s = (hipStream_t)stream;
+  if (dtype == TQF_F32)
+    return launch_typed<float>(m, g, r, out, n_paths, x0, x0_stride_path, s);
+  return launch_typed<double>(m, g, r, out, n_paths, x0, x0_stride_path, s);
+}

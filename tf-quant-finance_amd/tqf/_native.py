@@ -1,0 +1,146 @@
+"""ctypes binding of libtqf.so (the C ABI declared in include/tqf.h).
+
+The library is built in-tree by tf-quant-finance_amd/build.py.  Loading it
+fails loudly: there is no fallback implementation behind this module.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, '_lib', 'libtqf.so')
+
+TQF_F32, TQF_F64 = 0, 1
+
+MODEL_GBM, MODEL_CONST, MODEL_HESTON_FT, MODEL_LOCAL_VOL, MODEL_HESTON_QE = (
+    0, 1, 2, 3, 4)
+RNG_DRAWS, RNG_PHILOX, RNG_PHILOX_ANTITHETIC, RNG_SOBOL, RNG_SOBOL_SHIFTED = (
+    0, 1, 2, 3, 4)
+OUT_PATH_MAJOR, OUT_STEP_MAJOR, OUT_PAYOFF = 0, 1, 2
+
+_c_void_p = ctypes.c_void_p
+_i32, _i64, _u32, _u64, _f64 = (ctypes.c_int32, ctypes.c_int64,
+                                ctypes.c_uint32, ctypes.c_uint64,
+                                ctypes.c_double)
+
+
+class Model(ctypes.Structure):
+  _fields_ = [('kind', _i32), ('dim', _i32), ('p', _f64 * 8),
+              ('lv_vol', _c_void_p), ('lv_t', _c_void_p), ('lv_x0', _f64),
+              ('lv_dx', _f64), ('lv_nt', _i32), ('lv_nx', _i32),
+              ('step_coef', _c_void_p)]
+
+
+class Grid(ctypes.Structure):
+  _fields_ = [('dt', _c_void_p), ('sqrt_dt', _c_void_p), ('t_next', _c_void_p),
+              ('slot', _c_void_p), ('n_steps', _i32), ('n_slots', _i32),
+              ('init_slot', _i32), ('n_rng_steps', _i32)]
+
+
+class Rng(ctypes.Structure):
+  _fields_ = [('kind', _i32), ('stream_id', _u32), ('seed', _u64),
+              ('draws', _c_void_p), ('stride_path', _i64),
+              ('stride_step', _i64), ('sobol_v', _c_void_p),
+              ('sobol_lo', _c_void_p), ('sobol_shift', _c_void_p),
+              ('sobol_first_index', _i64), ('sobol_nd', _i32),
+              ('sobol_ndims', _i32)]
+
+
+class Output(ctypes.Structure):
+  _fields_ = [('mode', _i32), ('n_strikes', _i32), ('paths', _c_void_p),
+              ('strikes', _c_void_p), ('discount', _c_void_p),
+              ('workspace', _c_void_p), ('prices', _c_void_p),
+              ('stderrs', _c_void_p), ('sums', _c_void_p)]
+
+
+class Gbm(ctypes.Structure):
+  _fields_ = [('dim', _i32), ('drift', _c_void_p), ('vol', _c_void_p),
+              ('chol', _c_void_p), ('s0', _c_void_p),
+              ('s0_stride_path', _i64)]
+
+
+_SIGS = {
+    'tqf_version': (_i32, []),
+    'tqf_last_error': (ctypes.c_char_p, []),
+    'tqf_sobol_direction_numbers': (_i32, [_c_void_p, _c_void_p, _i32, _i32,
+                                           _c_void_p]),
+    'tqf_sobol_points': (_i32, [_c_void_p, _i32, _i32, _i64, _i64, _c_void_p,
+                                _c_void_p, _i32, _i32, _c_void_p, _i64,
+                                _c_void_p]),
+    'tqf_sobol_lo_table': (_i32, [_c_void_p, _i32, _c_void_p, _c_void_p]),
+    'tqf_sobol_normals': (_i32, [_c_void_p, _i32, _i32, _i64, _i64, _c_void_p,
+                                 _i32, _i32, _c_void_p, _c_void_p]),
+    'tqf_philox_normals': (_i32, [_u64, _u32, _i64, _i64, _i32, _i32,
+                                  _c_void_p, _c_void_p]),
+    'tqf_euler_workspace_size': (_i64, [_i64, _i32, _i32]),
+    'tqf_euler': (_i32, [ctypes.POINTER(Model), ctypes.POINTER(Grid),
+                         ctypes.POINTER(Rng), ctypes.POINTER(Output), _i64,
+                         _c_void_p, _i64, _i32, _c_void_p]),
+    'tqf_gbm_exact': (_i32, [ctypes.POINTER(Gbm), ctypes.POINTER(Grid),
+                             ctypes.POINTER(Rng), _i32, _c_void_p, _i64, _i32,
+                             _c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+  """Returns the loaded library; raises if it has not been built."""
+  global _lib
+  if _lib is None:
+    if not os.path.exists(LIB_PATH):
+      raise RuntimeError(
+          'libtqf.so not found at %s; build it with '
+          '`python tf-quant-finance_amd/build.py`' % LIB_PATH)
+    l = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+      f = getattr(l, name)
+      f.restype = res
+      f.argtypes = args
+    _lib = l
+  return _lib
+
+
+def exported_symbols():
+  return list(_SIGS)
+
+
+def check(rc):
+  """Maps a C ABI status to the reference's exception types."""
+  if rc == 0:
+    return
+  msg = lib().tqf_last_error().decode('utf-8', 'replace')
+  if rc == -1:
+    raise ValueError(msg)
+  if rc == -2:
+    raise NotImplementedError(msg)
+  raise RuntimeError('HIP error %d: %s' % (rc, msg))
+
+
+def call(name, *args):
+  check(getattr(lib(), name)(*args))
+
+
+def device():
+  """The GPU the engine runs on; raises when no ROCm device is present."""
+  if not torch.cuda.is_available():
+    raise RuntimeError('tqf kernels need a ROCm GPU (torch.cuda is not '
+                       'available); there is no CPU fallback')
+  return torch.device('cuda', torch.cuda.current_device())
+
+
+def stream_ptr(dev=None):
+  return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def ptr(t):
+  return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def dtype_code(dtype):
+  if dtype == torch.float32:
+    return TQF_F32
+  if dtype == torch.float64:
+    return TQF_F64
+  raise ValueError('dtype must be float32 or float64, got %s' % dtype)

@@ -1,0 +1,329 @@
+"""Launch layer of the fused Euler-Maruyama kernels (tqf_euler).
+
+A *coefficient spec* names a model whose drift/volatility the kernels know:
+  ('gbm', mu, sigma)                 dX = mu X dt + sigma X dW
+  ('const', a, b)                    dX = a dt + b dW
+  ('heston', r, kappa, theta, xi, rho, variant)   (log S, v)
+  ('local_vol', LocalVolTable)       d ln S = (r - q - s^2/2) dt + s dW
+  ('heston_qe', coef[steps, 12])     Andersen QE with per-step constants
+Callables returned by the tqf model classes carry their spec in the
+`_tqf_spec` attribute, so `euler_sampling.sample` can route them here.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from tqf import _native
+from tqf.math.random import sobol as sobol_lib
+from tqf.math.random.multivariate_normal import RandomType, seed_to_key
+from tqf.models import utils
+
+__all__ = ['tag', 'spec_of', 'model_dim', 'run', 'prepare', 'Launch',
+           'RngSpec', 'LocalVolTable']
+
+
+def tag(fn, spec):
+  fn._tqf_spec = spec
+  return fn
+
+
+def spec_of(drift_fn, volatility_fn):
+  a = getattr(drift_fn, '_tqf_spec', None)
+  b = getattr(volatility_fn, '_tqf_spec', None)
+  if a is None or b is None:
+    return None
+  if a[0] != b[0]:
+    return None
+  if a[0] == 'local_vol':
+    return a if a[1] is b[1] else None
+  return a if tuple(a) == tuple(b) else None
+
+
+def model_dim(spec):
+  return 2 if spec[0] in ('heston', 'heston_qe') else 1
+
+
+class LocalVolTable:
+  """sigma(t, S) on a (t_i, ln S_j) grid with a uniform log-spot axis.
+
+  Lookups are bilinear in (t, ln S) with flat extrapolation; the kernel
+  stages the table in LDS when it has at most 8192 entries.
+  """
+
+  def __init__(self, times, log_spot_min, log_spot_step, vols, drift_rate):
+    self.times = np.asarray(times, np.float64)
+    self.x0 = float(log_spot_min)
+    self.dx = float(log_spot_step)
+    self.vols = np.asarray(vols, np.float64)
+    self.drift_rate = float(drift_rate)
+    if self.vols.shape != (self.times.shape[0], self.vols.shape[1]):
+      raise ValueError('vols must have shape [len(times), n_spots]')
+
+  def sigma(self, t, x):
+    """Host evaluation of the table (same interpolation as the kernel)."""
+    nt, nx = self.vols.shape
+    fx = np.clip((np.asarray(x) - self.x0) / self.dx, 0, nx - 1)
+    j = np.minimum(fx.astype(np.int64), nx - 2)
+    wx = fx - j
+    row = max(int(np.searchsorted(self.times, t, side='right')) - 1, 0)
+    r1 = min(row + 1, nt - 1)
+    wt = 0.0
+    if row + 1 < nt and t > self.times[row]:
+      wt = (t - self.times[row]) / (self.times[row + 1] - self.times[row])
+    s0 = self.vols[row, j] + wx * (self.vols[row, j + 1] - self.vols[row, j])
+    s1 = self.vols[r1, j] + wx * (self.vols[r1, j + 1] - self.vols[r1, j])
+    return s0 + wt * (s1 - s0)
+
+
+class RngSpec:
+  """Where the normals come from.
+
+  kind: 'draws' (tensor [N, steps, d]), 'philox', 'philox_antithetic',
+  'sobol' (reference transform, index skip+1+p), 'sobol_shifted' (32-digit
+  points from index `first_index`, XOR `shift`).
+  """
+
+  def __init__(self, kind, seed=0, stream_id=0, draws=None, skip=0,
+               shift=None, first_index=0):
+    self.kind = kind
+    self.seed = seed
+    self.stream_id = stream_id
+    self.draws = draws
+    self.skip = skip
+    self.shift = shift
+    self.first_index = first_index
+
+  @classmethod
+  def from_random_type(cls, random_type, seed, skip, normal_draws=None):
+    if normal_draws is not None:
+      return cls('draws', draws=normal_draws)
+    if random_type is None:
+      random_type = RandomType.PSEUDO
+    if random_type in (RandomType.PSEUDO, RandomType.STATELESS):
+      return cls('philox', seed=seed_to_key(seed, random_type))
+    if random_type in (RandomType.PSEUDO_ANTITHETIC,
+                       RandomType.STATELESS_ANTITHETIC):
+      return cls('philox_antithetic', seed=seed_to_key(seed, random_type))
+    if random_type == RandomType.SOBOL:
+      return cls('sobol', skip=int(skip or 0))
+    raise NotImplementedError(
+        'random type {} is not supported by the engine'.format(random_type))
+
+
+_RNG_CODES = {'draws': _native.RNG_DRAWS, 'philox': _native.RNG_PHILOX,
+              'philox_antithetic': _native.RNG_PHILOX_ANTITHETIC,
+              'sobol': _native.RNG_SOBOL,
+              'sobol_shifted': _native.RNG_SOBOL_SHIFTED}
+
+
+def _model_struct(spec, dtype, dev, keep):
+  m = _native.Model()
+  d = model_dim(spec)
+  m.dim = d
+  if spec[0] == 'gbm':
+    m.kind = _native.MODEL_GBM
+    m.p[0], m.p[1] = float(spec[1]), float(spec[2])
+  elif spec[0] == 'const':
+    m.kind = _native.MODEL_CONST
+    m.p[0], m.p[1] = float(spec[1]), float(spec[2])
+  elif spec[0] == 'heston':
+    m.kind = _native.MODEL_HESTON_FT
+    for i in range(6):
+      m.p[i] = float(spec[1 + i])
+  elif spec[0] == 'heston_qe':
+    m.kind = _native.MODEL_HESTON_QE
+    coef = torch.as_tensor(np.ascontiguousarray(spec[1]), dtype=dtype,
+                           device=dev).contiguous()
+    keep.append(coef)
+    m.step_coef = coef.data_ptr()
+  elif spec[0] == 'local_vol':
+    tab = spec[1]
+    m.kind = _native.MODEL_LOCAL_VOL
+    m.p[0] = tab.drift_rate
+    vol = torch.as_tensor(tab.vols, dtype=dtype, device=dev).contiguous()
+    tt = torch.as_tensor(tab.times, dtype=dtype, device=dev).contiguous()
+    keep.extend([vol, tt])
+    m.lv_vol, m.lv_t = vol.data_ptr(), tt.data_ptr()
+    m.lv_x0, m.lv_dx = tab.x0, tab.dx
+    m.lv_nt, m.lv_nx = tab.vols.shape
+  else:
+    raise NotImplementedError('unknown model spec %r' % (spec[0],))
+  return m
+
+
+class Launch:
+  """A fully prepared tqf_euler call: descriptors and device buffers are
+  built once; `launch()` only enqueues the kernels on the current stream."""
+
+  def __init__(self, m, g, r, o, num_samples, x0_t, x0_stride, dtype, dev,
+               keep, result):
+    self.m, self.g, self.r, self.o = m, g, r, o
+    self.num_samples = num_samples
+    self.x0_t, self.x0_stride = x0_t, x0_stride
+    self.dtype, self.dev = dtype, dev
+    self.keep = keep
+    self.result = result
+
+  def launch(self):
+    _native.call('tqf_euler', ctypes.byref(self.m), ctypes.byref(self.g),
+                 ctypes.byref(self.r), ctypes.byref(self.o),
+                 ctypes.c_int64(self.num_samples),
+                 ctypes.c_void_p(self.x0_t.data_ptr()),
+                 ctypes.c_int64(self.x0_stride),
+                 ctypes.c_int32(_native.dtype_code(self.dtype)),
+                 _native.stream_ptr(self.dev))
+    return self.result
+
+
+def run(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
+        dtype, mode='paths', strikes=None, discount=None,
+        initial_written=None, dev=None):
+  """Runs the fused kernel once (see `prepare`)."""
+  return prepare(spec, all_times, keep_mask, num_requested, x0, num_samples,
+                 rng, dtype, mode, strikes, discount, initial_written,
+                 dev).launch()
+
+
+def prepare(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
+            dtype, mode='paths', strikes=None, discount=None,
+            initial_written=None, dev=None):
+  """Builds a Launch of the fused kernel.
+
+  mode 'paths'  -> [N, k, d] (reference layout)
+  mode 'store'  -> [k, d, N] (coalesced)
+  mode 'payoff' -> dict(prices [k, n_strikes], stderrs, sums [k, K, 2])
+  """
+  dev = dev or _native.device()
+  npd = utils.np_dtype(dtype)
+  d = model_dim(spec)
+  all_times = np.asarray(all_times, npd)
+  dt = (all_times[1:] - all_times[:-1]).astype(npd)
+  sqrt_dt = np.sqrt(dt).astype(npd)
+  slots, init_slot, executed, filled = utils.record_plan(
+      keep_mask, num_requested, initial_written)
+  n_rng_steps = dt.shape[0]
+  keep = []
+
+  def dev_t(a, dt_=dtype):
+    t = torch.as_tensor(np.ascontiguousarray(a), device=dev).to(dt_)
+    keep.append(t)
+    return t
+
+  g = _native.Grid()
+  g.dt = dev_t(dt[:max(executed, 1)]).data_ptr()
+  g.sqrt_dt = dev_t(sqrt_dt[:max(executed, 1)]).data_ptr()
+  g.t_next = dev_t(all_times[1:][:max(executed, 1)]).data_ptr()
+  g.slot = dev_t(np.concatenate([slots, [-1]]).astype(np.int32),
+                 torch.int32).data_ptr()
+  g.n_steps = executed
+  g.n_slots = int(num_requested)
+  g.init_slot = init_slot
+  g.n_rng_steps = max(n_rng_steps, executed)
+
+  m = _model_struct(spec, dtype, dev, keep)
+
+  r = _native.Rng()
+  r.kind = _RNG_CODES[rng.kind]
+  r.stream_id = int(rng.stream_id)
+  r.seed = int(rng.seed) & 0xFFFFFFFFFFFFFFFF
+  if rng.kind == 'draws':
+    draws = rng.draws
+    if draws.dtype != dtype or draws.device != dev:
+      draws = draws.to(device=dev, dtype=dtype)
+    if draws.dim() != 3 or draws.shape[2] != d:
+      raise ValueError('normal_draws must have shape [num_samples, '
+                       'num_time_steps, {}]'.format(d))
+    if draws.shape[1] < executed:
+      raise ValueError('normal_draws has {} steps, the grid needs {}'.format(
+          draws.shape[1], executed))
+    keep.append(draws)
+    r.draws = draws.data_ptr()
+    r.stride_path, r.stride_step = draws.stride(0), draws.stride(1)
+    if draws.stride(2) != 1:
+      draws = draws.contiguous()
+      keep.append(draws)
+      r.draws = draws.data_ptr()
+      r.stride_path, r.stride_step = draws.stride(0), draws.stride(1)
+  elif rng.kind in ('sobol', 'sobol_shifted'):
+    ndims = max(n_rng_steps, 1) * d
+    if rng.kind == 'sobol':
+      nd = sobol_lib.num_digits_for(rng.skip, num_samples)
+      first = rng.skip + 1
+    else:
+      nd = 32
+      first = rng.first_index
+    v = sobol_lib.device_columns(ndims, nd, dev=dev)
+    lo = sobol_lib.device_lo_table(ndims, nd, dev=dev)
+    r.sobol_v, r.sobol_lo = v.data_ptr(), lo.data_ptr()
+    if rng.shift is not None:
+      sh = dev_t(np.asarray(rng.shift, np.uint64).astype(np.uint32).view(
+          np.int32), torch.int32)
+      if sh.numel() < ndims:
+        raise ValueError('digital shift has %d entries, need %d' %
+                         (sh.numel(), ndims))
+      r.sobol_shift = sh.data_ptr()
+    r.sobol_first_index = first
+    r.sobol_nd = nd
+    r.sobol_ndims = ndims
+
+  x0_t = torch.as_tensor(x0, dtype=dtype, device=dev)
+  if x0_t.dim() == 0:
+    x0_t = x0_t.reshape(1)
+  if x0_t.dim() == 1:
+    if x0_t.shape[0] != d:
+      x0_t = x0_t.expand(d) if x0_t.shape[0] == 1 else x0_t
+    x0_stride = 0
+  else:
+    x0_t = x0_t.expand(num_samples, d)
+    x0_stride = d
+  x0_t = x0_t.contiguous()
+  keep.append(x0_t)
+
+  o = _native.Output()
+  result = None
+  if mode == 'paths':
+    o.mode = _native.OUT_PATH_MAJOR
+    result = torch.zeros((num_samples, int(num_requested), d), dtype=dtype,
+                         device=dev)
+    o.paths = result.data_ptr()
+  elif mode == 'store':
+    o.mode = _native.OUT_STEP_MAJOR
+    result = torch.zeros((int(num_requested), d, num_samples), dtype=dtype,
+                         device=dev)
+    o.paths = result.data_ptr()
+  elif mode == 'payoff':
+    if len(filled) != int(num_requested):
+      raise ValueError('every requested time must be reached for payoffs')
+    o.mode = _native.OUT_PAYOFF
+    k = torch.as_tensor(np.asarray(strikes, np.float64), dtype=torch.float64,
+                        device=dev).contiguous()
+    keep.append(k)
+    o.n_strikes = k.numel()
+    o.strikes = k.data_ptr()
+    disc = np.ones(int(num_requested)) if discount is None else discount
+    dd = dev_t(np.asarray(disc, np.float64), torch.float64)
+    o.discount = dd.data_ptr()
+    ws_bytes = _native.lib().tqf_euler_workspace_size(
+        num_samples, int(num_requested), k.numel())
+    ws = torch.empty(max(ws_bytes // 8, 1), dtype=torch.float64, device=dev)
+    keep.append(ws)
+    o.workspace = ws.data_ptr()
+    prices = torch.empty((int(num_requested), k.numel()), dtype=torch.float64,
+                         device=dev)
+    ses = torch.empty_like(prices)
+    sums = torch.empty((int(num_requested), k.numel(), 2),
+                       dtype=torch.float64, device=dev)
+    o.prices, o.stderrs, o.sums = (prices.data_ptr(), ses.data_ptr(),
+                                   sums.data_ptr())
+    result = {'prices': prices, 'stderrs': ses, 'sums': sums}
+  else:
+    raise ValueError('unknown mode %r' % mode)
+
+  if isinstance(result, dict):
+    result['_keep'] = keep
+  else:
+    # keep the inputs alive until the stream has consumed them
+    result._tqf_keep = keep
+  return Launch(m, g, r, o, num_samples, x0_t, x0_stride, dtype, dev, keep,
+                result)

@@ -1,0 +1,9 @@
+"""Path-simulation models (mirror of tff.models on the simulation path)."""
+from tqf.models import euler_sampling
+from tqf.models import utils
+from tqf.models.geometric_brownian_motion import GeometricBrownianMotion
+from tqf.models.geometric_brownian_motion import MultivariateGeometricBrownianMotion
+from tqf.models.heston import HestonModel
+
+__all__ = ['euler_sampling', 'utils', 'GeometricBrownianMotion',
+           'MultivariateGeometricBrownianMotion', 'HestonModel']
