@@ -209,6 +209,91 @@ int tqf_gbm_exact(const tqf_gbm* gbm, const tqf_grid* grid, const tqf_rng* rng,
                   int32_t out_mode, void* out, int64_t n_paths, int32_t dtype,
                   void* stream);
 
+/* ---- Longstaff-Schwartz ------------------------------------------------- */
+
+#define TQF_PAYOFF_BASKET_PUT 0  /* relu(K - mean_j S_j) (payoff_utils.py:27) */
+#define TQF_PAYOFF_MAX_CALL 1    /* relu(max_j S_j - K)                        */
+
+typedef struct tqf_lsm {
+  const double* paths;      /* device [n_paths][n_times][dim] float64        */
+  int64_t n_paths;
+  int32_t n_times;
+  int32_t dim;              /* assets, 1..8                                  */
+  int32_t payoff;           /* TQF_PAYOFF_*                                  */
+  int32_t degree;           /* make_polynomial_basis(degree)                 */
+  int32_t n_basis;          /* (degree + 1)^dim, at most 255                 */
+  int32_t pad0;
+  const uint8_t* exponents; /* device [n_basis][dim] exponent table in the
+                               reference's meshgrid order (lsm.py:116-118)   */
+  double strike;
+  int64_t num_calibration;  /* rows used for the regression (0 = all)       */
+  double rcond;             /* pivot cutoff relative to the largest scaled
+                               diagonal of X^T X                            */
+} tqf_lsm;
+
+int64_t tqf_lsm_workspace_size(int64_t n_paths, int32_t n_basis);
+
+/* Device.  Longstaff-Schwartz price of one payoff: backward induction over
+ * the exercise indices ex_idx[n_ex] (time indices into the paths), with the
+ * padded discount factors df[n_ex + 1] (df[0] = 1, df[i] = discount of
+ * exercise date i-1), the ITM-masked float64 normal equations built in the
+ * kernel from the centred tensor-product basis, a column-scaled Cholesky
+ * solve with pivot truncation, and the exercise update.  cash/vals are
+ * device [n_paths] state buffers; result[2] = (price, standard error) over
+ * the rows after num_calibration; ranks[n_ex - 1] (optional) receives the
+ * rank kept at each regression.  Replaces least_square_mc and its loop body
+ * (tff/models/longstaff_schwartz/lsm.py:128-436). */
+int tqf_lsm_price(const tqf_lsm* lsm, const int32_t* ex_idx, int32_t n_ex,
+                  const double* df, void* workspace, double* cash,
+                  double* vals, double* result, int32_t* ranks, void* stream);
+
+/* ---- Batched heterogeneous pricing with adjoint Greeks ----------------- */
+
+#define TQF_CONTRACT_GBM 0     /* log-Euler GBM: Greeks delta, vega, rho     */
+#define TQF_CONTRACT_HESTON 1  /* full-truncation Heston: delta, d/dv0, rho,
+                                  d/dkappa, d/dtheta, d/dxi, d/drho          */
+
+typedef struct tqf_contract {
+  int32_t kind;
+  int32_t n_steps;      /* uniform grid dt = maturity / n_steps             */
+  uint32_t stream_id;   /* Philox sub-stream of the contract's paths        */
+  int32_t pad0;
+  int64_t n_paths;
+  double s0, strike, maturity, rate;
+  double sigma;         /* GBM                                              */
+  double v0, kappa, theta, xi, rho;  /* Heston                              */
+} tqf_contract;
+
+typedef struct tqf_work_item {
+  int32_t contract;
+  int32_t pad0;
+  int64_t first_path;   /* Philox path index of the item's first path       */
+  int64_t n_paths;
+} tqf_work_item;
+
+int64_t tqf_batch_workspace_size(int32_t n_items, int32_t n_blocks,
+                                 int32_t max_steps);
+
+/* Device.  Prices European calls for n_contracts contracts in one
+ * persistent launch over n_items work items (device arrays; the items of a
+ * contract are contiguous rows; `order` (device [n_items], or NULL for row
+ * order) is the processing order, longest item first), with
+ * pathwise adjoint Greeks (Heston: checkpoints every 16 steps and a reverse
+ * sweep).  Philox stream: key = seed, sub-stream = the contract's
+ * stream_id, path index = first_path + offset.  out: device [n_contracts][16] doubles:
+ * 8 means (price, Greeks) then their 8 standard errors.  The launch records
+ * no memset, so it can be captured in a hipGraph.  Replaces sampling with
+ * euler_sampling.sample per contract (tff/models/euler_sampling.py:27-537)
+ * plus tf.GradientTape / fwd_gradient Greeks (tff/math/gradient.py:22-235,
+ * tff/math/custom_loops.py:20-214). */
+int tqf_batch_price(const tqf_contract* contracts, int32_t n_contracts,
+                    const tqf_work_item* items, const int32_t* order,
+                    int32_t n_items,
+                    const int32_t* contract_first_item,
+                    const int32_t* contract_n_items, uint64_t seed,
+                    int32_t n_blocks, int32_t max_steps, int32_t use_counter,
+                    void* workspace, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

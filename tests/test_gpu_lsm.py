@@ -1,0 +1,117 @@
+"""GPU Longstaff-Schwartz against the reference's table values and the
+CPU oracle (numpy pinv) on simulated paths."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import load_golden
+from oracle import gbm as ogbm
+from oracle import lsm as olsm
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+  import tqf
+  lsm = tqf.models.longstaff_schwartz
+
+
+def _np(t):
+  return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+def test_lsm_reference_table(dtype):
+  g = load_golden('lsm_ls2001.json')
+  s = np.array(g['samples'])[..., None]
+  df = np.exp(-np.cumsum(g['interest_rates']))
+  tol = g['tolerance']
+  basis = lsm.make_polynomial_basis(2)
+  put = lsm.make_basket_put_payoff([1.1], dtype=dtype)
+  np.testing.assert_allclose(
+      _np(lsm.least_square_mc(s, [3], put, basis, discount_factors=[df[-1]],
+                              dtype=dtype)), [g['european_put']], atol=tol,
+      rtol=tol)
+  np.testing.assert_allclose(
+      _np(lsm.least_square_mc(s, [1, 2, 3], put, basis, discount_factors=df,
+                              dtype=dtype)), [g['american_put']], atol=tol,
+      rtol=tol)
+  np.testing.assert_allclose(
+      _np(lsm.least_square_mc(s, [1, 2, 3], put, basis, discount_factors=df,
+                              num_calibration_samples=4, dtype=dtype)),
+      [g['american_put_calib4']], atol=tol, rtol=tol)
+  put2 = lsm.make_basket_put_payoff([1.1, 1.2], dtype=dtype)
+  df2 = np.exp(-np.cumsum(g['batch_rates'], -1))[None]
+  np.testing.assert_allclose(
+      _np(lsm.least_square_mc(s, [1, 2, 3], put2, basis, discount_factors=df2,
+                              dtype=dtype)), g['batch_payoff'], atol=tol,
+      rtol=tol)
+  np.testing.assert_allclose(
+      _np(lsm.least_square_mc(np.stack([s, s + 0.1]), [1, 2, 3], put2, basis,
+                              discount_factors=df2, dtype=dtype)),
+      g['batch_samples'], atol=tol, rtol=tol)
+
+
+def test_lsm_duplicate_assets_equal_one_asset():
+  g = load_golden('lsm_ls2001.json')
+  s = np.array(g['samples'])[..., None]
+  df = np.exp(-np.cumsum(g['interest_rates']))
+  basis = lsm.make_polynomial_basis(10)
+  put = lsm.make_basket_put_payoff([1.1, 1.2, 1.3], dtype=np.float64)
+  a = lsm.least_square_mc(np.concatenate([s, s], -1), [1, 2, 3], put, basis,
+                          discount_factors=df, dtype=np.float64)
+  b = lsm.least_square_mc(s, [1, 2, 3], put, basis, discount_factors=df,
+                          dtype=np.float64)
+  np.testing.assert_allclose(_np(a), _np(b), rtol=1e-4, atol=1e-4)
+  assert a.shape == (3,)
+
+
+@pytest.mark.parametrize('dim,degree,kind', [(1, 3, 'put'), (2, 2, 'put'),
+                                             (3, 2, 'max_call')])
+def test_lsm_matches_oracle_on_simulated_paths(dim, degree, kind):
+  rs = np.random.RandomState(dim)
+  n, times = 20000, np.linspace(0.25, 1.0, 4)
+  corr = np.full((dim, dim), 0.3) + 0.7 * np.eye(dim)
+  z = rs.standard_normal((n, times.shape[0], dim))
+  paths = ogbm.mv_sample_paths(times, [0.02] * dim, [0.25] * dim, corr,
+                               [100.0] * dim, z)
+  paths = np.concatenate([np.full((n, 1, dim), 100.0), paths], 1)
+  df = np.exp(-0.05 * times)
+  if kind == 'put':
+    pay_o = olsm.make_basket_put_payoff([100.0])
+    pay_t = lsm.make_basket_put_payoff([100.0])
+  else:
+    pay_o = olsm.make_basket_call_max_payoff([100.0])
+    pay_t = lsm.make_max_call_payoff([100.0])
+  for calib in (None, n // 2):
+    want = olsm.least_square_mc(paths, [1, 2, 3, 4], pay_o,
+                                olsm.make_polynomial_basis(degree),
+                                discount_factors=df,
+                                num_calibration_samples=calib)
+    got = lsm.least_square_mc(paths, [1, 2, 3, 4], pay_t,
+                              lsm.make_polynomial_basis(degree),
+                              discount_factors=df,
+                              num_calibration_samples=calib,
+                              dtype=np.float64)
+    # same exercise decisions up to rounding of the regression: the price
+    # agrees to far below its Monte Carlo error
+    np.testing.assert_allclose(_np(got), want, rtol=2e-6)
+
+
+def test_generic_torch_lsm_matches_native():
+  rs = np.random.RandomState(9)
+  n, times = 5000, np.linspace(0.5, 2.0, 4)
+  z = rs.standard_normal((n, 4, 1))
+  paths = ogbm.sample_paths(times, 0.03, 0.3, 100.0, z)
+  paths = np.concatenate([np.full((n, 1, 1), 100.0), paths], 1)
+  put = lsm.make_basket_put_payoff([100.0])
+  native = lsm.least_square_mc(paths, [1, 2, 3, 4], put,
+                               lsm.make_polynomial_basis(3),
+                               discount_factors=np.exp(-0.03 * times),
+                               dtype=np.float64)
+  basis = lsm.make_polynomial_basis(3)
+  plain_basis = lambda p, t: basis(p, t)
+  plain_put = lambda p, t: put(p, t)
+  generic = lsm.least_square_mc(paths, [1, 2, 3, 4], plain_put, plain_basis,
+                                discount_factors=np.exp(-0.03 * times),
+                                dtype=np.float64)
+  np.testing.assert_allclose(_np(generic), _np(native), rtol=1e-6)

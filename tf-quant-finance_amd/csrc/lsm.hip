@@ -1,0 +1,568 @@
+// Longstaff-Schwartz regression on the GPU (tff/models/longstaff_schwartz/
+// lsm.py:128-436) for the reference's polynomial basis
+// (make_polynomial_basis, lsm.py:50-125) and the built-in payoffs.
+//
+// Per exercise date (backward induction):
+//   tqf_lsm_prepare   exercise value, continuation target y = df_ratio *
+//                     (cashflow + values), per-asset sums for the centring
+//   tqf_lsm_gram      G = X^T W X and X^T W y in float64 over the ITM
+//                     calibration rows, with the basis built on the fly
+//                     (X is never written to HBM)
+//   tqf_lsm_solve     float64 Cholesky of the column-scaled G with pivot
+//                     truncation (rank-deficient columns are dropped, which
+//                     leaves the fitted values X beta of the pseudo-inverse
+//                     solution unchanged) and the triangular solves
+//   tqf_lsm_update    continuation = relu(X beta), exercise decision,
+//                     cashflow / value update for every path
+// and tqf_lsm_finalize for the discounted mean over the pricing rows.
+//
+// Basis: the centred variables c_j = (x_j - mean_j) * scale_j are scaled by
+// a per-asset factor (the host uses 1 / std); column k is
+// prod_j c_j^{e[k][j]} with the reference's exponent order.  Scaling a
+// column by a constant does not change the fitted values.
+#include "common.h"
+
+using namespace tqf;
+
+namespace {
+
+constexpr int kMaxAssets = 8;
+constexpr int kMaxDeg = 8;
+constexpr int kTile = 64;       // Gram output tile (kTile x kTile)
+constexpr int kSubPaths = 32;   // paths staged per LDS round
+
+struct LsmArgs {
+  const double* paths;      // [n_paths][n_times][d] (path-major) float64
+  int64_t n_paths;
+  int n_times, d;
+  int t_idx;                // time index of the exercise date
+  int payoff;               // TQF_PAYOFF_*
+  double strike;
+  const double* mean;       // [d]  (device) centring
+  const double* scale;      // [d]
+  const uint8_t* expo;      // [K][d] exponents
+  int K;
+  int deg;
+};
+
+__device__ __forceinline__ double payoff_value(const LsmArgs& a,
+                                               const double* x) {
+  if (a.payoff == TQF_PAYOFF_BASKET_PUT) {
+    double s = 0;
+    for (int j = 0; j < a.d; ++j) s += x[j];
+    const double v = a.strike - s / a.d;
+    return v > 0 ? v : 0;
+  } else {  // TQF_PAYOFF_MAX_CALL
+    double m = x[0];
+    for (int j = 1; j < a.d; ++j) m = x[j] > m ? x[j] : m;
+    const double v = m - a.strike;
+    return v > 0 ? v : 0;
+  }
+}
+
+// Exercise values, continuation targets and the per-block asset moments
+// (sum and sum of squares, for the centring and the column scaling).
+__global__ void __launch_bounds__(256)
+lsm_prepare_kernel(LsmArgs a, double df_ratio, const double* __restrict__ cash,
+                   const double* __restrict__ vals, double* __restrict__ exv,
+                   double* __restrict__ y, double* __restrict__ block_sums) {
+  __shared__ double red[2 * kMaxAssets][256];
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double x[kMaxAssets];
+  for (int j = 0; j < kMaxAssets; ++j) x[j] = 0;
+  if (p < a.n_paths) {
+    const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
+    for (int j = 0; j < a.d; ++j) x[j] = row[j];
+    exv[p] = payoff_value(a, x);
+    y[p] = df_ratio * (cash[p] + vals[p]);
+  }
+  for (int j = 0; j < a.d; ++j) {
+    red[j][threadIdx.x] = x[j];
+    red[kMaxAssets + j][threadIdx.x] = x[j] * x[j];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int j = 0; j < a.d; ++j) {
+        red[j][threadIdx.x] += red[j][threadIdx.x + s];
+        red[kMaxAssets + j][threadIdx.x] += red[kMaxAssets + j][threadIdx.x + s];
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < a.d) {
+    block_sums[(int64_t)blockIdx.x * 2 * kMaxAssets + threadIdx.x] =
+        red[threadIdx.x][0];
+    block_sums[(int64_t)blockIdx.x * 2 * kMaxAssets + kMaxAssets + threadIdx.x] =
+        red[kMaxAssets + threadIdx.x][0];
+  }
+}
+
+// mean[d] and scale[d] = 1 / std (1 when the spread is zero), fixed order.
+__global__ void lsm_mean_kernel(const double* __restrict__ block_sums,
+                                int64_t n_blocks, int d, int64_t n_paths,
+                                double* __restrict__ mean,
+                                double* __restrict__ scale) {
+  const int j = threadIdx.x;
+  if (j >= d) return;
+  double s = 0, q = 0;
+  for (int64_t b = 0; b < n_blocks; ++b) {
+    s += block_sums[b * 2 * kMaxAssets + j];
+    q += block_sums[b * 2 * kMaxAssets + kMaxAssets + j];
+  }
+  const double n = (double)n_paths;
+  const double m = s / n;
+  const double var = q / n - m * m;
+  mean[j] = m;
+  scale[j] = var > 1e-300 && sqrt(var) > 1e-12 * fabs(m) ? 1.0 / sqrt(var) : 1.0;
+}
+
+// Basis column k of one path from its per-asset power table.
+__device__ __forceinline__ double basis_col(const double* pw, int deg1,
+                                            const uint8_t* e, int d) {
+  double v = 1.0;
+  for (int j = 0; j < d; ++j) v *= pw[j * deg1 + e[j]];
+  return v;
+}
+
+// G tile (I, J) partial over one chunk of paths.  Column K of the extended
+// design is y (so G[:, K] = X^T W y).  grid = (n_chunks, n_pairs).
+__global__ void __launch_bounds__(256)
+lsm_gram_kernel(LsmArgs a, const double* __restrict__ exv,
+                const double* __restrict__ y, int64_t n_calib,
+                int64_t chunk, const int2* __restrict__ pairs,
+                double* __restrict__ partial) {
+  __shared__ double xi[kSubPaths][kTile];
+  __shared__ double xj[kSubPaths][kTile];
+  __shared__ double pw[kSubPaths][kMaxAssets * (kMaxDeg + 1)];
+  __shared__ double wrow[kSubPaths];
+  __shared__ double yrow[kSubPaths];
+  __shared__ uint8_t expo[256 * kMaxAssets];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int2 pr = pairs[blockIdx.y];
+  const int i0 = pr.x * kTile, j0 = pr.y * kTile;
+  const int deg1 = a.deg + 1;
+  const int kext = a.K + 1;
+  for (int e = threadIdx.x; e < a.K * a.d; e += 256) expo[e] = a.expo[e];
+
+  double acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = 0;
+
+  const int64_t p_begin = (int64_t)blockIdx.x * chunk;
+  int64_t p_end = p_begin + chunk;
+  if (p_end > n_calib) p_end = n_calib;
+  for (int64_t p0 = p_begin; p0 < p_end; p0 += kSubPaths) {
+    __syncthreads();
+    // per-path powers and weights
+    if (threadIdx.x < kSubPaths) {
+      const int s = threadIdx.x;
+      const int64_t p = p0 + s;
+      double w = 0;
+      if (p < p_end) {
+        const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
+        for (int j = 0; j < a.d; ++j) {
+          const double c = (row[j] - a.mean[j]) * a.scale[j];
+          double v = 1.0;
+          for (int e = 0; e <= a.deg; ++e) {
+            pw[s][j * deg1 + e] = v;
+            v *= c;
+          }
+        }
+        w = exv[p] > 0 ? 1.0 : 0.0;  // in-the-money rows only
+        yrow[s] = y[p];
+      } else {
+        yrow[s] = 0;
+      }
+      wrow[s] = w;
+    }
+    __syncthreads();
+    // design tiles: row s, column i0 + c and j0 + c (column K is y)
+    for (int e = threadIdx.x; e < kSubPaths * kTile; e += 256) {
+      const int s = e / kTile, c = e - s * kTile;
+      const int ci = i0 + c, cj = j0 + c;
+      double vi = 0, vj = 0;
+      if (wrow[s] != 0) {
+        if (ci < a.K) vi = basis_col(pw[s], deg1, expo + ci * a.d, a.d);
+        else if (ci == a.K) vi = yrow[s];
+        if (cj < a.K) vj = basis_col(pw[s], deg1, expo + cj * a.d, a.d);
+        else if (cj == a.K) vj = yrow[s];
+      }
+      xi[s][c] = vi;
+      xj[s][c] = vj;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int s = 0; s < kSubPaths; ++s) {
+      double ai[4], bj[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ai[r] = xi[s][ty * 4 + r];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bj[c] = xj[s][tx * 4 + c];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = fma(ai[r], bj[c], acc[r][c]);
+    }
+  }
+  // partial[(chunk, pair)][64][64]
+  double* out = partial + ((int64_t)blockIdx.x * gridDim.y + blockIdx.y) *
+                              (kTile * kTile);
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      out[(ty * 4 + r) * kTile + tx * 4 + c] = acc[r][c];
+  (void)kext;
+}
+
+// Sums chunk partials into the dense extended Gram G[kext][kext] (symmetric
+// fill).  One thread per tile entry per pair.
+__global__ void __launch_bounds__(256)
+lsm_gram_reduce_kernel(const double* __restrict__ partial, int n_chunks,
+                       int n_pairs, const int2* __restrict__ pairs, int kext,
+                       double* __restrict__ g) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  const int pair = blockIdx.y;
+  if (e >= kTile * kTile) return;
+  double s = 0;
+  for (int c = 0; c < n_chunks; ++c)
+    s += partial[((int64_t)c * n_pairs + pair) * (kTile * kTile) + e];
+  const int2 pr = pairs[pair];
+  const int r = pr.x * kTile + e / kTile, col = pr.y * kTile + e % kTile;
+  if (r < kext && col < kext) {
+    g[(int64_t)r * kext + col] = s;
+    g[(int64_t)col * kext + r] = s;
+  }
+}
+
+// Column-scaled Cholesky with pivot truncation and the two triangular
+// solves, one workgroup, in place on the device (float64).  beta[K] out.
+__global__ void __launch_bounds__(1024)
+lsm_solve_kernel(double* __restrict__ g, int K, double rtol,
+                 double* __restrict__ beta, double* __restrict__ work,
+                 int* __restrict__ rank_out) {
+  const int kext = K + 1;
+  double* dsc = work;          // [K] column scales
+  double* z = work + K;        // [K] forward-substitution result
+  __shared__ double piv;
+  __shared__ double maxdiag;
+  __shared__ int rank;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int i = t; i < K; i += nt) {
+    const double d = g[(int64_t)i * kext + i];
+    dsc[i] = d > 0 ? 1.0 / sqrt(d) : 0.0;
+  }
+  __syncthreads();
+  // scale G and the rhs column
+  for (int64_t e = t; e < (int64_t)K * K; e += nt) {
+    const int r = (int)(e / K), c = (int)(e % K);
+    g[(int64_t)r * kext + c] *= dsc[r] * dsc[c];
+  }
+  for (int r = t; r < K; r += nt) g[(int64_t)r * kext + K] *= dsc[r];
+  if (t == 0) {
+    double m = 0;
+    for (int i = 0; i < K; ++i) m = fmax(m, g[(int64_t)i * kext + i]);
+    maxdiag = m;
+    rank = 0;
+  }
+  __syncthreads();
+  // right-looking Cholesky on the lower triangle
+  for (int k = 0; k < K; ++k) {
+    if (t == 0) {
+      const double d = g[(int64_t)k * kext + k];
+      if (d > rtol * maxdiag && d > 0) {
+        piv = sqrt(d);
+        ++rank;
+      } else {
+        piv = 0;  // truncated: drop this column
+      }
+      g[(int64_t)k * kext + k] = piv;
+    }
+    __syncthreads();
+    const double pk = piv;
+    for (int i = k + 1 + t; i < K; i += nt)
+      g[(int64_t)i * kext + k] = pk > 0 ? g[(int64_t)i * kext + k] / pk : 0.0;
+    __syncthreads();
+    const int m = K - k - 1;
+    for (int64_t e = t; e < (int64_t)m * m; e += nt) {
+      const int i = k + 1 + (int)(e / m), j = k + 1 + (int)(e % m);
+      if (j <= i)
+        g[(int64_t)i * kext + j] -= g[(int64_t)i * kext + k] *
+                                    g[(int64_t)j * kext + k];
+    }
+    __syncthreads();
+  }
+  // L z = b and L^T w = z, column-oriented so every step is parallel;
+  // beta = D w.  z overwrites the rhs column, w goes to beta.
+  for (int j = 0; j < K; ++j) {
+    if (t == 0) {
+      const double ljj = g[(int64_t)j * kext + j];
+      z[j] = ljj > 0 ? g[(int64_t)j * kext + K] / ljj : 0.0;
+    }
+    __syncthreads();
+    const double zj = z[j];
+    for (int i = j + 1 + t; i < K; i += nt)
+      g[(int64_t)i * kext + K] -= g[(int64_t)i * kext + j] * zj;
+    __syncthreads();
+  }
+  for (int j = K - 1; j >= 0; --j) {
+    if (t == 0) {
+      const double ljj = g[(int64_t)j * kext + j];
+      beta[j] = ljj > 0 ? z[j] / ljj : 0.0;
+    }
+    __syncthreads();
+    const double bj = beta[j];
+    for (int i = t; i < j; i += nt) z[i] -= g[(int64_t)j * kext + i] * bj;
+    __syncthreads();
+  }
+  for (int i = t; i < K; i += nt) beta[i] *= dsc[i];
+  if (t == 0) *rank_out = rank;
+}
+
+// continuation = relu(X beta); exercise iff ex > continuation.
+__global__ void __launch_bounds__(256)
+lsm_update_kernel(LsmArgs a, double df_ratio, const double* __restrict__ exv,
+                  const double* __restrict__ beta, double* __restrict__ cash,
+                  double* __restrict__ vals) {
+  __shared__ double sb[256];
+  __shared__ uint8_t expo[256 * kMaxAssets];
+  for (int k = threadIdx.x; k < a.K; k += 256) sb[k] = beta[k];
+  for (int e = threadIdx.x; e < a.K * a.d; e += 256) expo[e] = a.expo[e];
+  __syncthreads();
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.n_paths) return;
+  const int deg1 = a.deg + 1;
+  double pw[kMaxAssets * (kMaxDeg + 1)];
+  const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
+  for (int j = 0; j < a.d; ++j) {
+    const double c = (row[j] - a.mean[j]) * a.scale[j];
+    double v = 1.0;
+    for (int e = 0; e <= a.deg; ++e) {
+      pw[j * deg1 + e] = v;
+      v *= c;
+    }
+  }
+  double cont = 0;
+  for (int k = 0; k < a.K; ++k) cont += basis_col(pw, deg1, expo + k * a.d, a.d) * sb[k];
+  cont = cont > 0 ? cont : 0;
+  const double ex = exv[p];
+  if (ex > cont) {
+    vals[p] = 0;
+    cash[p] = ex;
+  } else {
+    vals[p] = df_ratio * (cash[p] + vals[p]);
+    cash[p] = 0;
+  }
+}
+
+// Terminal exercise values as the initial cashflow.
+__global__ void __launch_bounds__(256)
+lsm_terminal_kernel(LsmArgs a, double* __restrict__ cash,
+                    double* __restrict__ vals) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.n_paths) return;
+  double x[kMaxAssets];
+  const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
+  for (int j = 0; j < a.d; ++j) x[j] = row[j];
+  cash[p] = payoff_value(a, x);
+  vals[p] = 0;
+}
+
+// Per-block sums of df0 * (cash + vals) over rows [first, n) (and squares).
+__global__ void __launch_bounds__(256)
+lsm_final_kernel(const double* __restrict__ cash, const double* __restrict__ vals,
+                 double df0, int64_t first, int64_t n,
+                 double* __restrict__ block_sums) {
+  __shared__ double s1[256], s2[256];
+  const int64_t p = first + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0;
+  if (p < n) v = df0 * (cash[p] + vals[p]);
+  s1[threadIdx.x] = v;
+  s2[threadIdx.x] = v * v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      s1[threadIdx.x] += s1[threadIdx.x + s];
+      s2[threadIdx.x] += s2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    block_sums[2 * blockIdx.x] = s1[0];
+    block_sums[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+
+__global__ void lsm_final_reduce_kernel(const double* __restrict__ block_sums,
+                                        int64_t n_blocks, int64_t count,
+                                        double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double a = 0, b = 0;
+  for (int64_t i = 0; i < n_blocks; ++i) {
+    a += block_sums[2 * i];
+    b += block_sums[2 * i + 1];
+  }
+  const double n = (double)count;
+  const double mean = a / n;
+  double var = n > 1 ? (b - n * mean * mean) / (n - 1) : 0.0;
+  out[0] = mean;
+  out[1] = sqrt((var > 0 ? var : 0.0) / n);
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+LsmArgs make_args(const tqf_lsm* l, int t_idx) {
+  LsmArgs a{};
+  a.paths = l->paths;
+  a.n_paths = l->n_paths;
+  a.n_times = l->n_times;
+  a.d = l->dim;
+  a.t_idx = t_idx;
+  a.payoff = l->payoff;
+  a.strike = l->strike;
+  a.mean = nullptr;
+  a.scale = nullptr;
+  a.expo = l->exponents;
+  a.K = l->n_basis;
+  a.deg = l->degree;
+  return a;
+}
+
+int validate(const tqf_lsm* l) {
+  TQF_REQUIRE(l != nullptr, "null descriptor");
+  TQF_REQUIRE(l->dim >= 1 && l->dim <= kMaxAssets,
+              "LSM supports 1..%d assets, got %d", kMaxAssets, l->dim);
+  TQF_REQUIRE(l->degree >= 0 && l->degree <= kMaxDeg,
+              "polynomial degree must be in [0, %d]", kMaxDeg);
+  TQF_REQUIRE(l->n_basis >= 1 && l->n_basis <= 255,
+              "basis size must be in [1, 255], got %d", l->n_basis);
+  TQF_REQUIRE(l->n_paths >= 1 && l->n_times >= 1, "empty sample paths");
+  TQF_REQUIRE(l->payoff == TQF_PAYOFF_BASKET_PUT ||
+                  l->payoff == TQF_PAYOFF_MAX_CALL,
+              "unknown payoff %d", l->payoff);
+  TQF_REQUIRE(l->paths && l->exponents, "null LSM arrays");
+  return TQF_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t tqf_lsm_workspace_size(int64_t n_paths, int32_t n_basis) {
+  const int64_t nb = (n_paths + 255) / 256;
+  const int tiles = (n_basis + 1 + kTile - 1) / kTile;
+  const int pairs = tiles * (tiles + 1) / 2;
+  const int64_t chunks = 256;
+  const int64_t kext = n_basis + 1;
+  int64_t bytes = 0;
+  bytes += 3 * n_paths * 8;                     // exv, y, spare
+  bytes += nb * 2 * kMaxAssets * 8 + 2 * nb * 8;  // block sums
+  bytes += chunks * pairs * kTile * kTile * 8;  // Gram partials
+  bytes += kext * kext * 8;                     // G
+  bytes += 4 * kext * 8 + 64;                   // beta, work, rank
+  bytes += pairs * 8 + 4 * kMaxAssets * 8;      // pairs, mean, scale
+  return bytes + 4096;
+}
+
+extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
+                             int32_t n_ex, const double* df, void* workspace,
+                             double* cash, double* vals, double* result,
+                             int32_t* ranks, void* stream) {
+  clear_error();
+  int rc = validate(l);
+  if (rc) return rc;
+  TQF_REQUIRE(ex_idx && n_ex >= 1 && df && workspace && cash && vals && result,
+              "null LSM argument");
+  TQF_REQUIRE(l->num_calibration >= 0 && l->num_calibration <= l->n_paths,
+              "num_calibration_samples out of range");
+  for (int i = 0; i < n_ex; ++i)
+    TQF_REQUIRE(ex_idx[i] >= 0 && ex_idx[i] < l->n_times,
+                "exercise index %d out of range", ex_idx[i]);
+  const hipStream_t s = (hipStream_t)stream;
+  const int64_t n = l->n_paths;
+  const int64_t nb = (n + 255) / 256;
+  const int K = l->n_basis, kext = K + 1;
+  // per-date centring and scaling live in the workspace
+  const int tiles = (kext + kTile - 1) / kTile;
+  const int n_pairs = tiles * (tiles + 1) / 2;
+  const int64_t n_calib = l->num_calibration > 0 ? l->num_calibration : n;
+  int64_t chunk = (n_calib + 255) / 256;
+  chunk = ((chunk + kSubPaths - 1) / kSubPaths) * kSubPaths;
+  if (chunk < kSubPaths) chunk = kSubPaths;
+  const int n_chunks = (int)((n_calib + chunk - 1) / chunk);
+
+  char* w = (char*)workspace;
+  auto take = [&](int64_t bytes) {
+    char* p = w;
+    w += (bytes + 255) & ~(int64_t)255;
+    return p;
+  };
+  double* exv = (double*)take(n * 8);
+  double* y = (double*)take(n * 8);
+  double* bsum = (double*)take(nb * 2 * kMaxAssets * 8);
+  double* fsum = (double*)take(2 * nb * 8);
+  double* partial = (double*)take((int64_t)256 * n_pairs * kTile * kTile * 8);
+  double* g = (double*)take((int64_t)kext * kext * 8);
+  double* beta = (double*)take(kext * 8);
+  double* work = (double*)take(2 * kext * 8);
+  int* rank = (int*)take(64);
+  int2* pairs = (int2*)take(n_pairs * 8);
+  double* mean = (double*)take(kMaxAssets * 8);
+  double* scale = (double*)take(kMaxAssets * 8);
+
+  // pair table (host -> device, tiny)
+  int2 hp[64];
+  TQF_REQUIRE(n_pairs <= 64, "basis too large");
+  // the pair table is a device copy of this host array; the copy must be
+  // complete before `hp` leaves scope
+
+  {
+    int q = 0;
+    for (int i = 0; i < tiles; ++i)
+      for (int j = i; j < tiles; ++j) hp[q++] = int2{i, j};
+  }
+  hipError_t he = hipMemcpyAsync(pairs, hp, n_pairs * sizeof(int2),
+                                 hipMemcpyHostToDevice, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  if (he != hipSuccess) return set_error((int)he, "copy pairs");
+
+  tqf_lsm l2 = *l;
+  LsmArgs a = make_args(&l2, ex_idx[n_ex - 1]);
+  hipLaunchKernelGGL(lsm_terminal_kernel, blocks_for(n), 256, 0, s, a, cash,
+                     vals);
+  for (int e = n_ex - 1; e > 0; --e) {
+    a = make_args(&l2, ex_idx[e - 1]);
+    a.mean = mean;
+    a.scale = scale;
+    const double ratio = df[e + 1] / df[e];
+    hipLaunchKernelGGL(lsm_prepare_kernel, blocks_for(n), 256, 0, s, a, ratio,
+                       cash, vals, exv, y, bsum);
+    hipLaunchKernelGGL(lsm_mean_kernel, 1, 64, 0, s, bsum, nb, a.d, n, mean,
+                       scale);
+    hipLaunchKernelGGL(lsm_gram_kernel, dim3(n_chunks, n_pairs), 256, 0, s, a,
+                       exv, y, n_calib, chunk, pairs, partial);
+    hipLaunchKernelGGL(lsm_gram_reduce_kernel,
+                       dim3((kTile * kTile + 255) / 256, n_pairs), 256, 0, s,
+                       partial, n_chunks, n_pairs, pairs, kext, g);
+    hipLaunchKernelGGL(lsm_solve_kernel, 1, 1024, 0, s, g, K, l->rcond, beta,
+                       work, rank);
+    if (ranks) {
+      he = hipMemcpyAsync(ranks + (e - 1), rank, sizeof(int),
+                          hipMemcpyDeviceToDevice, s);
+      if (he != hipSuccess) return set_error((int)he, "copy rank");
+    }
+    hipLaunchKernelGGL(lsm_update_kernel, blocks_for(n), 256, 0, s, a, ratio,
+                       exv, beta, cash, vals);
+    rc = check_launch("tqf_lsm_price");
+    if (rc) return rc;
+  }
+  const int64_t first = l->num_calibration > 0 ? l->num_calibration : 0;
+  const int64_t count = n - first;
+  TQF_REQUIRE(count >= 1, "no pricing rows after the calibration samples");
+  const int64_t fb = (count + 255) / 256;
+  hipLaunchKernelGGL(lsm_final_kernel, blocks_for(count), 256, 0, s, cash,
+                     vals, df[1] / df[0], first, n, fsum);
+  hipLaunchKernelGGL(lsm_final_reduce_kernel, 1, 64, 0, s, fsum, fb, count,
+                     result);
+  return check_launch("tqf_lsm_price finalize");
+}

@@ -60,6 +60,26 @@ class Gbm(ctypes.Structure):
               ('s0_stride_path', _i64)]
 
 
+class Lsm(ctypes.Structure):
+  _fields_ = [('paths', _c_void_p), ('n_paths', _i64), ('n_times', _i32),
+              ('dim', _i32), ('payoff', _i32), ('degree', _i32),
+              ('n_basis', _i32), ('pad0', _i32), ('exponents', _c_void_p),
+              ('strike', _f64), ('num_calibration', _i64), ('rcond', _f64)]
+
+
+class Contract(ctypes.Structure):
+  _fields_ = [('kind', _i32), ('n_steps', _i32), ('stream_id', _u32),
+              ('pad0', _i32), ('n_paths', _i64),
+              ('s0', _f64), ('strike', _f64), ('maturity', _f64),
+              ('rate', _f64), ('sigma', _f64), ('v0', _f64), ('kappa', _f64),
+              ('theta', _f64), ('xi', _f64), ('rho', _f64)]
+
+
+class WorkItem(ctypes.Structure):
+  _fields_ = [('contract', _i32), ('pad0', _i32), ('first_path', _i64),
+              ('n_paths', _i64)]
+
+
 _SIGS = {
     'tqf_version': (_i32, []),
     'tqf_last_error': (ctypes.c_char_p, []),
@@ -80,6 +100,14 @@ _SIGS = {
     'tqf_gbm_exact': (_i32, [ctypes.POINTER(Gbm), ctypes.POINTER(Grid),
                              ctypes.POINTER(Rng), _i32, _c_void_p, _i64, _i32,
                              _c_void_p]),
+    'tqf_lsm_workspace_size': (_i64, [_i64, _i32]),
+    'tqf_batch_workspace_size': (_i64, [_i32, _i32, _i32]),
+    'tqf_batch_price': (_i32, [_c_void_p, _i32, _c_void_p, _c_void_p, _i32,
+                               _c_void_p, _c_void_p, _u64, _i32, _i32, _i32,
+                               _c_void_p, _c_void_p, _c_void_p]),
+    'tqf_lsm_price': (_i32, [ctypes.POINTER(Lsm), _c_void_p, _i32, _c_void_p,
+                             _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                             _c_void_p, _c_void_p]),
 }
 
 _lib = None

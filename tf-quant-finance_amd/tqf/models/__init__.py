@@ -1,9 +1,10 @@
 """Path-simulation models (mirror of tff.models on the simulation path)."""
 from tqf.models import euler_sampling
+from tqf.models import longstaff_schwartz
 from tqf.models import utils
 from tqf.models.geometric_brownian_motion import GeometricBrownianMotion
 from tqf.models.geometric_brownian_motion import MultivariateGeometricBrownianMotion
 from tqf.models.heston import HestonModel
 
-__all__ = ['euler_sampling', 'utils', 'GeometricBrownianMotion',
+__all__ = ['euler_sampling', 'longstaff_schwartz', 'utils', 'GeometricBrownianMotion',
            'MultivariateGeometricBrownianMotion', 'HestonModel']
