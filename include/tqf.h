@@ -247,6 +247,50 @@ int tqf_lsm_price(const tqf_lsm* lsm, const int32_t* ex_idx, int32_t n_ex,
                   const double* df, void* workspace, double* cash,
                   double* vals, double* result, int32_t* ranks, void* stream);
 
+/* Per-date building blocks of the same backward induction, for callers that
+ * supply their own continuation estimate (the MLP model below):
+ *   tqf_lsm_terminal     cash = payoff at t_idx, vals = 0
+ *   tqf_lsm_date_inputs  exv = payoff, y = df_ratio (cash + vals), centring
+ *                        moments (tqf_lsm_moments_size bytes), and MLP rows
+ *                        x[n][8] = (standardised S_1..S_dim, exv / K, 0...),
+ *                        tgt = y / K, w = 1{exv > 0}   (dim <= 5)
+ *   tqf_lsm_update_cont  exercise iff exv > K max(cont, 0) (lsm.py:381-400)
+ *   tqf_lsm_final        (mean, stderr) of df0 (cash + vals) over [first, n)
+ */
+int tqf_lsm_terminal(const tqf_lsm* lsm, int32_t t_idx, double* cash,
+                     double* vals, void* stream);
+int64_t tqf_lsm_moments_size(int64_t n_paths);
+int tqf_lsm_date_inputs(const tqf_lsm* lsm, int32_t t_idx, double df_ratio,
+                        const double* cash, const double* vals, double* exv,
+                        double* y, double* moments, float* x, float* tgt,
+                        float* w, void* stream);
+int tqf_lsm_update_cont(int64_t n_paths, double strike, double df_ratio,
+                        const double* exv, const float* cont, double* cash,
+                        double* vals, void* stream);
+int tqf_lsm_final(const double* cash, const double* vals, double df0,
+                  int64_t first, int64_t n, double* workspace, double* result,
+                  void* stream);
+
+/* ---- MLP continuation value (6 -> 64 -> 64 -> 1, SiLU) ------------------ */
+
+/* Parameters: float32 [4673] = W1[6][64] | b1[64] | W2[64][64] | b2[64] |
+ * w3[64] | b3.  Inputs x: [n][8] float32 rows (6 used).  The 64x64 layer
+ * uses bf16 MFMA with float32 accumulation.  No counterpart in the
+ * reference: this model stands in for the polynomial regression of
+ * least_square_mc (lsm.py:328-383), which stays the fallback. */
+int tqf_mlp_forward(const float* params, const float* x, int64_t n,
+                    float* out, void* stream);
+int64_t tqf_mlp_workspace_size(int32_t n_blocks);
+/* One Adam step on L = sum_s w_s (f(x_s) - t_s)^2 * inv_norm over rows
+ * [first, first + n).  lr = 0 computes the gradient only (grad_out
+ * [4673], loss_out [1] optional, device). */
+int tqf_mlp_train_step(float* params, float* adam_m, float* adam_v,
+                       const float* x, const float* targets,
+                       const float* weights, int64_t first, int64_t n,
+                       float inv_norm, float lr, int32_t step,
+                       int32_t n_blocks, void* workspace, float* grad_out,
+                       float* loss_out, void* stream);
+
 /* ---- Batched heterogeneous pricing with adjoint Greeks ----------------- */
 
 #define TQF_CONTRACT_GBM 0     /* log-Euler GBM: Greeks delta, vega, rho     */
@@ -293,6 +337,27 @@ int tqf_batch_price(const tqf_contract* contracts, int32_t n_contracts,
                     const int32_t* contract_n_items, uint64_t seed,
                     int32_t n_blocks, int32_t max_steps, int32_t use_counter,
                     void* workspace, double* out, void* stream);
+
+/* ---- Basket call with adjoint Greeks ------------------------------------ */
+
+int64_t tqf_basket_workspace_size(int32_t dim, int32_t n_blocks);
+
+/* Device.  Arithmetic-average basket call e^{-rT} (sum_i w_i S_i(T) - K)^+
+ * on correlated GBM (exact scheme over n_steps uniform steps, Philox
+ * normals: key = seed, sub-stream stream_id, draw j = step * dim + asset),
+ * with pathwise adjoint Greeks when greeks != 0.
+ * params: device float32 [s0[dim], sigma[dim], weight[dim], chol[dim][dim]]
+ * (dim a multiple of 4, <= 16; pad assets with zero weight).
+ * out: device [2 * (2 + 2 dim)] doubles: means of (price, rho, delta[dim],
+ * vega[dim]) then their standard errors.  Replaces
+ * MultivariateGeometricBrownianMotion.sample_paths (tff/models/
+ * geometric_brownian_motion/multivariate_geometric_brownian_motion.py:
+ * 183-282) plus tf.GradientTape Greeks of the payoff mean. */
+int tqf_basket_call(const float* params, int32_t dim, int32_t n_steps,
+                    double maturity, double rate, double strike,
+                    int64_t n_paths, uint64_t seed, uint32_t stream_id,
+                    int32_t greeks, int32_t n_blocks, void* workspace,
+                    double* out, void* stream);
 
 #ifdef __cplusplus
 }

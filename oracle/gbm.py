@@ -38,3 +38,45 @@ def mv_sample_paths(times, means, vols, corr, s0, draws, dtype=np.float64):
     z = np.einsum('ij,nkj->nki', chol, z)
   inc = (means - vols**2 / 2) * dt + np.sqrt(dt) * vols * z
   return (np.asarray(s0, dtype) * np.exp(np.cumsum(inc, axis=1))).astype(dtype)
+
+
+def basket_call_paths(s0, sigma, corr, r, t, strike, z, weights=None):
+  """Per-path discounted basket-call payoff and its pathwise derivatives.
+
+  z: normals [N, steps, d] (the exact correlated scheme of
+  multivariate_geometric_brownian_motion.py:229-282 over uniform steps).
+  Returns dict of arrays: pay [N], delta [N, d], vega [N, d], rho [N].
+  Pinned by central finite differences of `basket_call_price` with common
+  random numbers (tests/test_oracle.py).
+  """
+  s0 = np.asarray(s0, np.float64)
+  sigma = np.asarray(sigma, np.float64)
+  d = s0.shape[0]
+  w = np.full(d, 1.0 / d) if weights is None else np.asarray(weights)
+  n, steps, _ = z.shape
+  dt = t / steps
+  chol = np.linalg.cholesky(np.asarray(corr, np.float64))
+  wv = np.sqrt(dt) * np.einsum('ij,nj->ni', chol, z.sum(axis=1))
+  s = s0 * np.exp((r - 0.5 * sigma**2) * t + sigma * wv)
+  a = s @ w
+  disc = np.exp(-r * t)
+  itm = (a > strike).astype(np.float64)
+  pay = disc * itm * (a - strike)
+  return {'pay': pay,
+          'delta': disc * itm[:, None] * w * s / s0,
+          'vega': disc * itm[:, None] * w * s * (wv - sigma * t),
+          'rho': -t * pay + disc * itm * t * a}
+
+
+def basket_call_price(s0, sigma, corr, r, t, strike, z, weights=None):
+  return basket_call_paths(s0, sigma, corr, r, t, strike, z,
+                           weights)['pay'].mean()
+
+
+def gbm_log_euler_call(s0, sigma, r, t, strike, z):
+  """Discounted call payoffs of log-Euler GBM over uniform steps; z [N, steps]."""
+  steps = z.shape[1]
+  dt = t / steps
+  x = np.log(s0) + np.sum((r - 0.5 * sigma**2) * dt + sigma * np.sqrt(dt) * z,
+                          axis=1)
+  return np.exp(-r * t) * np.maximum(np.exp(x) - strike, 0)

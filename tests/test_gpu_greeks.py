@@ -1,0 +1,166 @@
+"""GPU batched pricing and adjoint Greeks against closed forms, the CPU
+oracle with common random numbers, and central finite differences."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import closed_form, gbm as ogbm, heston as oheston, rng as orng
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+  from tqf.engine import batch as batch_lib
+  from tqf.engine import greeks
+
+
+def _gbm(i, s0=100., k=100., t=1.0, sigma=0.25, steps=50, paths=2**16):
+  return dict(kind=batch_lib.KIND_GBM, n_steps=steps, n_paths=paths, s0=s0,
+              strike=k, maturity=t, rate=0.03, sigma=sigma, v0=0., kappa=0.,
+              theta=0., xi=0., rho=0., stream_id=i)
+
+
+def _heston(i, s0=100., k=100., t=1.0, steps=60, paths=4096, v0=0.04,
+            kappa=1.5, theta=0.05, xi=0.5, rho=-0.6):
+  return dict(kind=batch_lib.KIND_HESTON, n_steps=steps, n_paths=paths, s0=s0,
+              strike=k, maturity=t, rate=0.03, sigma=0., v0=v0, kappa=kappa,
+              theta=theta, xi=xi, rho=rho, stream_id=i)
+
+
+def test_batch_gbm_prices_and_greeks_vs_black_scholes():
+  cs = [_gbm(i, s0=80 + 10 * i, k=100., t=0.5 + 0.25 * i,
+             sigma=0.15 + 0.05 * i) for i in range(6)]
+  plan = batch_lib.BatchPlan(cs, tile=1024)
+  plan.launch()
+  res = plan.results()
+  for c, r in zip(cs, res):
+    args = (c['s0'], c['strike'], c['maturity'], c['rate'], c['sigma'])
+    for name, fn in (('price', closed_form.bs_price),
+                     ('delta', closed_form.bs_delta),
+                     ('vega', closed_form.bs_vega),
+                     ('rho', closed_form.bs_rho)):
+      m, se = r[name]
+      assert abs(m - fn(*args)) <= 4 * se + 1e-4 * abs(fn(*args)), (name, m,
+                                                                     fn(*args),
+                                                                     se)
+
+
+def _oracle_heston(c, z, bump=None, h=0.0):
+  p = dict(c)
+  if bump is not None:
+    p[bump] = p[bump] + h
+  n = c['n_steps']
+  t = np.linspace(0, p['maturity'], n + 1)
+  keep = np.zeros(n + 1, bool)
+  keep[-1] = True
+  pr, _ = oheston.ft_euler_calls(np.log(p['s0']), p['v0'], p['kappa'],
+                                 p['theta'], p['xi'], p['rho'], p['rate'],
+                                 [p['strike']], t, keep, z)
+  return pr[-1, 0]
+
+
+def test_batch_heston_price_and_adjoint_vs_oracle_fd():
+  cs = [_heston(0), _heston(1, k=110., xi=0.8, rho=-0.3, steps=45),
+        _heston(2, k=90., v0=0.09, kappa=3.0, steps=33)]
+  plan = batch_lib.BatchPlan(cs, tile=1024, seed=5)
+  plan.launch()
+  res = plan.results()
+  for c, r in zip(cs, res):
+    z = orng.normals(c['n_paths'], 2 * c['n_steps'], 5,
+                     stream=c['stream_id'], dtype=np.float32)
+    z = z.reshape(c['n_paths'], c['n_steps'], 2)
+    price = _oracle_heston(c, z)
+    assert abs(r['price'][0] - price) <= 2e-4 * price + 1e-4
+    for name, field, h in (('delta', 's0', 1e-4), ('dv0', 'v0', 1e-6),
+                           ('rho', 'rate', 1e-6), ('dkappa', 'kappa', 1e-5),
+                           ('dtheta', 'theta', 1e-6), ('dxi', 'xi', 1e-5),
+                           ('drho', 'rho', 1e-5)):
+      fd = (_oracle_heston(c, z, field, h) - _oracle_heston(c, z, field, -h)) / (
+          2 * h)
+      got = r[name][0]
+      scale = max(abs(fd), 1e-2 * abs(price))
+      assert abs(got - fd) <= 3e-3 * scale + 3 * r[name][1] * 0.05, (
+          name, got, fd)
+
+
+def test_batch_equals_loop_of_single_contracts():
+  cs = [_gbm(0), _heston(1), _gbm(2, sigma=0.4, steps=77, paths=5000),
+        _heston(3, steps=121, paths=3000)]
+  plan = batch_lib.BatchPlan(cs, tile=512, seed=9)
+  out = plan.launch().cpu().numpy()
+  for i, c in enumerate(cs):
+    one = batch_lib.BatchPlan([c], tile=512, seed=9).launch().cpu().numpy()
+    np.testing.assert_allclose(out[i], one[0], rtol=1e-6, atol=1e-9)
+
+
+def test_batch_graph_replay_matches_eager():
+  cs = [_gbm(i, steps=40 + i, paths=3000 + 100 * i) for i in range(3)] + [
+      _heston(3, paths=2000)]
+  plan = batch_lib.BatchPlan(cs, tile=512, seed=4)
+  eager = plan.launch().clone()
+  g = plan.capture()
+  for _ in range(3):
+    plan.out.zero_()
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    torch.testing.assert_close(plan.out, eager, rtol=1e-7, atol=1e-12)
+    _ = torch.ones(1000, device=plan.dev).sum()  # eager work between replays
+
+
+def test_basket_adjoint_vs_oracle_pathwise_and_fd():
+  d, steps, n = 16, 64, 2**14
+  rs = np.random.RandomState(3)
+  a = rs.standard_normal((d, d))
+  c = a @ a.T + np.eye(d)
+  corr = c / np.sqrt(np.outer(np.diag(c), np.diag(c)))
+  sig = rs.uniform(0.15, 0.45, d)
+  s0 = np.full(d, 100.0)
+  res = greeks.basket_call_greeks(s0, sig, corr, 0.02, 1.0, 100.0, steps, n,
+                                  seed=3)
+  z = orng.normals(n, steps * d, 3, stream=0,
+                   dtype=np.float32).reshape(n, steps, d)
+  p = ogbm.basket_call_paths(s0, sig, corr, 0.02, 1.0, 100.0, z)
+  np.testing.assert_allclose(res['price'][0], p['pay'].mean(), rtol=2e-4)
+  np.testing.assert_allclose(res['rho'][0], p['rho'].mean(), rtol=2e-4)
+  np.testing.assert_allclose(res['delta'][0], p['delta'].mean(0), rtol=5e-4,
+                             atol=1e-6)
+  np.testing.assert_allclose(res['vega'][0], p['vega'].mean(0), rtol=5e-4,
+                             atol=1e-4)
+  # central differences with common random numbers (h = 1e-4 relative)
+  for i in (0, 7, 15):
+    e = np.zeros(d)
+    e[i] = 1e-2
+    fd = (ogbm.basket_call_price(s0 + e, sig, corr, .02, 1., 100., z) -
+          ogbm.basket_call_price(s0 - e, sig, corr, .02, 1., 100., z)) / 2e-2
+    np.testing.assert_allclose(res['delta'][0][i], fd, rtol=2e-3)
+
+
+def test_autograd_functions_return_adjoint_greeks():
+  s0 = torch.tensor(100.0, dtype=torch.float64, requires_grad=True)
+  v0 = torch.tensor(0.04, dtype=torch.float64, requires_grad=True)
+  r = torch.tensor(0.03, dtype=torch.float64, requires_grad=True)
+  k, th, xi, rho = (torch.tensor(v, dtype=torch.float64, requires_grad=True)
+                    for v in (1.5, 0.05, 0.5, -0.6))
+  cfg = dict(n_steps=60, strike=100.0, maturity=1.0, n_paths=4096, seed=5,
+             stream_id=0)
+  price = greeks.HestonCall.apply(s0, v0, r, k, th, xi, rho, cfg)
+  grads = torch.autograd.grad(price, [s0, v0, r, k, th, xi, rho])
+  plan = batch_lib.BatchPlan([_heston(0)], tile=1024, seed=5)
+  direct = plan.launch()[0, 1:8].cpu().numpy()
+  np.testing.assert_allclose([g.item() for g in grads], direct, rtol=1e-12)
+  # basket
+  d = 4
+  s = torch.full((d,), 100.0, dtype=torch.float64, requires_grad=True)
+  sg = torch.tensor([0.2, 0.25, 0.3, 0.35], dtype=torch.float64,
+                    requires_grad=True)
+  rr = torch.tensor(0.02, dtype=torch.float64, requires_grad=True)
+  corr = 0.5 * np.eye(d) + 0.5
+  p = greeks.basket_call(s, sg, rr, corr=corr, maturity=1.0, strike=100.0,
+                         n_steps=16, n_paths=2**15, seed=1)
+  gs, gsig, gr = torch.autograd.grad(p, [s, sg, rr])
+  ref = greeks.basket_call_greeks(s.detach().cpu().numpy(),
+                                  sg.detach().cpu().numpy(), corr, 0.02, 1.0,
+                                  100.0, 16, 2**15, seed=1)
+  np.testing.assert_allclose(gs.cpu().numpy(), ref['delta'][0], rtol=1e-12)
+  np.testing.assert_allclose(gsig.cpu().numpy(), ref['vega'][0], rtol=1e-12)
+  np.testing.assert_allclose(gr.item(), ref['rho'][0], rtol=1e-12)

@@ -318,3 +318,30 @@ def test_black_scholes_parity_and_greeks():
         closed_form.bs_price(100, 95, .7, .03 - h, .25)) / (2 * h)
   np.testing.assert_allclose(closed_form.bs_rho(100, 95, .7, .03, .25), fd,
                              rtol=1e-6)
+
+
+def test_basket_pathwise_greeks_match_finite_differences():
+  rs = np.random.RandomState(6)
+  d = 4
+  a = rs.standard_normal((d, d))
+  c = a @ a.T + np.eye(d)
+  corr = c / np.sqrt(np.outer(np.diag(c), np.diag(c)))
+  s0 = np.full(d, 100.0)
+  sig = rs.uniform(0.15, 0.45, d)
+  z = rs.standard_normal((20000, 8, d))
+  p = gbm.basket_call_paths(s0, sig, corr, 0.02, 1.0, 100.0, z)
+  h = 1e-6
+  for i in range(d):
+    e = np.zeros(d)
+    e[i] = h * 100
+    fd = (gbm.basket_call_price(s0 + e, sig, corr, .02, 1., 100., z) -
+          gbm.basket_call_price(s0 - e, sig, corr, .02, 1., 100., z)) / (2 * e[i])
+    np.testing.assert_allclose(p['delta'][:, i].mean(), fd, rtol=1e-5)
+    e = np.zeros(d)
+    e[i] = h
+    fd = (gbm.basket_call_price(s0, sig + e, corr, .02, 1., 100., z) -
+          gbm.basket_call_price(s0, sig - e, corr, .02, 1., 100., z)) / (2 * h)
+    np.testing.assert_allclose(p['vega'][:, i].mean(), fd, rtol=1e-5)
+  fd = (gbm.basket_call_price(s0, sig, corr, .02 + h, 1., 100., z) -
+        gbm.basket_call_price(s0, sig, corr, .02 - h, 1., 100., z)) / (2 * h)
+  np.testing.assert_allclose(p['rho'].mean(), fd, rtol=1e-5)

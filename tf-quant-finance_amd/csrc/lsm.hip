@@ -21,6 +21,7 @@
 // prod_j c_j^{e[k][j]} with the reference's exponent order.  Scaling a
 // column by a constant does not change the fitted values.
 #include "common.h"
+#include "mlp.h"
 
 using namespace tqf;
 
@@ -411,6 +412,44 @@ __global__ void lsm_final_reduce_kernel(const double* __restrict__ block_sums,
   out[1] = sqrt((var > 0 ? var : 0.0) / n);
 }
 
+// MLP inputs of one date: standardised asset prices and exercise value / K
+// (row stride kMlpStride), regression target y / K and ITM weight.
+__global__ void __launch_bounds__(256)
+lsm_features_kernel(LsmArgs a, const double* __restrict__ exv,
+                    const double* __restrict__ y, float* __restrict__ x,
+                    float* __restrict__ tgt, float* __restrict__ w) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.n_paths) return;
+  const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
+  float* xr = x + p * kMlpStride;
+  for (int j = 0; j < kMlpIn - 1; ++j)
+    xr[j] = j < a.d ? (float)((row[j] - a.mean[j]) * a.scale[j]) : 0.f;
+  xr[kMlpIn - 1] = (float)(exv[p] / a.strike);
+  xr[6] = xr[7] = 0.f;
+  tgt[p] = (float)(y[p] / a.strike);
+  w[p] = exv[p] > 0 ? 1.f : 0.f;
+}
+
+// Exercise update from a given continuation estimate (in units of K).
+__global__ void __launch_bounds__(256)
+lsm_update_cont_kernel(int64_t n, double strike, double df_ratio,
+                       const double* __restrict__ exv,
+                       const float* __restrict__ cont,
+                       double* __restrict__ cash, double* __restrict__ vals) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  double c = (double)cont[p] * strike;
+  c = c > 0 ? c : 0;
+  const double ex = exv[p];
+  if (ex > c) {
+    vals[p] = 0;
+    cash[p] = ex;
+  } else {
+    vals[p] = df_ratio * (cash[p] + vals[p]);
+    cash[p] = 0;
+  }
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 LsmArgs make_args(const tqf_lsm* l, int t_idx) {
@@ -565,4 +604,83 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
   hipLaunchKernelGGL(lsm_final_reduce_kernel, 1, 64, 0, s, fsum, fb, count,
                      result);
   return check_launch("tqf_lsm_price finalize");
+}
+
+// ---- per-date building blocks (used by the MLP continuation model) --------
+
+extern "C" int tqf_lsm_terminal(const tqf_lsm* l, int32_t t_idx, double* cash,
+                                double* vals, void* stream) {
+  clear_error();
+  int rc = validate(l);
+  if (rc) return rc;
+  TQF_REQUIRE(t_idx >= 0 && t_idx < l->n_times, "time index out of range");
+  TQF_REQUIRE(cash && vals, "null state");
+  LsmArgs a = make_args(l, t_idx);
+  hipLaunchKernelGGL(lsm_terminal_kernel, blocks_for(l->n_paths), 256, 0,
+                     (hipStream_t)stream, a, cash, vals);
+  return check_launch("tqf_lsm_terminal");
+}
+
+extern "C" int tqf_lsm_date_inputs(const tqf_lsm* l, int32_t t_idx,
+                                   double df_ratio, const double* cash,
+                                   const double* vals, double* exv, double* y,
+                                   double* moments, float* x, float* tgt,
+                                   float* w, void* stream) {
+  clear_error();
+  int rc = validate(l);
+  if (rc) return rc;
+  TQF_REQUIRE(t_idx >= 0 && t_idx < l->n_times, "time index out of range");
+  TQF_REQUIRE(l->dim <= kMlpIn - 1, "the MLP takes at most %d assets",
+              kMlpIn - 1);
+  TQF_REQUIRE(cash && vals && exv && y && moments && x && tgt && w,
+              "null argument");
+  const hipStream_t s = (hipStream_t)stream;
+  const int64_t n = l->n_paths;
+  const int64_t nb = (n + 255) / 256;
+  // moments: [mean 8 | scale 8 | block sums nb * 16]
+  double* mean = moments;
+  double* scale = moments + kMaxAssets;
+  double* bsum = moments + 2 * kMaxAssets;
+  LsmArgs a = make_args(l, t_idx);
+  hipLaunchKernelGGL(lsm_prepare_kernel, blocks_for(n), 256, 0, s, a, df_ratio,
+                     cash, vals, exv, y, bsum);
+  hipLaunchKernelGGL(lsm_mean_kernel, 1, 64, 0, s, bsum, nb, a.d, n, mean,
+                     scale);
+  a.mean = mean;
+  a.scale = scale;
+  hipLaunchKernelGGL(lsm_features_kernel, blocks_for(n), 256, 0, s, a, exv, y,
+                     x, tgt, w);
+  return check_launch("tqf_lsm_date_inputs");
+}
+
+extern "C" int64_t tqf_lsm_moments_size(int64_t n_paths) {
+  return (2 * kMaxAssets + ((n_paths + 255) / 256) * 2 * kMaxAssets) * 8;
+}
+
+extern "C" int tqf_lsm_update_cont(int64_t n_paths, double strike,
+                                   double df_ratio, const double* exv,
+                                   const float* cont, double* cash,
+                                   double* vals, void* stream) {
+  clear_error();
+  TQF_REQUIRE(n_paths >= 1 && exv && cont && cash && vals, "bad argument");
+  hipLaunchKernelGGL(lsm_update_cont_kernel, blocks_for(n_paths), 256, 0,
+                     (hipStream_t)stream, n_paths, strike, df_ratio, exv, cont,
+                     cash, vals);
+  return check_launch("tqf_lsm_update_cont");
+}
+
+extern "C" int tqf_lsm_final(const double* cash, const double* vals,
+                             double df0, int64_t first, int64_t n,
+                             double* workspace, double* result, void* stream) {
+  clear_error();
+  TQF_REQUIRE(cash && vals && workspace && result && n > first && first >= 0,
+              "bad argument");
+  const int64_t count = n - first;
+  const int64_t fb = (count + 255) / 256;
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(lsm_final_kernel, blocks_for(count), 256, 0, s, cash,
+                     vals, df0, first, n, workspace);
+  hipLaunchKernelGGL(lsm_final_reduce_kernel, 1, 64, 0, s, workspace, fb,
+                     count, result);
+  return check_launch("tqf_lsm_final");
 }

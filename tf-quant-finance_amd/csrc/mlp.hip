@@ -1,0 +1,434 @@
+// Small MLP (6 -> 64 -> 64 -> 1, SiLU) for the Longstaff-Schwartz
+// continuation value: forward, backward and Adam as hand-written kernels.
+//
+// The 64x64 layer runs on MFMA (v_mfma_f32_16x16x16_bf16: bf16 operands,
+// float32 accumulation); the 6->64 and 64->1 layers are VALU.  One wave
+// handles 16 samples per MFMA group and keeps the whole layer-2 weight
+// block as MFMA operands in registers:
+//   A operand (16 x 16, K chunk c): lane l holds row l % 16, k = c*16 +
+//     4*(l/16) + {0..3};   B operand: column l % 16, same k;
+//   C (16 x 16 float32): lane l holds column l % 16, rows 4*(l/16) + i.
+// So the VALU layer-1 output of lane l (sample l % 16, features
+// c*16 + 4*(l/16) + j) is already the A operand of layer 2, and the layer-2
+// output in C layout is already the B operand of the weight-gradient
+// product dW2 = H1^T dZ2.  Two small LDS transposes remain (H1^T and dZ2 as
+// an A operand).
+//
+// Parameter vector layout (float32, kParams = 4673):
+//   W1[6][64] | b1[64] | W2[64][64] (in, out) | b2[64] | w3[64] | b3
+#include "common.h"
+#include "mlp.h"
+
+namespace tqf {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = kMlpIn, kH = 64;
+constexpr int oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH,
+              oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH;
+static_assert(oB3 + 1 == kMlpParams, "parameter layout");
+
+__device__ __forceinline__ short to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (short)(u >> 16);
+}
+
+__device__ __forceinline__ float silu(float z) {
+  return z / (1.0f + __expf(-z));
+}
+__device__ __forceinline__ float silu_grad(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Orders this wave's LDS writes and reads (the transpose buffers are private
+// to a wave; the waves of a block run different numbers of groups, so no
+// workgroup barrier may be used inside the group loop).  LDS instructions of
+// one wave execute in order; this keeps the compiler from moving them.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum over the 16 lanes sharing l / 16 (lanes differing in bits 0..3).
+__device__ __forceinline__ float sum16(float v) {
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  v += __shfl_xor(v, 8, 64);
+  return v;
+}
+
+// Forward for one group.  Returns z1 (A layout, 16), h1 bf16 A operands,
+// z2 (C layout, 16), y (4 samples of the lane's row block).
+struct Fwd {
+  float z1a[4][4];    // [chunk c][j]: sample l%16, feature c*16+4*(l/16)+j
+  bf16x4 h1a[4];      // A operands of layer 2
+  f32x4 z2[4];        // [n-tile t]: rows 4*(l/16)+i, column t*16 + l%16
+  float y[4];         // outputs of rows 4*(l/16)+i
+};
+
+__device__ __forceinline__ void forward_group(const float* __restrict__ P,
+                                              const bf16x4* w2b, const float* xa,
+                                              Fwd& f) {
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, r = lane & 15;
+  // layer 1 (VALU): features c*16 + 4q + j of sample r
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    short h[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int fo = c * 16 + 4 * q + j;
+      float z = P[oB1 + fo];
+#pragma unroll
+      for (int i = 0; i < kIn; ++i) z += xa[i] * P[oW1 + i * kH + fo];
+      f.z1a[c][j] = z;
+      h[j] = to_bf16(silu(z));
+    }
+    f.h1a[c] = bf16x4{h[0], h[1], h[2], h[3]};
+  }
+  // layer 2 (MFMA): Z2[16 x 64] = H1[16 x 64] W2[64 x 64]
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc = mfma(f.h1a[c], w2b[t * 4 + c], acc);
+    const float bb = P[oB2 + t * 16 + r];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] += bb;
+    f.z2[t] = acc;
+  }
+  // layer 3 (VALU + 16-lane sums): y for rows 4q + i
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float s = 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) s += silu(f.z2[t][i]) * P[oW3 + t * 16 + r];
+    f.y[i] = sum16(s) + P[oB3];
+  }
+}
+
+// W2 as layer-2 B operands: tile (t, c): column t*16 + r, rows
+// c*16 + 4q + {0..3}.  W2 is stored [in][out].
+__device__ __forceinline__ void load_w2(const float* __restrict__ P,
+                                        bf16x4* w2b) {
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      short v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        v[j] = to_bf16(P[oW2 + (c * 16 + 4 * q + j) * kH + t * 16 + r]);
+      w2b[t * 4 + c] = bf16x4{v[0], v[1], v[2], v[3]};
+    }
+}
+
+// W2^T as B operands of dH1 = dZ2 W2^T: tile (t, c) over (features f =
+// t*16 + r, k = n = c*16 + 4q + j): value W2[f][n].
+__device__ __forceinline__ void load_w2t(const float* __restrict__ P,
+                                         bf16x4* w2t) {
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      short v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        v[j] = to_bf16(P[oW2 + (t * 16 + r) * kH + c * 16 + 4 * q + j]);
+      w2t[t * 4 + c] = bf16x4{v[0], v[1], v[2], v[3]};
+    }
+}
+
+__global__ void __launch_bounds__(256)
+mlp_forward_kernel(const float* __restrict__ params,
+                   const float* __restrict__ x, int64_t n,
+                   float* __restrict__ out) {
+  __shared__ float P[kMlpParams];
+  for (int e = threadIdx.x; e < kMlpParams; e += 256) P[e] = params[e];
+  __syncthreads();
+  bf16x4 w2b[16];
+  load_w2(P, w2b);
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  const int64_t wid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  for (int64_t g = wid; g * 16 < n; g += waves) {
+    const int64_t base = g * 16;
+    float xa[kIn];
+    const int64_t sa = base + r;
+#pragma unroll
+    for (int i = 0; i < kIn; ++i)
+      xa[i] = sa < n ? x[sa * kMlpStride + i] : 0.f;
+    Fwd f;
+    forward_group(P, w2b, xa, f);
+    if (r == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t s = base + 4 * q + i;
+        if (s < n) out[s] = f.y[i];
+      }
+    }
+  }
+}
+
+// Gradient of L = sum_s w_s (y_s - t_s)^2 / norm over [0, n), per-block
+// partial sums of the kMlpParams gradient entries (and the loss at the end).
+__global__ void __launch_bounds__(256)
+mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
+                const float* __restrict__ tgt, const float* __restrict__ wts,
+                int64_t first, int64_t n, float inv_norm,
+                float* __restrict__ partial) {
+  __shared__ float P[kMlpParams];
+  __shared__ float G[kMlpParams + 1];
+  __shared__ short tr[4][16 * 64];  // per-wave transpose buffer (bf16)
+  for (int e = threadIdx.x; e < kMlpParams; e += 256) {
+    P[e] = params[e];
+    G[e] = 0.f;
+  }
+  if (threadIdx.x == 0) G[kMlpParams] = 0.f;
+  __syncthreads();
+  bf16x4 w2b[16], w2t[16];
+  load_w2(P, w2b);
+  load_w2t(P, w2t);
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  const int wave = threadIdx.x >> 6;
+  short* T = tr[wave];
+  f32x4 dw2[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) dw2[e] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float dw1[kIn][4], db1[4], db2[4], dw3[4], db3 = 0.f, loss = 0.f;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    db1[t] = db2[t] = dw3[t] = 0.f;
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) dw1[i][t] = 0.f;
+  }
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  const int64_t wid = (int64_t)blockIdx.x * 4 + wave;
+  for (int64_t g = wid; g * 16 < n; g += waves) {
+    const int64_t base = first + g * 16;
+    const int64_t end = first + n;
+    float xa[kIn];
+    const int64_t sa = base + r;
+#pragma unroll
+    for (int i = 0; i < kIn; ++i)
+      xa[i] = sa < end ? x[sa * kMlpStride + i] : 0.f;
+    Fwd f;
+    forward_group(P, w2b, xa, f);
+    // dy for the lane's rows
+    float dy[4], xr[4][kIn];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t s = base + 4 * q + i;
+      float w = 0.f, t = 0.f;
+      if (s < end) {
+        w = wts[s];
+        t = tgt[s];
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) xr[i][k] = x[s * kMlpStride + k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) xr[i][k] = 0.f;
+      }
+      const float e = f.y[i] - t;
+      dy[i] = 2.f * w * e * inv_norm;
+      if (r == 0) loss += w * e * e * inv_norm;
+      db3 += r == 0 ? dy[i] : 0.f;
+    }
+    // layer 3 and SiLU' of layer 2: dZ2 in C layout (rows 4q+i, col t*16+r)
+    f32x4 dz2[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float w3 = P[oW3 + t * 16 + r];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float z = f.z2[t][i];
+        dw3[t] += silu(z) * dy[i];
+        const float d = dy[i] * w3 * silu_grad(z);
+        dz2[t][i] = d;
+        db2[t] += d;
+      }
+    }
+    // dW2 += H1^T dZ2: A = H1^T tile (features mt*16 + r, samples 4q + j)
+    // from the transpose of the A-layout h1 via LDS.
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        T[r * 64 + c * 16 + 4 * q + j] = f.h1a[c][j];  // [sample][feature]
+    wave_lds_sync();
+    bf16x4 h1t[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      short v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = T[(4 * q + j) * 64 + mt * 16 + r];
+      h1t[mt] = bf16x4{v[0], v[1], v[2], v[3]};
+    }
+    bf16x4 dzb[4];  // dZ2 as B operand: col t*16+r, k (samples) 4q + i
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      dzb[t] = bf16x4{to_bf16(dz2[t][0]), to_bf16(dz2[t][1]),
+                      to_bf16(dz2[t][2]), to_bf16(dz2[t][3])};
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) dw2[mt * 4 + t] = mfma(h1t[mt], dzb[t], dw2[mt * 4 + t]);
+    // dH1 = dZ2 W2^T: A = dZ2 (rows = samples r, k = n c*16 + 4q + j) via LDS
+    wave_lds_sync();
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        T[(4 * q + i) * 64 + t * 16 + r] = dzb[t][i];  // [sample][n]
+    wave_lds_sync();
+    bf16x4 dza[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      short v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = T[r * 64 + c * 16 + 4 * q + j];
+      dza[c] = bf16x4{v[0], v[1], v[2], v[3]};
+    }
+    wave_lds_sync();
+    // dH1 tiles (C layout): rows (samples) 4q + i, features t*16 + r
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      f32x4 dh = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) dh = mfma(dza[c], w2t[t * 4 + c], dh);
+      const int fo = t * 16 + r;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        // recompute z1 for (sample 4q+i, feature fo)
+        float z = P[oB1 + fo];
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) z += xr[i][k] * P[oW1 + k * kH + fo];
+        const float d = dh[i] * silu_grad(z);
+        db1[t] += d;
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) dw1[k][t] += xr[i][k] * d;
+      }
+    }
+  }
+  // block reduction into LDS (atomics on LDS floats), then one row out
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        atomicAdd(&G[oW2 + (mt * 16 + 4 * q + i) * kH + t * 16 + r],
+                  dw2[mt * 4 + t][i]);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int fo = t * 16 + r;
+    atomicAdd(&G[oB1 + fo], db1[t]);
+    atomicAdd(&G[oB2 + fo], db2[t]);
+    atomicAdd(&G[oW3 + fo], dw3[t]);
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) atomicAdd(&G[oW1 + k * kH + fo], dw1[k][t]);
+  }
+  if (r == 0) {
+    atomicAdd(&G[oB3], db3);
+    atomicAdd(&G[kMlpParams], loss);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kMlpParams + 1; e += 256)
+    partial[(int64_t)blockIdx.x * (kMlpParams + 1) + e] = G[e];
+}
+
+__global__ void __launch_bounds__(256)
+mlp_adam_kernel(float* __restrict__ params, float* __restrict__ m,
+                float* __restrict__ v, const float* __restrict__ partial,
+                int n_blocks, float lr, float b1, float b2, float eps,
+                int step, float* __restrict__ grad_out,
+                float* __restrict__ loss_out) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e > kMlpParams) return;
+  float g = 0.f;
+  for (int b = 0; b < n_blocks; ++b) g += partial[(int64_t)b * (kMlpParams + 1) + e];
+  if (e == kMlpParams) {
+    if (loss_out) *loss_out = g;
+    return;
+  }
+  if (grad_out) grad_out[e] = g;
+  if (lr == 0.f) return;  // gradient only
+  const float mm = b1 * m[e] + (1.f - b1) * g;
+  const float vv = b2 * v[e] + (1.f - b2) * g * g;
+  m[e] = mm;
+  v[e] = vv;
+  const float mh = mm / (1.f - __powf(b1, (float)step));
+  const float vh = vv / (1.f - __powf(b2, (float)step));
+  params[e] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+}  // namespace
+
+int mlp_forward(const float* params, const float* x, int64_t n, float* out,
+                hipStream_t s) {
+  const int64_t groups = (n + 15) / 16;
+  int blocks = (int)((groups + 3) / 4);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(mlp_forward_kernel, blocks, 256, 0, s, params, x, n, out);
+  return check_launch("mlp_forward");
+}
+
+int mlp_grad_step(float* params, float* m, float* v, const float* x,
+                  const float* tgt, const float* w, int64_t first, int64_t n,
+                  float inv_norm, float lr, int step, float* partial,
+                  int n_blocks, float* grad_out, float* loss_out,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(mlp_grad_kernel, n_blocks, 256, 0, s, params, x, tgt, w,
+                     first, n, inv_norm, partial);
+  hipLaunchKernelGGL(mlp_adam_kernel, (kMlpParams + 1 + 255) / 256, 256, 0, s,
+                     params, m, v, partial, n_blocks, lr, 0.9f, 0.999f, 1e-8f,
+                     step, grad_out, loss_out);
+  return check_launch("mlp_grad_step");
+}
+
+}  // namespace tqf
+
+using namespace tqf;
+
+extern "C" int tqf_mlp_forward(const float* params, const float* x, int64_t n,
+                               float* out, void* stream) {
+  clear_error();
+  TQF_REQUIRE(params && x && out, "null argument");
+  TQF_REQUIRE(n >= 0, "n must be non-negative");
+  if (n == 0) return TQF_OK;
+  return mlp_forward(params, x, n, out, (hipStream_t)stream);
+}
+
+extern "C" int64_t tqf_mlp_workspace_size(int32_t n_blocks) {
+  return (int64_t)n_blocks * (kMlpParams + 1) * 4;
+}
+
+extern "C" int tqf_mlp_train_step(float* params, float* adam_m, float* adam_v,
+                                  const float* x, const float* targets,
+                                  const float* weights, int64_t first,
+                                  int64_t n, float inv_norm, float lr,
+                                  int32_t step, int32_t n_blocks,
+                                  void* workspace, float* grad_out,
+                                  float* loss_out, void* stream) {
+  clear_error();
+  TQF_REQUIRE(params && x && targets && weights && workspace, "null argument");
+  TQF_REQUIRE(lr == 0.f || (adam_m && adam_v), "Adam state is null");
+  TQF_REQUIRE(n >= 1 && first >= 0 && n_blocks >= 1 && step >= 1,
+              "bad sizes");
+  return mlp_grad_step(params, adam_m, adam_v, x, targets, weights, first, n,
+                       inv_norm, lr, step, (float*)workspace, n_blocks,
+                       grad_out, loss_out, (hipStream_t)stream);
+}

@@ -101,7 +101,30 @@ _SIGS = {
                              ctypes.POINTER(Rng), _i32, _c_void_p, _i64, _i32,
                              _c_void_p]),
     'tqf_lsm_workspace_size': (_i64, [_i64, _i32]),
+    'tqf_lsm_terminal': (_i32, [ctypes.POINTER(Lsm), _i32, _c_void_p,
+                                _c_void_p, _c_void_p]),
+    'tqf_lsm_moments_size': (_i64, [_i64]),
+    'tqf_lsm_date_inputs': (_i32, [ctypes.POINTER(Lsm), _i32, _f64, _c_void_p,
+                                   _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                                   _c_void_p, _c_void_p, _c_void_p,
+                                   _c_void_p]),
+    'tqf_lsm_update_cont': (_i32, [_i64, _f64, _f64, _c_void_p, _c_void_p,
+                                   _c_void_p, _c_void_p, _c_void_p]),
+    'tqf_lsm_final': (_i32, [_c_void_p, _c_void_p, _f64, _i64, _i64,
+                             _c_void_p, _c_void_p, _c_void_p]),
+    'tqf_mlp_forward': (_i32, [_c_void_p, _c_void_p, _i64, _c_void_p,
+                               _c_void_p]),
+    'tqf_mlp_workspace_size': (_i64, [_i32]),
+    'tqf_mlp_train_step': (_i32, [_c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                                  _c_void_p, _c_void_p, _i64, _i64,
+                                  ctypes.c_float, ctypes.c_float, _i32, _i32,
+                                  _c_void_p, _c_void_p, _c_void_p,
+                                  _c_void_p]),
     'tqf_batch_workspace_size': (_i64, [_i32, _i32, _i32]),
+    'tqf_basket_workspace_size': (_i64, [_i32, _i32]),
+    'tqf_basket_call': (_i32, [_c_void_p, _i32, _i32, _f64, _f64, _f64, _i64,
+                               _u64, _u32, _i32, _i32, _c_void_p, _c_void_p,
+                               _c_void_p]),
     'tqf_batch_price': (_i32, [_c_void_p, _i32, _c_void_p, _c_void_p, _i32,
                                _c_void_p, _c_void_p, _u64, _i32, _i32, _i32,
                                _c_void_p, _c_void_p, _c_void_p]),
