@@ -63,6 +63,95 @@ def cpu_baseline(n_paths, chunk=2**14):
   return n_paths * steps / dt, dt
 
 
+def accuracy_check(prices, ses):
+  """FP32 2^20-path prices against the engine's float64 run of the same
+  scheme at 2^24 paths (the float64 path is pinned to the numpy oracle at
+  1e-12 by the parity tests) and against the semi-analytic Heston price
+  (which also sees the Euler discretisation bias)."""
+  import torch
+  from oracle import heston as oheston
+  from tqf.engine import pricing
+  ref = pricing.heston_ft_calls(num_paths=2**24, num_time_steps=NUM_STEPS,
+                                maturities=MATURITIES, grid='reference',
+                                random='sobol_shifted', seed=7,
+                                dtype=torch.float64, **PARAMS)
+  rp = ref['prices'].cpu().numpy()
+  rse = ref['stderrs'].cpu().numpy()
+  strikes = np.linspace(60, 140, 64)
+  sa = np.array([[oheston.european_call(PARAMS['s0'], k, t, PARAMS['r'],
+                                        PARAMS['v0'], PARAMS['kappa'],
+                                        PARAMS['theta'], PARAMS['xi'],
+                                        PARAMS['rho'])
+                  for k in strikes] for t in MATURITIES])
+  big = rp > 1.0
+  err = np.abs(prices - rp)
+  comb = np.sqrt(ses**2 + rse**2)
+  return {
+      'vs': 'engine float64, same scheme, 2^24 paths',
+      'max_rel_err_price_gt_1': float(np.max(err[big] / rp[big])),
+      'max_abs_err': float(np.max(err)),
+      'max_err_in_combined_se': float(np.max(err / comb)),
+      'frac_within_3se': float(np.mean(err <= 3 * comb)),
+      'max_stderr': float(np.max(ses)),
+      'fp64_vs_semi_analytic_max_abs': float(np.max(np.abs(rp - sa))),
+  }
+
+
+def secondary_metrics():
+  """One timed run each of the other configurations (not the headline)."""
+  import torch
+  from tqf.engine import batch as batch_lib
+  from tqf.engine import greeks
+  from tqf.engine import euler as engine
+  from tqf.engine import pricing
+  out = {}
+
+  def timed(fn, reps=3):
+    fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+      r = fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / 1e3 / reps, r
+
+  # plain method: GBM Euler float64, 1e5 paths x 100 steps, 8 calls
+  all_times, keep = pricing.heston_grid_for([1.0], 100, 'reference',
+                                            torch.float64)
+  c1 = pricing.european_calls(('gbm', 0.05, 0.2), [100.0],
+                              np.linspace(80, 120, 8), all_times, keep,
+                              100000, engine.RngSpec('philox', seed=42),
+                              discount_rate=0.05, dtype=torch.float64,
+                              prepare=True)
+  t, _ = timed(c1.launch, 10)
+  out['gbm_euler_f64'] = {'value': 1e7 / t, 'unit': 'path-steps/s',
+                          'seconds': t}
+  # batched heterogeneous contracts with Greeks
+  cs = batch_lib.random_contracts(4096)
+  plan = batch_lib.BatchPlan(cs, tile=1024)
+  g = plan.capture()
+  t, _ = timed(lambda: g.replay(), 2)
+  out['batch_4096'] = {'value': 4096 / t, 'unit': 'options+Greeks/s',
+                       'seconds': t,
+                       'path_steps': plan.total_path_steps,
+                       'path_steps_per_s': plan.total_path_steps / t}
+  # 16-asset basket with adjoint Greeks
+  d = 16
+  rs = np.random.RandomState(3)
+  a = rs.standard_normal((d, d))
+  c = a @ a.T + np.eye(d)
+  corr = c / np.sqrt(np.outer(np.diag(c), np.diag(c)))
+  sig = rs.uniform(0.15, 0.45, d)
+  t, _ = timed(lambda: greeks.basket_call_greeks(
+      np.full(d, 100.0), sig, corr, 0.02, 1.0, 100.0, 64, 262144), 3)
+  out['basket16_greeks'] = {'value': 262144 * 64 / t, 'unit': 'path-steps/s',
+                            'seconds': t, 'greeks': 33}
+  return out
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument('--gpus', type=int, default=1)
@@ -72,6 +161,8 @@ def main():
   ap.add_argument('--cpu-paths', type=int, default=2**17)
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--no-check', action='store_true')
+  ap.add_argument('--quick', action='store_true',
+                  help='headline workload only (no secondary metrics)')
   args = ap.parse_args()
   if args.gpus != 1:
     print('bench.py measures one GPU: this build runs on a single MI355X and '
@@ -119,20 +210,11 @@ def main():
   ses = res['stderrs'].cpu().numpy()
   check = None
   if not args.no_check:
-    from oracle import heston as oheston
-    strikes = np.linspace(60, 140, 64)
-    ref = np.array([[oheston.european_call(PARAMS['s0'], k, t, PARAMS['r'],
-                                           PARAMS['v0'], PARAMS['kappa'],
-                                           PARAMS['theta'], PARAMS['xi'],
-                                           PARAMS['rho'])
-                     for k in strikes] for t in MATURITIES])
-    check = {
-        'vs': 'semi-analytic Heston price (discretisation bias included)',
-        'max_abs_err': float(np.max(np.abs(prices - ref))),
-        'max_rel_err_price_gt_1': float(np.max(
-            np.abs(prices - ref)[ref > 1] / ref[ref > 1])),
-        'max_stderr': float(np.max(ses)),
-    }
+    check = accuracy_check(prices, ses)
+
+  secondary = None
+  if not args.quick:
+    secondary = secondary_metrics()
 
   cpu = None
   if not args.no_cpu:
@@ -168,6 +250,7 @@ def main():
                    'flop_per_path_step': FLOP_PER_PATH_STEP},
       'cpu_baseline': cpu,
       'check': check,
+      'secondary': secondary,
       'wall_s': wall,
   }
   print(json.dumps(out))

@@ -220,7 +220,8 @@ typedef struct tqf_lsm {
   int32_t n_times;
   int32_t dim;              /* assets, 1..8                                  */
   int32_t payoff;           /* TQF_PAYOFF_*                                  */
-  int32_t degree;           /* make_polynomial_basis(degree)                 */
+  int32_t degree;           /* make_polynomial_basis(degree),
+                               (degree + 1) * dim <= 64                       */
   int32_t n_basis;          /* (degree + 1)^dim, at most 255                 */
   int32_t pad0;
   const uint8_t* exponents; /* device [n_basis][dim] exponent table in the

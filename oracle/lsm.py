@@ -28,13 +28,20 @@ def exponent_grid(degree, dim):
   return np.stack(mesh, -1).reshape(-1, dim)
 
 
-def make_polynomial_basis(degree):
+def make_polynomial_basis(degree, standardize=False):
+  """standardize=True divides the centred values by their standard
+  deviation: every column is rescaled by a constant, so the least-squares
+  fit is unchanged in exact arithmetic while the Gram matrix becomes well
+  conditioned (the form the GPU kernels solve)."""
   def basis(sample_paths, time_index):
     x = np.asarray(sample_paths)
     if x.ndim == 3:
       x = x[None]
     sl = x[:, :, time_index, :]  # [B, N, d]
     c = sl - sl.mean(axis=1, keepdims=True)
+    if standardize:
+      sd = sl.std(axis=1, keepdims=True)
+      c = c / np.where(sd > 0, sd, 1.0)
     e = exponent_grid(degree, x.shape[-1]).astype(x.dtype)  # [K, d]
     out = np.prod(c[:, :, None, :]**e[None, None], axis=-1)  # [B, N, K]
     return np.transpose(out, (0, 2, 1))

@@ -3,8 +3,9 @@
   * quasi draws: z = erfinv((u - 0.5) * 2) * sqrt(2)
     (tff/math/random_ops/multivariate_normal.py:420), evaluated here in
     float64 with scipy.special.erfinv.
-  * the engine's shifted-Sobol transform: 24-bit uniform u = (m + 1/2) 2^-24
-    from the top 24 bits m of a 32-bit point, q = 2u - 1 formed exactly,
+  * the engine's shifted-Sobol transform: float32 uses the 24-bit uniform
+    u = (m + 1/2) 2^-24 of the top 24 bits m of a 32-bit point, float64 the
+    32-bit u = (v + 1/2) 2^-32; q = 2u - 1 is formed exactly and
     z = sqrt(2) erfinv(q).
 
 Pinned by scipy's independent special functions (erfinv/ndtri agree) and by
@@ -20,7 +21,12 @@ def quasi_normals(u):
   return special.erfinv((u - 0.5) * 2) * np.sqrt(2.0)
 
 
-def shifted_sobol_normals(ints_u32):
-  m = (np.asarray(ints_u32, np.uint64) >> np.uint64(8)).astype(np.float64)
-  q = (2 * m + 1 - 2.0**24) * 2.0**-24
+def shifted_sobol_normals(ints_u32, dtype=np.float32):
+  """float32 kernels use the top 24 bits, float64 kernels all 32 bits."""
+  v = np.asarray(ints_u32, np.uint64)
+  if dtype == np.float64:
+    q = (v.astype(np.float64) + 0.5) * 2.0**-31 - 1.0
+  else:
+    m = (v >> np.uint64(8)).astype(np.float64)
+    q = (2 * m + 1 - 2.0**24) * 2.0**-24
   return special.erfinv(q) * np.sqrt(2.0)

@@ -106,4 +106,8 @@ def normals(num_paths, num_draws, seed, stream=0, dtype=np.float64,
       raise ValueError('antithetic sampling needs an even number of paths')
     half = normals(num_paths // 2, num_draws, seed, stream, dtype)
     return np.concatenate([half, -half], axis=0)
-  return box_muller(raw_words(num_paths, num_draws, seed, stream), dtype)
+  # pairs are (w0, w1), (w2, w3) of each 4-word block, also for the last
+  # partial block
+  nb4 = (num_draws + 3) // 4 * 4
+  w = raw_words(num_paths, nb4, seed, stream)
+  return box_muller(w, dtype)[:, :num_draws]

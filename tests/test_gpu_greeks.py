@@ -70,16 +70,19 @@ def test_batch_heston_price_and_adjoint_vs_oracle_fd():
     z = z.reshape(c['n_paths'], c['n_steps'], 2)
     price = _oracle_heston(c, z)
     assert abs(r['price'][0] - price) <= 2e-4 * price + 1e-4
-    for name, field, h in (('delta', 's0', 1e-4), ('dv0', 'v0', 1e-6),
-                           ('rho', 'rate', 1e-6), ('dkappa', 'kappa', 1e-5),
-                           ('dtheta', 'theta', 1e-6), ('dxi', 'xi', 1e-5),
-                           ('drho', 'rho', 1e-5)):
+    # Central differences of the float64 oracle with common random numbers.
+    # The steps must be tiny: sqrt(v+) is far from linear for the paths whose
+    # variance sits near 0, so h = 1e-6 on theta is already outside the
+    # linear range (it converges to the pathwise value only for h <= 1e-7).
+    for name, field, h in (('delta', 's0', 1e-5), ('dv0', 'v0', 1e-9),
+                           ('rho', 'rate', 1e-9), ('dkappa', 'kappa', 1e-8),
+                           ('dtheta', 'theta', 1e-9), ('dxi', 'xi', 1e-8),
+                           ('drho', 'rho', 1e-8)):
       fd = (_oracle_heston(c, z, field, h) - _oracle_heston(c, z, field, -h)) / (
           2 * h)
       got = r[name][0]
       scale = max(abs(fd), 1e-2 * abs(price))
-      assert abs(got - fd) <= 3e-3 * scale + 3 * r[name][1] * 0.05, (
-          name, got, fd)
+      assert abs(got - fd) <= 2e-3 * scale, (name, got, fd)
 
 
 def test_batch_equals_loop_of_single_contracts():

@@ -83,8 +83,10 @@ def test_lsm_matches_oracle_on_simulated_paths(dim, degree, kind):
     pay_o = olsm.make_basket_call_max_payoff([100.0])
     pay_t = lsm.make_max_call_payoff([100.0])
   for calib in (None, n // 2):
+    # the kernels solve the standardised normal equations exactly; the
+    # oracle does the same with numpy pinv on the standardised basis
     want = olsm.least_square_mc(paths, [1, 2, 3, 4], pay_o,
-                                olsm.make_polynomial_basis(degree),
+                                olsm.make_polynomial_basis(degree, True),
                                 discount_factors=df,
                                 num_calibration_samples=calib)
     got = lsm.least_square_mc(paths, [1, 2, 3, 4], pay_t,
@@ -97,21 +99,52 @@ def test_lsm_matches_oracle_on_simulated_paths(dim, degree, kind):
     np.testing.assert_allclose(_np(got), want, rtol=2e-6)
 
 
-def test_generic_torch_lsm_matches_native():
+def test_generic_torch_lsm_matches_oracle_and_native():
   rs = np.random.RandomState(9)
   n, times = 5000, np.linspace(0.5, 2.0, 4)
   z = rs.standard_normal((n, 4, 1))
   paths = ogbm.sample_paths(times, 0.03, 0.3, 100.0, z)
   paths = np.concatenate([np.full((n, 1, 1), 100.0), paths], 1)
+  df = np.exp(-0.03 * times)
   put = lsm.make_basket_put_payoff([100.0])
-  native = lsm.least_square_mc(paths, [1, 2, 3, 4], put,
-                               lsm.make_polynomial_basis(3),
-                               discount_factors=np.exp(-0.03 * times),
-                               dtype=np.float64)
-  basis = lsm.make_polynomial_basis(3)
+  basis = lsm.make_polynomial_basis(2)
   plain_basis = lambda p, t: basis(p, t)
   plain_put = lambda p, t: put(p, t)
   generic = lsm.least_square_mc(paths, [1, 2, 3, 4], plain_put, plain_basis,
-                                discount_factors=np.exp(-0.03 * times),
-                                dtype=np.float64)
-  np.testing.assert_allclose(_np(generic), _np(native), rtol=1e-6)
+                                discount_factors=df, dtype=np.float64)
+  want = olsm.least_square_mc(paths, [1, 2, 3, 4],
+                              olsm.make_basket_put_payoff([100.0]),
+                              olsm.make_polynomial_basis(2),
+                              discount_factors=df)
+  np.testing.assert_allclose(_np(generic), want, rtol=1e-9)
+  native = lsm.least_square_mc(paths, [1, 2, 3, 4], put, basis,
+                               discount_factors=df, dtype=np.float64)
+  np.testing.assert_allclose(_np(generic), _np(native), rtol=1e-9)
+
+
+def test_reference_pinv_cutoff_loses_the_fit_on_ill_conditioned_bases():
+  # Documented deviation: on a 3-asset degree-2 max-call the reference's
+  # pinv of the raw Gram truncates directions and prices lower; the kernel
+  # solves the least-squares problem (standardised basis) exactly.
+  dim, n = 3, 20000
+  rs = np.random.RandomState(dim)
+  times = np.linspace(0.25, 1.0, 4)
+  corr = np.full((dim, dim), 0.3) + 0.7 * np.eye(dim)
+  z = rs.standard_normal((n, 4, dim))
+  paths = ogbm.mv_sample_paths(times, [0.02] * dim, [0.25] * dim, corr,
+                               [100.0] * dim, z)
+  paths = np.concatenate([np.full((n, 1, dim), 100.0), paths], 1)
+  df = np.exp(-0.05 * times)
+  pay_o = olsm.make_basket_call_max_payoff([100.0])
+  raw = olsm.least_square_mc(paths, [1, 2, 3, 4], pay_o,
+                             olsm.make_polynomial_basis(2),
+                             discount_factors=df)
+  exact = olsm.least_square_mc(paths, [1, 2, 3, 4], pay_o,
+                               olsm.make_polynomial_basis(2, True),
+                               discount_factors=df)
+  got = lsm.least_square_mc(paths, [1, 2, 3, 4],
+                            lsm.make_max_call_payoff([100.0]),
+                            lsm.make_polynomial_basis(2),
+                            discount_factors=df, dtype=np.float64)
+  np.testing.assert_allclose(_np(got), exact, rtol=2e-6)
+  assert raw[0] < exact[0]

@@ -1,0 +1,124 @@
+"""GPU MLP kernels (bf16 MFMA forward/backward, Adam) against the numpy
+oracle, and the learned Longstaff-Schwartz against the polynomial judge."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import gbm as ogbm
+from oracle import lsm as olsm
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+  from tqf.engine import mlp_lsm
+  from tqf.models import longstaff_schwartz as lsm_lib
+
+
+def _data(n, seed=0):
+  rs = np.random.RandomState(seed)
+  x = np.zeros((n, 8), np.float32)
+  x[:, :6] = rs.standard_normal((n, 6))
+  t = (np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2]).astype(np.float32)
+  w = (rs.uniform(size=n) > 0.25).astype(np.float32)
+  return x, t, w
+
+
+def _params(seed=1):
+  rs = np.random.RandomState(seed)
+  return (omlp.init_params(seed) +
+          0.05 * rs.standard_normal(omlp.N_PARAMS)).astype(np.float32)
+
+
+def test_mlp_forward_matches_oracle():
+  n = 1000 + 7  # a ragged last group of 16
+  x, _, _ = _data(n)
+  p = _params()
+  m = mlp_lsm.MlpModel(params=p)
+  got = m.forward(torch.as_tensor(x, device=m.dev)).cpu().numpy()
+  want_bf, _ = omlp.forward(p.astype(np.float64), x.astype(np.float64),
+                            bf=True)
+  want, _ = omlp.forward(p.astype(np.float64), x.astype(np.float64))
+  scale = np.max(np.abs(want))
+  # bf16 operands as the kernel: float32 accumulation order only
+  assert np.max(np.abs(got - want_bf)) <= 2e-5 * scale
+  # against the exact float64 network: bf16 rounding of the 64 x 64 layer
+  assert np.max(np.abs(got - want)) <= 2e-2 * scale
+
+
+def test_mlp_gradient_matches_oracle():
+  n = 4096 + 5
+  x, t, w = _data(n, 2)
+  p = _params(3)
+  m = mlp_lsm.MlpModel(params=p, n_blocks=37)
+  dev = m.dev
+  g, loss = m.grad(torch.as_tensor(x, device=dev),
+                   torch.as_tensor(t, device=dev),
+                   torch.as_tensor(w, device=dev), inv_norm=1.0 / n)
+  g = g.cpu().numpy()
+  lo, go = omlp.loss_and_grad(p.astype(np.float64), x.astype(np.float64),
+                              t.astype(np.float64), w.astype(np.float64),
+                              1.0 / n, bf=True)
+  np.testing.assert_allclose(loss.item(), lo, rtol=1e-4)
+  scale = np.max(np.abs(go))
+  assert np.max(np.abs(g - go)) <= 2e-4 * scale, np.argmax(np.abs(g - go))
+  _, gx = omlp.loss_and_grad(p.astype(np.float64), x.astype(np.float64),
+                             t.astype(np.float64), w.astype(np.float64),
+                             1.0 / n)
+  assert np.max(np.abs(g - gx)) <= 3e-2 * np.max(np.abs(gx))
+  # a gradient on a row window [first, first + n)
+  g2, _ = m.grad(torch.as_tensor(x, device=dev), torch.as_tensor(t, device=dev),
+                 torch.as_tensor(w, device=dev), first=100, n=1000,
+                 inv_norm=1e-3)
+  _, go2 = omlp.loss_and_grad(p.astype(np.float64), x[100:1100].astype(
+      np.float64), t[100:1100].astype(np.float64),
+                              w[100:1100].astype(np.float64), 1e-3, bf=True)
+  assert np.max(np.abs(g2.cpu().numpy() - go2)) <= 2e-4 * np.max(np.abs(go2))
+
+
+def test_mlp_adam_steps_match_oracle_and_training_converges():
+  n = 8192
+  x, t, w = _data(n, 4)
+  p = _params(5)
+  m = mlp_lsm.MlpModel(params=p)
+  dev = m.dev
+  xt, tt, wt = (torch.as_tensor(a, device=dev) for a in (x, t, w))
+  po = p.astype(np.float64)
+  mo = np.zeros_like(po)
+  vo = np.zeros_like(po)
+  for k in range(1, 4):
+    m.train_step(xt, tt, wt, 0, n, 1.0 / n, 1e-3)
+    _, g = omlp.loss_and_grad(po, x.astype(np.float64), t.astype(np.float64),
+                              w.astype(np.float64), 1.0 / n, bf=True)
+    po, mo, vo = omlp.adam(po, mo, vo, g, k)
+  np.testing.assert_allclose(m.params.cpu().numpy(), po, rtol=0, atol=3e-5)
+  first = None
+  for k in range(300):
+    m.train_step(xt, tt, wt, 0, n, 1.0 / n, 3e-3)
+    if k == 0:
+      first = m.loss.item()
+  assert m.loss.item() < 0.2 * first
+
+
+def test_mlp_lsm_prices_against_polynomial_judge():
+  # 3-asset Bermudan max-call, small sizes
+  paths, df = mlp_lsm.bermudan_max_call_paths(2**16, dim=3, n_dates=5,
+                                              steps_per_date=4, maturity=1.0)
+  res = mlp_lsm.mlp_lsm_price(paths, [1, 2, 3, 4, 5], 100.0, df, 2**15,
+                              steps_per_date=60, batch=8192, lr=3e-3)
+  # the polynomial judge matches the oracle (standardised basis) on the
+  # same paths
+  p = paths.cpu().numpy()
+  want = olsm.least_square_mc(p, [1, 2, 3, 4, 5],
+                              olsm.make_basket_call_max_payoff([100.0]),
+                              olsm.make_polynomial_basis(2, True),
+                              discount_factors=df,
+                              num_calibration_samples=2**15)
+  np.testing.assert_allclose(res['poly_price'], want[0], rtol=2e-6)
+  assert res['mlp_se'] > 0 and res['poly_se'] > 0
+  # both are lower-bound prices of the same option on the same pricing
+  # paths: a reasonable learned policy is within a few SE of the judge
+  assert abs(res['gain']) < 6 * res['gain_se'] + 0.02 * res['poly_price']
+  assert res['accepted'] == (res['mlp_price'] >=
+                             res['poly_price'] - 2 * res['gain_se'])
+  assert all(np.isfinite(res['losses']))

@@ -345,3 +345,22 @@ def test_basket_pathwise_greeks_match_finite_differences():
   fd = (gbm.basket_call_price(s0, sig, corr, .02 + h, 1., 100., z) -
         gbm.basket_call_price(s0, sig, corr, .02 - h, 1., 100., z)) / (2 * h)
   np.testing.assert_allclose(p['rho'].mean(), fd, rtol=1e-5)
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+  from oracle import mlp
+  rs = np.random.RandomState(0)
+  p = mlp.init_params(1) + 0.01 * rs.standard_normal(mlp.N_PARAMS)
+  x = rs.standard_normal((40, 8))
+  t = rs.standard_normal(40)
+  w = (rs.uniform(size=40) > 0.3).astype(np.float64)
+  loss, g = mlp.loss_and_grad(p, x, t, w, 1 / 40)
+  idx = rs.choice(mlp.N_PARAMS, 60, replace=False)
+  idx = np.concatenate([idx, [0, 383, 384, 447, 448, 4543, 4544, 4608, 4672]])
+  h = 1e-6
+  for i in idx:
+    e = np.zeros_like(p)
+    e[i] = h
+    fd = (mlp.loss_and_grad(p + e, x, t, w, 1 / 40)[0] -
+          mlp.loss_and_grad(p - e, x, t, w, 1 / 40)[0]) / (2 * h)
+    np.testing.assert_allclose(g[i], fd, rtol=1e-5, atol=1e-9)

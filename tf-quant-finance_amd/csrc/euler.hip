@@ -261,19 +261,34 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   }
 }
 
+// One workgroup per output: strided partial sums over the blocks, then a
+// fixed-shape LDS tree (deterministic), then mean / standard error.
 __global__ void __launch_bounds__(256)
 payoff_finalize_kernel(const double* __restrict__ partials, int64_t n_blocks,
                        int n_out, int n_strikes, int64_t n_paths,
                        const double* __restrict__ discount,
                        double* __restrict__ prices,
                        double* __restrict__ stderrs, double* __restrict__ sums) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= n_out) return;
+  __shared__ double r1[256], r2[256];
+  const int o = blockIdx.x;
   double s1 = 0, s2 = 0;
-  for (int64_t b = 0; b < n_blocks; ++b) {
+  for (int64_t b = threadIdx.x; b < n_blocks; b += 256) {
     s1 += partials[(b * n_out + o) * 2];
     s2 += partials[(b * n_out + o) * 2 + 1];
   }
+  r1[threadIdx.x] = s1;
+  r2[threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  s1 = r1[0];
+  s2 = r2[0];
   const double n = (double)n_paths;
   const double mean = s1 / n;
   double var = n > 1 ? (s2 - n * mean * mean) / (n - 1) : 0.0;
@@ -360,8 +375,8 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   if (rc) return rc;
   if (out->mode == TQF_OUT_PAYOFF) {
     const int n_out = g->n_slots * out->n_strikes;
-    hipLaunchKernelGGL(payoff_finalize_kernel, dim3((n_out + 255) / 256),
-                       dim3(256), 0, stream, out->workspace, (int64_t)n_blocks,
+    hipLaunchKernelGGL(payoff_finalize_kernel, dim3(n_out), dim3(256), 0,
+                       stream, out->workspace, (int64_t)n_blocks,
                        n_out, out->n_strikes, n_paths, out->discount,
                        out->prices, out->stderrs, out->sums);
     rc = check_launch("tqf_euler finalize");

@@ -28,7 +28,7 @@ using namespace tqf;
 namespace {
 
 constexpr int kMaxAssets = 8;
-constexpr int kMaxDeg = 8;
+constexpr int kMaxPw = 64;      // (degree + 1) * dim power-table entries
 constexpr int kTile = 64;       // Gram output tile (kTile x kTile)
 constexpr int kSubPaths = 32;   // paths staged per LDS round
 
@@ -134,7 +134,7 @@ lsm_gram_kernel(LsmArgs a, const double* __restrict__ exv,
                 double* __restrict__ partial) {
   __shared__ double xi[kSubPaths][kTile];
   __shared__ double xj[kSubPaths][kTile];
-  __shared__ double pw[kSubPaths][kMaxAssets * (kMaxDeg + 1)];
+  __shared__ double pw[kSubPaths][kMaxPw];
   __shared__ double wrow[kSubPaths];
   __shared__ double yrow[kSubPaths];
   __shared__ uint8_t expo[256 * kMaxAssets];
@@ -335,7 +335,7 @@ lsm_update_kernel(LsmArgs a, double df_ratio, const double* __restrict__ exv,
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= a.n_paths) return;
   const int deg1 = a.deg + 1;
-  double pw[kMaxAssets * (kMaxDeg + 1)];
+  double pw[kMaxPw];
   const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
   for (int j = 0; j < a.d; ++j) {
     const double c = (row[j] - a.mean[j]) * a.scale[j];
@@ -473,8 +473,8 @@ int validate(const tqf_lsm* l) {
   TQF_REQUIRE(l != nullptr, "null descriptor");
   TQF_REQUIRE(l->dim >= 1 && l->dim <= kMaxAssets,
               "LSM supports 1..%d assets, got %d", kMaxAssets, l->dim);
-  TQF_REQUIRE(l->degree >= 0 && l->degree <= kMaxDeg,
-              "polynomial degree must be in [0, %d]", kMaxDeg);
+  TQF_REQUIRE(l->degree >= 0 && (l->degree + 1) * l->dim <= kMaxPw,
+              "(degree + 1) * dim must be at most %d", kMaxPw);
   TQF_REQUIRE(l->n_basis >= 1 && l->n_basis <= 255,
               "basis size must be in [1, 255], got %d", l->n_basis);
   TQF_REQUIRE(l->n_paths >= 1 && l->n_times >= 1, "empty sample paths");

@@ -87,12 +87,14 @@ class BasketCall(torch.autograd.Function):
     ctx.save_for_backward(mean[2:2 + d].to(s0.dtype),
                           mean[2 + d:].to(sigma.dtype))
     ctx.rho = mean[1]
-    return mean[0].to(s0.dtype)
+    return mean[0].to(device=s0.device, dtype=s0.dtype)
 
   @staticmethod
   def backward(ctx, g):
     delta, vega = ctx.saved_tensors
-    return g * delta, g * vega, (g * ctx.rho).to(g.dtype), None
+    dev = g.device
+    return (g * delta.to(dev), g * vega.to(dev),
+            (g * ctx.rho.to(dev)).to(g.dtype), None)
 
 
 def basket_call(s0, sigma, rate, **cfg):
@@ -120,11 +122,12 @@ class HestonCall(torch.autograd.Function):
              xi=float(xi), rho=float(rho), stream_id=cfg.get('stream_id', 0))
     mean, _, _ = _contract_launch(c, cfg['n_paths'], cfg.get('seed', 11), dev)
     ctx.g = mean[1:8].clone()
-    return mean[0].to(torch.as_tensor(s0).dtype)
+    s0t = torch.as_tensor(s0)
+    return mean[0].to(device=s0t.device, dtype=s0t.dtype)
 
   @staticmethod
   def backward(ctx, g):
-    d = ctx.g.to(g.dtype)
+    d = ctx.g.to(device=g.device, dtype=g.dtype)
     return tuple(g * d[i] for i in range(7)) + (None,)
 
 
@@ -141,9 +144,10 @@ class GbmCall(torch.autograd.Function):
              stream_id=cfg.get('stream_id', 0))
     mean, _, _ = _contract_launch(c, cfg['n_paths'], cfg.get('seed', 11), dev)
     ctx.g = mean[1:4].clone()
-    return mean[0].to(torch.as_tensor(s0).dtype)
+    s0t = torch.as_tensor(s0)
+    return mean[0].to(device=s0t.device, dtype=s0t.dtype)
 
   @staticmethod
   def backward(ctx, g):
-    d = ctx.g.to(g.dtype)
+    d = ctx.g.to(device=g.device, dtype=g.dtype)
     return g * d[0], g * d[1], g * d[2], None

@@ -1,0 +1,217 @@
+"""Learned Longstaff-Schwartz: an MLP continuation value trained per
+exercise date with the tqf_mlp_* kernels, judged against the polynomial
+regression of the reference (tff/models/longstaff_schwartz/lsm.py).
+
+Backward induction as least_square_mc (lsm.py:258-436), with the
+regression step replaced: at each date the network (6 -> 64 -> 64 -> 1,
+SiLU, bf16 MFMA) is fitted by Adam to the discounted continuation cashflows
+of the in-the-money calibration paths, then predicts the continuation value
+of every path.  The price is the discounted mean over the independent
+pricing paths, a lower bound for any exercise policy.  The polynomial LSM
+price on the same paths is the judge: the MLP price is accepted only when
+it is at least poly - 2 SE; otherwise the polynomial price is reported.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from tqf import _native
+from tqf.models import longstaff_schwartz as lsm_lib
+
+__all__ = ['MlpModel', 'mlp_lsm_price', 'init_params']
+
+N_IN, N_H = 6, 64
+N_PARAMS = N_IN * N_H + N_H + N_H * N_H + N_H + N_H + 1
+
+
+def init_params(seed=0):
+  """He-style initialisation in the kernels' parameter layout."""
+  rs = np.random.RandomState(seed)
+  w1 = rs.standard_normal((N_IN, N_H)) * np.sqrt(2.0 / N_IN)
+  w2 = rs.standard_normal((N_H, N_H)) * np.sqrt(2.0 / N_H)
+  w3 = rs.standard_normal(N_H) * np.sqrt(1.0 / N_H)
+  return np.concatenate([w1.ravel(), np.zeros(N_H), w2.ravel(), np.zeros(N_H),
+                         w3, [0.0]]).astype(np.float32)
+
+
+class MlpModel:
+  """Device parameters + Adam state; forward / train-step wrappers."""
+
+  def __init__(self, params=None, seed=0, n_blocks=None, dev=None):
+    self.dev = dev or _native.device()
+    p = init_params(seed) if params is None else np.asarray(params, np.float32)
+    self.params = torch.as_tensor(p, device=self.dev).contiguous()
+    self.m = torch.zeros_like(self.params)
+    self.v = torch.zeros_like(self.params)
+    self.step = 0
+    if n_blocks is None:
+      n_blocks = torch.cuda.get_device_properties(
+          self.dev).multi_processor_count * 2
+    self.n_blocks = int(n_blocks)
+    ws = _native.lib().tqf_mlp_workspace_size(self.n_blocks)
+    self._ws = torch.empty(ws // 4 + 1, dtype=torch.float32, device=self.dev)
+    self.loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
+
+  def forward(self, x, out=None):
+    n = x.shape[0]
+    if out is None:
+      out = torch.empty(n, dtype=torch.float32, device=self.dev)
+    _native.call('tqf_mlp_forward', ctypes.c_void_p(self.params.data_ptr()),
+                 ctypes.c_void_p(x.data_ptr()), ctypes.c_int64(n),
+                 ctypes.c_void_p(out.data_ptr()), _native.stream_ptr(self.dev))
+    return out
+
+  def grad(self, x, t, w, first=0, n=None, inv_norm=None):
+    """Gradient only (lr = 0): returns (grad [N_PARAMS], loss)."""
+    n = x.shape[0] - first if n is None else n
+    inv_norm = 1.0 / n if inv_norm is None else inv_norm
+    g = torch.empty(N_PARAMS, dtype=torch.float32, device=self.dev)
+    self._call(x, t, w, first, n, inv_norm, 0.0, 1, g)
+    return g, self.loss
+
+  def train_step(self, x, t, w, first, n, inv_norm, lr):
+    self.step += 1
+    self._call(x, t, w, first, n, inv_norm, lr, self.step, None)
+
+  def _call(self, x, t, w, first, n, inv_norm, lr, step, g):
+    _native.call('tqf_mlp_train_step', ctypes.c_void_p(self.params.data_ptr()),
+                 ctypes.c_void_p(self.m.data_ptr()),
+                 ctypes.c_void_p(self.v.data_ptr()),
+                 ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(t.data_ptr()),
+                 ctypes.c_void_p(w.data_ptr()), ctypes.c_int64(first),
+                 ctypes.c_int64(n), ctypes.c_float(inv_norm),
+                 ctypes.c_float(lr), ctypes.c_int32(step),
+                 ctypes.c_int32(self.n_blocks),
+                 ctypes.c_void_p(self._ws.data_ptr()),
+                 ctypes.c_void_p(0 if g is None else g.data_ptr()),
+                 ctypes.c_void_p(self.loss.data_ptr()),
+                 _native.stream_ptr(self.dev))
+
+
+def _lsm_struct(paths, payoff, strike, degree=2):
+  n, nt, d = paths.shape
+  expo = lsm_lib.exponent_grid(degree, d).astype(np.uint8)
+  expo_t = torch.as_tensor(expo, device=paths.device).contiguous()
+  l = _native.Lsm()
+  l.paths = paths.data_ptr()
+  l.n_paths = n
+  l.n_times = nt
+  l.dim = d
+  l.payoff = payoff
+  l.degree = degree
+  l.n_basis = expo.shape[0]
+  l.exponents = expo_t.data_ptr()
+  l.strike = float(strike)
+  return l, expo_t
+
+
+def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
+                  num_calibration, payoff=lsm_lib.PAYOFF_MAX_CALL,
+                  steps_per_date=200, batch=65536, lr=1e-3, seed=0,
+                  poly_degree=2, judge=True):
+  """Prices with the learned continuation value (and the polynomial judge).
+
+  paths: float64 device tensor [N, T, d], rows [0, num_calibration) train
+  the models, the rest price.  discount_factors: [n_ex] (unpadded).
+  Returns a dict with mlp / poly prices and standard errors, the accepted
+  price and the per-date final training losses.
+  """
+  dev = paths.device
+  paths = paths.contiguous()
+  n = paths.shape[0]
+  ex = [int(v) for v in exercise_idx]
+  df = np.concatenate([[1.0], np.asarray(discount_factors, np.float64)])
+  l, keep = _lsm_struct(paths, payoff, strike, poly_degree)
+  s = _native.stream_ptr(dev)
+  f64 = dict(dtype=torch.float64, device=dev)
+  cash = torch.empty(n, **f64)
+  vals = torch.empty(n, **f64)
+  exv = torch.empty(n, **f64)
+  y = torch.empty(n, **f64)
+  mom = torch.empty(_native.lib().tqf_lsm_moments_size(n) // 8 + 1, **f64)
+  x = torch.empty((n, 8), dtype=torch.float32, device=dev)
+  tgt = torch.empty(n, dtype=torch.float32, device=dev)
+  w = torch.empty(n, dtype=torch.float32, device=dev)
+  cont = torch.empty(n, dtype=torch.float32, device=dev)
+  fin = torch.empty((n + 255) // 256 * 2 + 2, **f64)
+  res = torch.empty(2, **f64)
+  model = MlpModel(seed=seed, dev=dev)
+  losses = []
+  _native.call('tqf_lsm_terminal', ctypes.byref(l), ctypes.c_int32(ex[-1]),
+               ctypes.c_void_p(cash.data_ptr()),
+               ctypes.c_void_p(vals.data_ptr()), s)
+  n_cal = int(num_calibration)
+  batch = min(int(batch), n_cal)
+  for e in range(len(ex) - 1, 0, -1):
+    ratio = df[e + 1] / df[e]
+    _native.call('tqf_lsm_date_inputs', ctypes.byref(l),
+                 ctypes.c_int32(ex[e - 1]), ctypes.c_double(ratio),
+                 ctypes.c_void_p(cash.data_ptr()),
+                 ctypes.c_void_p(vals.data_ptr()),
+                 ctypes.c_void_p(exv.data_ptr()),
+                 ctypes.c_void_p(y.data_ptr()),
+                 ctypes.c_void_p(mom.data_ptr()),
+                 ctypes.c_void_p(x.data_ptr()),
+                 ctypes.c_void_p(tgt.data_ptr()),
+                 ctypes.c_void_p(w.data_ptr()), s)
+    model.m.zero_()
+    model.v.zero_()
+    model.step = 0
+    for k in range(int(steps_per_date)):
+      first = (k * batch) % max(n_cal - batch + 1, 1)
+      model.train_step(x, tgt, w, first, batch, 1.0 / batch, lr)
+    losses.append(model.loss.clone())
+    model.forward(x, cont)
+    _native.call('tqf_lsm_update_cont', ctypes.c_int64(n),
+                 ctypes.c_double(strike), ctypes.c_double(ratio),
+                 ctypes.c_void_p(exv.data_ptr()),
+                 ctypes.c_void_p(cont.data_ptr()),
+                 ctypes.c_void_p(cash.data_ptr()),
+                 ctypes.c_void_p(vals.data_ptr()), s)
+  _native.call('tqf_lsm_final', ctypes.c_void_p(cash.data_ptr()),
+               ctypes.c_void_p(vals.data_ptr()), ctypes.c_double(df[1] / df[0]),
+               ctypes.c_int64(n_cal), ctypes.c_int64(n),
+               ctypes.c_void_p(fin.data_ptr()), ctypes.c_void_p(res.data_ptr()),
+               s)
+  mlp_price, mlp_se = [float(v) for v in res.cpu().numpy()]
+  out = {'mlp_price': mlp_price, 'mlp_se': mlp_se,
+         'losses': [float(t.item()) for t in losses]}
+  if judge:
+    poly, poly_se, _ = lsm_lib._native_lsm(
+        paths, ex, (payoff, np.array([float(strike)])), poly_degree,
+        np.asarray(df).reshape(-1, 1, 1), n_cal, torch.float64, dev,
+        return_details=True)
+    poly_price = float(poly[0])
+    poly_se = float(poly_se[0])
+    comb = np.hypot(mlp_se, poly_se)
+    accepted = mlp_price >= poly_price - 2 * comb
+    out.update({'poly_price': poly_price, 'poly_se': poly_se,
+                'accepted': bool(accepted),
+                'price': mlp_price if accepted else poly_price,
+                'gain': mlp_price - poly_price, 'gain_se': float(comb)})
+  return out
+
+
+def bermudan_max_call_paths(n_paths, dim=5, s0=100.0, sigma=0.2, corr=0.3,
+                            r=0.05, q=0.1, maturity=3.0, n_dates=9,
+                            steps_per_date=20, seed=17):
+  """Correlated GBM paths at t = 0 and the exercise dates, float64
+  [n_paths, n_dates + 1, dim], simulated over n_dates * steps_per_date
+  exact steps by tqf_gbm_exact (Philox).  Also returns the discount factors
+  of the exercise dates."""
+  from tqf.engine import euler as engine
+  from tqf.models.geometric_brownian_motion import run_exact
+  steps = n_dates * steps_per_date
+  grid = np.linspace(0.0, maturity, steps + 1)
+  keep = np.zeros(steps + 1, bool)
+  keep[::steps_per_date] = True
+  c = np.full((dim, dim), corr) + (1 - corr) * np.eye(dim)
+  drift = np.full(dim, (r - q) - 0.5 * sigma * sigma)
+  vol = np.full(dim, sigma)
+  out = run_exact(grid[1:], drift, vol, np.linalg.cholesky(c),
+                  np.full(dim, s0), int(n_paths),
+                  engine.RngSpec('philox', seed=seed), torch.float32,
+                  keep_mask=keep)
+  dates = grid[keep][1:]
+  return out.to(torch.float64), np.exp(-r * dates)

@@ -8,8 +8,13 @@ tff.models.longstaff_schwartz).
 
 With the basis and payoff built here, the whole backward induction runs in
 the tqf_lsm_price kernels (float64 normal equations built on the fly,
-Cholesky solve on the GPU).  Other callables run the reference's algorithm
-as torch ops on the same GPU (torch.linalg.pinv for the solve).
+Cholesky solve on the GPU).  The kernels standardise each asset before
+forming the monomials and Jacobi-scale the normal equations, so they solve
+the least-squares problem exactly where the reference's pinv of the raw
+tensor-product Gram (condition numbers past 1e15 for degree >= 2 on three
+or more assets) discards directions; on well-conditioned problems the two
+agree.  Other callables run the reference's algorithm as torch ops on the
+same GPU (torch.linalg.pinv with TF's cutoff).
 """
 import ctypes
 
@@ -210,7 +215,10 @@ def _torch_lsm(x, ex, payoff_fn, basis_fn, df, num_calib, dtype):
     lhs = torch.matmul(sub.transpose(1, 2).contiguous(), sub)
     rhs = torch.matmul(sub.transpose(1, 2).contiguous(),
                        y.T.unsqueeze(-1))
-    beta = torch.matmul(torch.linalg.pinv(lhs), rhs)
+    # tf.linalg.pinv cutoff: 10 * max(M, N) * eps of the largest singular
+    # value (lsm.py:372)
+    rtol = 10 * lhs.shape[-1] * torch.finfo(lhs.dtype).eps
+    beta = torch.matmul(torch.linalg.pinv(lhs, rtol=rtol), rhs)
     c = torch.relu(torch.matmul(dt, beta).squeeze(-1).T)
     upd = exv > c
     new_vals = torch.where(upd, torch.zeros_like(cash), cash + vals)
