@@ -64,10 +64,12 @@ int tqf_sobol_points(const uint32_t* v, int32_t dim, int32_t nd, int64_t n,
                      const uint32_t* shift, int32_t out_kind, int32_t tent,
                      void* out, int64_t out_stride, void* stream);
 
-/* Device.  lo[d][256] = XOR of v[d][b] over the set bits b < 8 of the table
- * index: the per-lane part of a point, used by the fused steppers. */
-int tqf_sobol_lo_table(const uint32_t* v, int32_t dim, uint32_t* lo,
-                       void* stream);
+/* Device.  lo = XOR of v[d][b] over the set bits b < 8 of the table index
+ * t: the per-lane part of a point, used by the fused steppers.  Entry
+ * (d, t) is stored at ((d / group) * 256 + t) * group + d % group, so the
+ * `group` dimensions a step consumes are adjacent (group divides dim). */
+int tqf_sobol_lo_table(const uint32_t* v, int32_t dim, int32_t group,
+                       uint32_t* lo, void* stream);
 
 /* Device.  Normal draws from Sobol points, z = sqrt(2) erfinv(2u - 1),
  * out[i*dim + d] in `dtype`.  mode 0: the reference's transform of
@@ -149,7 +151,8 @@ typedef struct tqf_rng {
   int64_t stride_path;
   int64_t stride_step;
   const uint32_t* sobol_v;     /* SOBOL*: device [ndims][32]                */
-  const uint32_t* sobol_lo;    /* SOBOL*: device [ndims][256]               */
+  const uint32_t* sobol_lo;    /* SOBOL*: device [ndims/D][256][D] (D =
+                                  normals per step, tqf_sobol_lo_table group) */
   const uint32_t* sobol_shift; /* SOBOL*: device [ndims] or NULL            */
   int64_t sobol_first_index;   /* sequence index of path 0                  */
   int32_t sobol_nd;

@@ -80,9 +80,14 @@ def test_batch_heston_price_and_adjoint_vs_oracle_fd():
                            ('drho', 'rho', 1e-8)):
       fd = (_oracle_heston(c, z, field, h) - _oracle_heston(c, z, field, -h)) / (
           2 * h)
-      got = r[name][0]
+      # Same estimator, same draws: the float32 kernel and the float64
+      # oracle differ by rounding, which near v = 0 moves the 1/sqrt(v)
+      # factors of a few paths; the bound is 0.2 % or a tenth of the
+      # estimator's own standard error (Feller-violating contracts have very
+      # heavy-tailed pathwise variance Greeks).
+      got, se = r[name]
       scale = max(abs(fd), 1e-2 * abs(price))
-      assert abs(got - fd) <= 2e-3 * scale, (name, got, fd)
+      assert abs(got - fd) <= 2e-3 * scale + 0.1 * se, (name, got, fd, se)
 
 
 def test_batch_equals_loop_of_single_contracts():

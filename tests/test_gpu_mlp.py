@@ -40,8 +40,11 @@ def test_mlp_forward_matches_oracle():
                             bf=True)
   want, _ = omlp.forward(p.astype(np.float64), x.astype(np.float64))
   scale = np.max(np.abs(want))
-  # bf16 operands as the kernel: float32 accumulation order only
-  assert np.max(np.abs(got - want_bf)) <= 2e-5 * scale
+  # bf16 operands as the kernel: float32 accumulation order, and rarely a
+  # hidden activation that rounds to the neighbouring bf16 value
+  err = np.abs(got - want_bf)
+  assert np.max(err) <= 1e-4 * scale
+  assert np.quantile(err, 0.99) <= 1e-5 * scale
   # against the exact float64 network: bf16 rounding of the 64 x 64 layer
   assert np.max(np.abs(got - want)) <= 2e-2 * scale
 

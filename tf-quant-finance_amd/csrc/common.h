@@ -57,7 +57,9 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0,
                                            float& z1) {
   const float u = (float)((a >> 8) + 1u) * 0x1p-24f;
   const float t = (float)(b >> 8) * 0x1p-24f;
-  const float r = __builtin_amdgcn_sqrtf(-2.0f * __logf(u));
+  // v_log_f32 is log2: -2 ln u = -2 ln2 log2 u (u >= 2^-24, no denormals)
+  const float r =
+      __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u));
   z0 = r * __builtin_amdgcn_cosf(t);
   z1 = r * __builtin_amdgcn_sinf(t);
 }
@@ -79,7 +81,10 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& z0,
 // least squares against scipy (csrc/fit_erfinv.py); max relative error
 // 2.9e-7 over float32 inputs with 1 - x^2 rounded once.
 __device__ __forceinline__ float erfinv_f32(float x) {
-  float w = -__logf(__builtin_fmaf(-x, x, 1.0f));
+  // w = -ln(1 - x^2) = -ln2 log2(1 - x^2); 1 - x^2 >= 2^-24 here, so the
+  // raw v_log_f32 needs no denormal scaling
+  float w = -0.69314718055994531f *
+            __builtin_amdgcn_logf(__builtin_fmaf(-x, x, 1.0f));
   float p;
   if (w < 5.0f) {
     w = w - 2.5f;
@@ -127,6 +132,13 @@ __device__ __forceinline__ float sobol_shifted_normal(uint32_t v, float) {
 __device__ __forceinline__ double sobol_shifted_normal(uint32_t v, double) {
   const double q = ((double)v + 0.5) * 0x1p-31 - 1.0;
   return erfinv(q) * 1.4142135623730951;
+}
+
+// Uniform loads through the constant address space: per-step tables that
+// every lane reads at the same index become scalar (s_load) loads.
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T* p, int64_t i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
 }
 
 // ------------------------------------------------------------ wave helpers --

@@ -115,7 +115,9 @@ struct Stepper {
       x[1] = x1n;
     } else if (MODEL == TQF_MODEL_HESTON_QE) {
       // Andersen QE (heston_model.py:522-572), psi_c = 1.5, gamma = 1/2.
-      const R* c = a.coef + (int64_t)i * 12;
+      R c[10];
+#pragma unroll
+      for (int q = 0; q < 10; ++q) c[q] = ld_uniform(a.coef, (int64_t)i * 12 + q);
       if (c[9] != (R)0) {
         const R v = x[1];
         const R m = c[8] + (v - c[8]) * c[0];
@@ -227,14 +229,14 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   const int n_dims = a.n_steps * D;
   for (int i = 0; i < a.n_steps; ++i) {
     nrm.begin_step(a.rng, hx, i, n_dims);
-    const R dt = a.dt[i];
-    const R sdt = a.sqrt_dt[i];
+    const R dt = ld_uniform(a.dt, i);
+    const R sdt = ld_uniform(a.sqrt_dt, i);
     R z[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) z[k] = nrm.get(a.rng, hx, i, k);
     R lv_w = 0;
     if (MODEL == TQF_MODEL_LOCAL_VOL) {
-      const R t = a.t_next[i];
+      const R t = ld_uniform(a.t_next, i);
       while (lv_row + 1 < a.lv_nt && a.lv_t[lv_row + 1] <= t) ++lv_row;
       if (lv_row + 1 < a.lv_nt) {
         const R t0 = a.lv_t[lv_row], t1 = a.lv_t[lv_row + 1];
@@ -242,7 +244,7 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
       }
     }
     Stepper<R, MODEL>::step(a, x, z, dt, sdt, tab, lv_row, lv_w, i);
-    const int s = a.slot[i];
+    const int s = ld_uniform(a.slot, i);
     if (s >= 0) record<R, MODEL>(a, lds_pay, s, x, path, active, n_active);
   }
 

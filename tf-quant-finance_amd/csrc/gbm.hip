@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(kPathBlock) gbm_exact_kernel(GbmArgs<R> a) {
   const int n_dims = a.n_steps * d;
   for (int i = 0; i < a.n_steps; ++i) {
     nrm.begin_step(a.rng, hx, i, n_dims);
-    const R dt = a.dt[i], sdt = a.sqrt_dt[i];
+    const R dt = ld_uniform(a.dt, i), sdt = ld_uniform(a.sqrt_dt, i);
     R z[MAXD];
 #pragma unroll
     for (int c = 0; c < MAXD; ++c) z[c] = c < d ? nrm.get(a.rng, hx, i, c) : (R)0;
@@ -80,15 +80,16 @@ __global__ void __launch_bounds__(kPathBlock) gbm_exact_kernel(GbmArgs<R> a) {
         if (r < d) {
           R w = 0;
 #pragma unroll
-          for (int c = 0; c <= r; ++c) w += a.chol[r * d + c] * z[c];
+          for (int c = 0; c <= r; ++c) w += ld_uniform(a.chol, r * d + c) * z[c];
           z[r] = w;
         }
       }
     }
 #pragma unroll
     for (int c = 0; c < MAXD; ++c)
-      if (c < d) cum[c] += a.drift[c] * dt + sdt * a.vol[c] * z[c];
-    const int s = a.slot[i];
+      if (c < d)
+        cum[c] += ld_uniform(a.drift, c) * dt + sdt * ld_uniform(a.vol, c) * z[c];
+    const int s = ld_uniform(a.slot, i);
     if (s >= 0) record(s, false);
   }
 }

@@ -8,8 +8,9 @@
 //                      normals (Box-Muller pairs), antithetic rows p + N/2
 //  - Sobol:            sequence index first + p, dimension j (the layout of
 //                      tff/models/utils.py:114-127); low byte of the index
-//                      from a [dim][256] table, high bits XOR-ed once per
-//                      chunk of steps into LDS for the block.
+//                      from a [steps][256][D] table (the D dimensions of a
+//                      step sit side by side, one wide load), high bits
+//                      XOR-ed once per chunk of steps into LDS for the block.
 #pragma once
 
 #include "common.h"
@@ -25,7 +26,7 @@ struct RngArgs {
   const R* draws;
   int64_t stride_path, stride_step;
   const uint32_t* sv;      // [ndims][32] generating columns
-  const uint32_t* slo;     // [ndims][256] low-byte XOR table
+  const uint32_t* slo;     // [ndims / D][256][D] low-byte XOR table
   const uint32_t* sshift;  // [ndims] or null
   int64_t sfirst;          // sequence index of path 0
   R sscale;                // 2^-nd
@@ -133,7 +134,8 @@ struct Normals {
       return sign * z;
     } else {
       const int jj = j - chunk_base * D;
-      const uint32_t v = a.slo[(int64_t)j * 256 + lo_idx] ^
+      // low-byte table interleaved by step: [step][256][D]
+      const uint32_t v = a.slo[((int64_t)i * 256 + lo_idx) * D + k] ^
                          hx[hi_sel * (kSobolChunk * D) + jj];
       if (RNG == TQF_RNG_SOBOL) return sobol_ref_normal<R>(v, a.sscale);
       return sobol_shifted_normal(v, R(0));

@@ -78,14 +78,16 @@ sobol_points_kernel(const uint32_t* __restrict__ v, int dim, double scale,
   }
 }
 
+// Entry (d, t) at ((d / group) * 256 + t) * group + d % group.
 __global__ void __launch_bounds__(256)
-sobol_lo_kernel(const uint32_t* __restrict__ v, int dim,
+sobol_lo_kernel(const uint32_t* __restrict__ v, int dim, int group,
                 uint32_t* __restrict__ lo) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)dim * 256) return;
   const int64_t d = g >> 8;
   const uint32_t t = (uint32_t)(g & 255);
-  lo[g] = sobol_point(v + d * 32, t, 0u);
+  lo[((d / group) * 256 + t) * group + d % group] =
+      sobol_point(v + d * 32, t, 0u);
 }
 
 template <typename R>
@@ -137,13 +139,15 @@ extern "C" int tqf_sobol_points(const uint32_t* v, int32_t dim, int32_t nd,
   return check_launch("tqf_sobol_points");
 }
 
-extern "C" int tqf_sobol_lo_table(const uint32_t* v, int32_t dim, uint32_t* lo,
-                                  void* stream) {
+extern "C" int tqf_sobol_lo_table(const uint32_t* v, int32_t dim,
+                                  int32_t group, uint32_t* lo, void* stream) {
   clear_error();
   TQF_REQUIRE(dim >= 1, "dim must be positive");
+  TQF_REQUIRE(group >= 1 && dim % group == 0,
+              "group must divide dim (dim %d, group %d)", dim, group);
   TQF_REQUIRE(v != nullptr && lo != nullptr, "null pointer argument");
   hipLaunchKernelGGL(sobol_lo_kernel, dim3(blocks_for((int64_t)dim * 256)),
-                     dim3(256), 0, (hipStream_t)stream, v, dim, lo);
+                     dim3(256), 0, (hipStream_t)stream, v, dim, group, lo);
   return check_launch("tqf_sobol_lo_table");
 }
 

@@ -88,7 +88,8 @@ _SIGS = {
     'tqf_sobol_points': (_i32, [_c_void_p, _i32, _i32, _i64, _i64, _c_void_p,
                                 _c_void_p, _i32, _i32, _c_void_p, _i64,
                                 _c_void_p]),
-    'tqf_sobol_lo_table': (_i32, [_c_void_p, _i32, _c_void_p, _c_void_p]),
+    'tqf_sobol_lo_table': (_i32, [_c_void_p, _i32, _i32, _c_void_p,
+                                  _c_void_p]),
     'tqf_sobol_normals': (_i32, [_c_void_p, _i32, _i32, _i64, _i64, _c_void_p,
                                  _i32, _i32, _c_void_p, _c_void_p]),
     'tqf_philox_normals': (_i32, [_u64, _u32, _i64, _i64, _i32, _i32,

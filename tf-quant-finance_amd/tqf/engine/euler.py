@@ -254,7 +254,7 @@ def prepare(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
       nd = 32
       first = rng.first_index
     v = sobol_lib.device_columns(ndims, nd, dev=dev)
-    lo = sobol_lib.device_lo_table(ndims, nd, dev=dev)
+    lo = sobol_lib.device_lo_table(ndims, nd, group=d, dev=dev)
     r.sobol_v, r.sobol_lo = v.data_ptr(), lo.data_ptr()
     if rng.shift is not None:
       sh = dev_t(np.asarray(rng.shift, np.uint64).astype(np.uint32).view(

@@ -92,16 +92,17 @@ def device_columns(dim, num_digits, num_columns=None, dev=None):
   return t
 
 
-def device_lo_table(dim, num_digits, dev=None):
-  """[dim][256] low-byte XOR table for the fused steppers (cached)."""
+def device_lo_table(dim, num_digits, group=1, dev=None):
+  """Low-byte XOR table [dim/group][256][group] for the fused steppers."""
   dev = dev or _native.device()
-  key = ('lo', dim, num_digits, str(dev))
+  key = ('lo', dim, num_digits, group, str(dev))
   t = _dev_cache.get(key)
   if t is None:
     v = device_columns(dim, num_digits, dev=dev)
-    t = torch.empty((dim, 256), dtype=torch.int32, device=dev)
+    t = torch.empty((dim // group, 256, group), dtype=torch.int32, device=dev)
     _native.call('tqf_sobol_lo_table', _native.ptr(v), ctypes.c_int32(dim),
-                 _native.ptr(t), _native.stream_ptr(dev))
+                 ctypes.c_int32(group), _native.ptr(t),
+                 _native.stream_ptr(dev))
     _dev_cache[key] = t
   return t
 

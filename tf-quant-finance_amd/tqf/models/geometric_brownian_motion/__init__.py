@@ -106,7 +106,8 @@ def run_exact(times, drift, vol, chol, s0, num_samples, rng, dtype,
     nd = (sobol_lib.num_digits_for(rng.skip, num_samples)
           if rng.kind == 'sobol' else 32)
     r.sobol_v = sobol_lib.device_columns(ndims, nd, dev=dev).data_ptr()
-    r.sobol_lo = sobol_lib.device_lo_table(ndims, nd, dev=dev).data_ptr()
+    r.sobol_lo = sobol_lib.device_lo_table(ndims, nd, group=d,
+                                           dev=dev).data_ptr()
     r.sobol_first_index = rng.skip + 1 if rng.kind == 'sobol' else rng.first_index
     r.sobol_nd, r.sobol_ndims = nd, ndims
     if rng.shift is not None:
