@@ -108,6 +108,8 @@ int tqf_philox_normals(uint64_t seed, uint32_t stream_id, int64_t n_paths,
 #define TQF_OUT_PATH_MAJOR 0  /* out[p][slot][d]  (reference layout)        */
 #define TQF_OUT_STEP_MAJOR 1  /* out[slot][d][p]  (coalesced path store)    */
 #define TQF_OUT_PAYOFF 2      /* European calls reduced in the kernel       */
+#define TQF_OUT_PAYOFF_PARTIALS 3 /* as PAYOFF, per-block sums only: the
+                                     caller runs tqf_payoff_finalize */
 
 typedef struct tqf_model {
   int32_t kind;
@@ -172,9 +174,20 @@ typedef struct tqf_output {
                                 (sum, sum of squares) of raw payoffs, or NULL */
 } tqf_output;
 
-/* Workspace bytes for TQF_OUT_PAYOFF. */
+/* Workspace bytes for TQF_OUT_PAYOFF and TQF_OUT_PAYOFF_PARTIALS. */
 int64_t tqf_euler_workspace_size(int64_t n_paths, int32_t n_slots,
                                  int32_t n_strikes);
+
+/* Device.  Second stage of TQF_OUT_PAYOFF, run by tqf_euler itself in that
+ * mode: reduces the per-block payoff sums a TQF_OUT_PAYOFF_PARTIALS launch
+ * left in `workspace` to discounted prices, standard errors and (if `sums`
+ * is not NULL) raw sums.  Replaces the reduce_mean over samples that follows
+ * euler_sampling.sample in the reference's pricing notebooks (tff/models/
+ * euler_sampling.py:27-195 returns the paths; the mean is the caller's). */
+int tqf_payoff_finalize(double* workspace, int64_t n_paths, int32_t n_slots,
+                        int32_t n_strikes, const double* discount,
+                        double* prices, double* stderrs, double* sums,
+                        void* stream);
 
 /* Device.  Fused Euler-Maruyama: draws, steps, records and (optionally)
  * payoffs in one kernel with the state in registers.  x0: device

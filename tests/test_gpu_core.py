@@ -241,6 +241,16 @@ def test_euler_payoff_mode_matches_recorded_paths():
                              rtol=1e-12)
   np.testing.assert_allclose(
       _np(res['stderrs']), disc * pay.std(0, ddof=1) / np.sqrt(n), rtol=1e-9)
+  # the two-call form (per-block sums, then tqf_payoff_finalize) gives the
+  # same bits; 196 blocks = 6 full chunks of 32 and a ragged one
+  lz = engine.prepare(spec, all_times, keep, 2, x0, n, rng, torch.float64,
+                      mode='payoff', strikes=strikes,
+                      discount=np.exp(-0.03 * np.array([0.5, 1.0])))
+  paths_call, finalize_call = lz.launch_split()
+  paths_call()
+  split = finalize_call()
+  for key in ('prices', 'stderrs', 'sums'):
+    np.testing.assert_array_equal(_np(split[key]), _np(res[key]))
   # store mode is the transpose of path mode
   store = engine.run(spec, all_times, keep, 2, x0, n, rng, torch.float64,
                      mode='store')

@@ -265,12 +265,32 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
 
 // One workgroup per output: strided partial sums over the blocks, then a
 // fixed-shape LDS tree (deterministic), then mean / standard error.
+// Level 1 of the block-partial reduction: thread t of workgroup (c, y) sums
+// entry y*256 + t (of the 2 * n_out interleaved sums) over chunk c of
+// kReduceChunk consecutive blocks; consecutive threads read consecutive
+// doubles.
+constexpr int kReduceChunk = 32;
+
+__global__ void __launch_bounds__(256)
+payoff_reduce_kernel(const double* __restrict__ partials, int64_t n_blocks,
+                     int n_out2, double* __restrict__ chunks) {
+  const int e = blockIdx.y * 256 + threadIdx.x;
+  if (e >= n_out2) return;
+  const int64_t b0 = (int64_t)blockIdx.x * kReduceChunk;
+  int64_t b1 = b0 + kReduceChunk;
+  if (b1 > n_blocks) b1 = n_blocks;
+  double s = 0;
+  for (int64_t b = b0; b < b1; ++b) s += partials[b * n_out2 + e];
+  chunks[(int64_t)blockIdx.x * n_out2 + e] = s;
+}
+
 __global__ void __launch_bounds__(256)
 payoff_finalize_kernel(const double* __restrict__ partials, int64_t n_blocks,
                        int n_out, int n_strikes, int64_t n_paths,
                        const double* __restrict__ discount,
                        double* __restrict__ prices,
                        double* __restrict__ stderrs, double* __restrict__ sums) {
+  // `partials` here are the level-1 chunk sums, [n_blocks][n_out][2]
   __shared__ double r1[256], r2[256];
   const int o = blockIdx.x;
   double s1 = 0, s2 = 0;
@@ -304,6 +324,27 @@ payoff_finalize_kernel(const double* __restrict__ partials, int64_t n_blocks,
   }
 }
 
+// Reduces the per-block payoff sums in `ws` (layout [block][slot][strike]
+// [2]) to prices and standard errors: level 1 into chunk sums stored after
+// the block sums, level 2 one workgroup per output.
+int launch_finalize(double* ws, int64_t n_paths, int n_slots, int n_strikes,
+                    const double* discount, double* prices, double* stderrs,
+                    double* sums, hipStream_t stream) {
+  const int64_t n_blocks = (n_paths + kBlock - 1) / kBlock;
+  const int n_out = n_slots * n_strikes;
+  const int64_t n_chunks = (n_blocks + kReduceChunk - 1) / kReduceChunk;
+  double* chunks = ws + n_blocks * n_out * 2;
+  hipLaunchKernelGGL(payoff_reduce_kernel,
+                     dim3((unsigned)n_chunks, (unsigned)((2 * n_out + 255) / 256)),
+                     dim3(256), 0, stream, ws, n_blocks, 2 * n_out, chunks);
+  int rc = check_launch("payoff reduce");
+  if (rc) return rc;
+  hipLaunchKernelGGL(payoff_finalize_kernel, dim3(n_out), dim3(256), 0, stream,
+                     chunks, n_chunks, n_out, n_strikes, n_paths, discount,
+                     prices, stderrs, sums);
+  return check_launch("payoff finalize");
+}
+
 template <typename R>
 int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
                  const tqf_output* out, int64_t n_paths, const void* x0,
@@ -332,7 +373,7 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   a.n_slots = g->n_slots;
   a.init_slot = g->init_slot;
   fill_rng_args(a.rng, r, n_paths);
-  a.mode = out->mode;
+  a.mode = out->mode == TQF_OUT_PAYOFF_PARTIALS ? TQF_OUT_PAYOFF : out->mode;
   a.n_strikes = out->n_strikes;
   a.paths = (R*)out->paths;
   a.strikes = out->strikes;
@@ -350,7 +391,7 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   }
   lds = (lds + 15) & ~(size_t)15;
   a.lds_pay_off = (int)lds;
-  if (out->mode == TQF_OUT_PAYOFF)
+  if (a.mode == TQF_OUT_PAYOFF)
     lds += (size_t)kWaves * g->n_slots * out->n_strikes * 2 * sizeof(R);
   if (lds == 0) lds = 16;
 
@@ -375,14 +416,10 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   hipLaunchKernelGGL(kern, dim3(n_blocks), dim3(kBlock), lds, stream, a);
   int rc = check_launch("tqf_euler");
   if (rc) return rc;
-  if (out->mode == TQF_OUT_PAYOFF) {
-    const int n_out = g->n_slots * out->n_strikes;
-    hipLaunchKernelGGL(payoff_finalize_kernel, dim3(n_out), dim3(256), 0,
-                       stream, out->workspace, (int64_t)n_blocks,
-                       n_out, out->n_strikes, n_paths, out->discount,
-                       out->prices, out->stderrs, out->sums);
-    rc = check_launch("tqf_euler finalize");
-  }
+  if (out->mode == TQF_OUT_PAYOFF)
+    rc = launch_finalize(out->workspace, n_paths, g->n_slots, out->n_strikes,
+                         out->discount, out->prices, out->stderrs, out->sums,
+                         stream);
   return rc;
 }
 
@@ -391,7 +428,9 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
 extern "C" int64_t tqf_euler_workspace_size(int64_t n_paths, int32_t n_slots,
                                             int32_t n_strikes) {
   const int64_t n_blocks = (n_paths + kBlock - 1) / kBlock;
-  return n_blocks * (int64_t)n_slots * n_strikes * 2 * (int64_t)sizeof(double);
+  const int64_t n_chunks = (n_blocks + kReduceChunk - 1) / kReduceChunk;
+  return (n_blocks + n_chunks) * (int64_t)n_slots * n_strikes * 2 *
+         (int64_t)sizeof(double);
 }
 
 extern "C" int tqf_euler(const tqf_model* m, const tqf_grid* g,
@@ -448,11 +487,15 @@ extern "C" int tqf_euler(const tqf_model* m, const tqf_grid* g,
                 "bad local volatility table");
     TQF_REQUIRE(g->t_next != nullptr, "null t_next");
   }
-  TQF_REQUIRE(out->mode >= TQF_OUT_PATH_MAJOR && out->mode <= TQF_OUT_PAYOFF,
+  TQF_REQUIRE(out->mode >= TQF_OUT_PATH_MAJOR &&
+                  out->mode <= TQF_OUT_PAYOFF_PARTIALS,
               "unknown output mode %d", out->mode);
-  if (out->mode == TQF_OUT_PAYOFF) {
+  if (out->mode == TQF_OUT_PAYOFF || out->mode == TQF_OUT_PAYOFF_PARTIALS) {
     TQF_REQUIRE(out->n_strikes >= 1 && out->strikes && out->workspace,
                 "payoff mode needs strikes and a workspace");
+    TQF_REQUIRE(out->mode == TQF_OUT_PAYOFF_PARTIALS ||
+                    (out->discount && out->prices && out->stderrs),
+                "payoff mode needs discount, prices and stderrs");
     const int64_t lds = (int64_t)kWaves * g->n_slots * out->n_strikes * 2 *
                         dtype_size(dtype);
     TQF_REQUIRE(lds <= 48 * 1024,
@@ -465,4 +508,19 @@ extern "C" int tqf_euler(const tqf_model* m, const tqf_grid* g,
   if (dtype == TQF_F32)
     return launch_typed<float>(m, g, r, out, n_paths, x0, x0_stride_path, s);
   return launch_typed<double>(m, g, r, out, n_paths, x0, x0_stride_path, s);
+}
+
+extern "C" int tqf_payoff_finalize(double* workspace, int64_t n_paths,
+                                   int32_t n_slots, int32_t n_strikes,
+                                   const double* discount, double* prices,
+                                   double* stderrs, double* sums,
+                                   void* stream) {
+  clear_error();
+  TQF_REQUIRE(n_paths >= 2, "need at least two paths, got %lld",
+              (long long)n_paths);
+  TQF_REQUIRE(n_slots >= 1 && n_strikes >= 1, "bad payoff sizes");
+  TQF_REQUIRE(workspace && discount && prices && stderrs,
+              "null payoff buffers");
+  return launch_finalize(workspace, n_paths, n_slots, n_strikes, discount,
+                         prices, stderrs, sums, (hipStream_t)stream);
 }

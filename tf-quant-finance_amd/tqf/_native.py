@@ -17,7 +17,7 @@ MODEL_GBM, MODEL_CONST, MODEL_HESTON_FT, MODEL_LOCAL_VOL, MODEL_HESTON_QE = (
     0, 1, 2, 3, 4)
 RNG_DRAWS, RNG_PHILOX, RNG_PHILOX_ANTITHETIC, RNG_SOBOL, RNG_SOBOL_SHIFTED = (
     0, 1, 2, 3, 4)
-OUT_PATH_MAJOR, OUT_STEP_MAJOR, OUT_PAYOFF = 0, 1, 2
+OUT_PATH_MAJOR, OUT_STEP_MAJOR, OUT_PAYOFF, OUT_PAYOFF_PARTIALS = 0, 1, 2, 3
 
 _c_void_p = ctypes.c_void_p
 _i32, _i64, _u32, _u64, _f64 = (ctypes.c_int32, ctypes.c_int64,
@@ -98,6 +98,9 @@ _SIGS = {
     'tqf_euler': (_i32, [ctypes.POINTER(Model), ctypes.POINTER(Grid),
                          ctypes.POINTER(Rng), ctypes.POINTER(Output), _i64,
                          _c_void_p, _i64, _i32, _c_void_p]),
+    'tqf_payoff_finalize': (_i32, [_c_void_p, _i64, _i32, _i32, _c_void_p,
+                                   _c_void_p, _c_void_p, _c_void_p,
+                                   _c_void_p]),
     'tqf_gbm_exact': (_i32, [ctypes.POINTER(Gbm), ctypes.POINTER(Grid),
                              ctypes.POINTER(Rng), _i32, _c_void_p, _i64, _i32,
                              _c_void_p]),

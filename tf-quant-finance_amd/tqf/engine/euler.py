@@ -175,6 +175,38 @@ class Launch:
                  _native.stream_ptr(self.dev))
     return self.result
 
+  def launch_split(self):
+    """Payoff mode in two calls: the path kernel with per-block sums
+    (TQF_OUT_PAYOFF_PARTIALS), then tqf_payoff_finalize.  Same result as
+    `launch()`; returns the two callables so a caller can time or order
+    them separately."""
+    if self.o.mode != _native.OUT_PAYOFF:
+      raise ValueError('launch_split needs payoff mode')
+    o2 = _native.Output.from_buffer_copy(self.o)
+    o2.mode = _native.OUT_PAYOFF_PARTIALS
+    n_strikes = int(self.o.n_strikes)
+    n_slots = int(self.g.n_slots)
+
+    def paths():
+      _native.call('tqf_euler', ctypes.byref(self.m), ctypes.byref(self.g),
+                   ctypes.byref(self.r), ctypes.byref(o2),
+                   ctypes.c_int64(self.num_samples),
+                   ctypes.c_void_p(self.x0_t.data_ptr()),
+                   ctypes.c_int64(self.x0_stride),
+                   ctypes.c_int32(_native.dtype_code(self.dtype)),
+                   _native.stream_ptr(self.dev))
+
+    def finalize():
+      _native.call('tqf_payoff_finalize', ctypes.c_void_p(self.o.workspace),
+                   ctypes.c_int64(self.num_samples), ctypes.c_int32(n_slots),
+                   ctypes.c_int32(n_strikes),
+                   ctypes.c_void_p(self.o.discount),
+                   ctypes.c_void_p(self.o.prices),
+                   ctypes.c_void_p(self.o.stderrs),
+                   ctypes.c_void_p(self.o.sums), _native.stream_ptr(self.dev))
+      return self.result
+    return paths, finalize
+
 
 def run(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
         dtype, mode='paths', strikes=None, discount=None,
