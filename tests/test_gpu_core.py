@@ -323,6 +323,66 @@ def test_gbm_exact_moments_sobol_and_philox():
     assert abs(lg.var() - 0.04) < 3 * 0.04 * np.sqrt(2 / (lg.size - 1))
 
 
+# ------------------------------------------- float32 shifted-Sobol normals
+
+def test_shifted_normal_transform_exhaustive():
+  # dimension 0 with 32 digits is the bit reversal of the index: indices
+  # 0 .. 2^24 - 1 put every value of the top 24 bits through the kernel
+  import ctypes
+  from tqf import _native
+  n = 2**24
+  v = tsobol.device_columns(1, 32)
+  want = None
+  for shift in (None, 0x5A):  # the low 8 bits must not matter
+    out = torch.empty(n, dtype=torch.float32, device='cuda')
+    sh = None
+    if shift is not None:
+      sh = torch.tensor([shift], dtype=torch.int32, device='cuda')
+    _native.call('tqf_sobol_normals', _native.ptr(v), ctypes.c_int32(1),
+                 ctypes.c_int32(32), ctypes.c_int64(n), ctypes.c_int64(0),
+                 _native.ptr(sh) if sh is not None else None,
+                 ctypes.c_int32(1), ctypes.c_int32(_native.TQF_F32),
+                 _native.ptr(out), _native.stream_ptr(out.device))
+    got = _np(out).astype(np.float64)
+    if want is None:
+      k = np.arange(n, dtype=np.uint64)
+      rev = np.zeros(n, np.uint64)
+      for b in range(24):
+        rev |= ((k >> np.uint64(b)) & np.uint64(1)) << np.uint64(31 - b)
+      want = onormal.shifted_sobol_normals(rev.astype(np.uint32))
+      ref_bits = got
+    else:
+      np.testing.assert_array_equal(got, ref_bits)
+    err = np.abs(got - want)
+    big = np.abs(want) >= 1e-3
+    assert err.max() <= 5e-7, err.max()
+    assert (err[big] / np.abs(want[big])).max() <= 3.5e-7
+    assert err[~big].max() <= 2e-10
+
+
+def test_euler_shifted_sobol_parity():
+  # fused float32 Heston FT with the shifted-Sobol stream against the
+  # oracle driven by the same points through the exact inverse CDF
+  n, steps = 4096, 50
+  spec, fns, x0 = _models()['heston_ft']
+  shift = pricing.sobol_digital_shift(2 * steps, 7)
+  ints = oheston.shifted_sobol_ints(2 * steps, n, shift, first=5)
+  z = onormal.shifted_sobol_normals(ints.astype(np.uint32)).reshape(
+      n, steps, 2)
+  times = np.asarray([1.0], np.float32)
+  want = oeuler.sample(2, fns[0], fns[1], times, num_samples=n,
+                       initial_state=np.asarray(x0, np.float32),
+                       normal_draws=z.astype(np.float32),
+                       num_time_steps=steps, dtype=np.float32)
+  all_times, keep = pricing.heston_grid_for([1.0], steps, 'reference',
+                                            torch.float32)
+  rng = engine.RngSpec('sobol_shifted', shift=shift, first_index=5)
+  got = engine.run(spec, all_times, keep, 1, np.asarray(x0), n, rng,
+                   torch.float32, mode='paths')
+  # inverse CDF within 5e-7 of the exact one, float32 path rounding
+  _close(_np(got), want, 5e-5)
+
+
 # ---------------------------------------------------------------- Heston QE
 
 @pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12),

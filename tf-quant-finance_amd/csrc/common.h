@@ -152,6 +152,23 @@ __device__ __forceinline__ double read_lane(double v, int lane) {
   return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
 }
 
+__device__ __forceinline__ float fma_r(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double fma_r(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+
+// max(x, 0) as one integer max on the bit pattern: negative values (and
+// -0) have the sign bit set, so they compare below 0; NaN with the sign bit
+// clear passes through unchanged.
+__device__ __forceinline__ float pos_part(float x) {
+  return __int_as_float(max(__float_as_int(x), 0));
+}
+__device__ __forceinline__ double pos_part(double x) {
+  return __longlong_as_double(max(__double_as_longlong(x), 0ll));
+}
+
 __device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
 __device__ __forceinline__ double fast_exp(double x) { return exp(x); }
 __device__ __forceinline__ float fast_sqrt(float x) {

@@ -31,7 +31,7 @@ namespace {
 
 constexpr int kBlock = kPathBlock;
 constexpr int kWaves = kBlock / kWave;
-constexpr int kMaxLvLds = 8192;   // table entries staged in LDS
+constexpr int kMaxLvLdsBytes = 32768;  // local-vol tables up to this size go to LDS
 
 template <typename R>
 struct EulerArgs {
@@ -60,10 +60,16 @@ struct EulerArgs {
   int lds_pay_off;  // byte offset of the payoff area in dynamic LDS
 };
 
+// Kernel-internal model code: HESTON_FT with the variant flag p[5] != 0.
+constexpr int kModelHestonOwn = 16;
+
 template <int MODEL>
 struct ModelDim {
-  static constexpr int D =
-      (MODEL == TQF_MODEL_HESTON_FT || MODEL == TQF_MODEL_HESTON_QE) ? 2 : 1;
+  static constexpr int D = (MODEL == TQF_MODEL_HESTON_FT ||
+                            MODEL == TQF_MODEL_HESTON_QE ||
+                            MODEL == kModelHestonOwn)
+                               ? 2
+                               : 1;
 };
 
 // ---- local volatility lookup -----------------------------------------------
@@ -98,21 +104,21 @@ struct Stepper {
       x[0] = (x[0] + dt * drift) + vol * (z[0] * sdt);
     } else if (MODEL == TQF_MODEL_CONST) {
       x[0] = (x[0] + dt * a.p[0]) + a.p[1] * (z[0] * sdt);
-    } else if (MODEL == TQF_MODEL_HESTON_FT) {
-      // p: r, kappa, theta, xi*rho, xi*sqrt(1-rho^2), variant.
-      // variant 0: full truncation, v+ = max(v, 0) in drift and vol.
-      // variant 1: HestonModel's own callables (heston_model.py:143-173):
-      //            raw v in the drift, sqrt(|v|) in the vol.
-      const bool ft = a.p[5] == (R)0;
-      const R vpos = x[1] > (R)0 ? x[1] : (R)0;
-      const R vp = ft ? vpos : x[1];
-      const R sq = fast_sqrt(ft ? vpos : (x[1] < (R)0 ? -x[1] : x[1]));
-      const R dw0 = z[0] * sdt, dw1 = z[1] * sdt;
-      const R x0n = (x[0] + dt * (a.p[0] - (R)0.5 * vp)) + sq * dw0;
-      const R x1n = (x[1] + dt * (a.p[1] * (a.p[2] - vp))) +
-                    ((a.p[3] * sq) * dw0 + (a.p[4] * sq) * dw1);
-      x[0] = x0n;
-      x[1] = x1n;
+    } else if (MODEL == TQF_MODEL_HESTON_FT || MODEL == kModelHestonOwn) {
+      // p: r, kappa, theta, xi*rho, xi*sqrt(1-rho^2).
+      // HESTON_FT: full truncation, v+ = max(v, 0) in drift and vol.
+      // kModelHestonOwn: HestonModel's own callables (heston_model.py:
+      //   143-173): raw v in the drift, sqrt(|v|) in the vol.
+      // x' = (x + dt a(x)) + b(x) (z sqrt_dt), with sqrt_dt folded into the
+      // volatility factor and each "+ dt a" / "+ b z" a single FMA.
+      const R vp = MODEL == TQF_MODEL_HESTON_FT ? pos_part(x[1]) : x[1];
+      const R sq = fast_sqrt(MODEL == TQF_MODEL_HESTON_FT ? vp : fabs(x[1]));
+      const R ssq = sq * sdt;
+      const R a0 = fma_r(vp, (R)-0.5, a.p[0]);
+      const R a1 = fma_r(vp, -a.p[1], a.p[1] * a.p[2]);
+      const R c1 = fma_r(a.p[4], z[1], a.p[3] * z[0]);
+      x[0] = fma_r(ssq, z[0], fma_r(dt, a0, x[0]));
+      x[1] = fma_r(ssq, c1, fma_r(dt, a1, x[1]));
     } else if (MODEL == TQF_MODEL_HESTON_QE) {
       // Andersen QE (heston_model.py:522-572), psi_c = 1.5, gamma = 1/2.
       R c[10];
@@ -195,9 +201,7 @@ __device__ __forceinline__ void record(const EulerArgs<R>& a, R* lds_pay,
 template <typename R, int MODEL, int RNG>
 __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   constexpr int D = ModelDim<MODEL>::D;
-  constexpr bool kSobol = RNG == TQF_RNG_SOBOL || RNG == TQF_RNG_SOBOL_SHIFTED;
   extern __shared__ __align__(16) unsigned char lds[];
-  uint32_t* hx = (uint32_t*)lds;
   R* lds_pay = (R*)(lds + a.lds_pay_off);
 
   const int64_t path = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -210,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   // Local-vol table: stage in LDS when small.
   const R* tab = a.lv_vol;
   if (MODEL == TQF_MODEL_LOCAL_VOL && a.lv_in_lds) {
-    R* t = (R*)(lds + (kSobol ? sobol_lds_words(D) * 4 : 0));
+    R* t = (R*)(lds + rng_lds_bytes(RNG, D));
     for (int e = threadIdx.x; e < a.lv_nt * a.lv_nx; e += kBlock) t[e] = a.lv_vol[e];
     tab = t;
     __syncthreads();
@@ -221,19 +225,23 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   for (int c = 0; c < D; ++c) x[c] = a.x0[pc * a.x0_stride + c];
 
   Normals<R, RNG> nrm;
-  nrm.init(a.rng, pc, D);
+  nrm.init(a.rng, pc, D, lds);
 
   if (a.init_slot >= 0) record<R, MODEL>(a, lds_pay, a.init_slot, x, path, active, n_active);
 
   int lv_row = 0;
   const int n_dims = a.n_steps * D;
   for (int i = 0; i < a.n_steps; ++i) {
-    nrm.begin_step(a.rng, hx, i, n_dims);
+    nrm.begin_step(a.rng, i, n_dims);
     const R dt = ld_uniform(a.dt, i);
     const R sdt = ld_uniform(a.sqrt_dt, i);
     R z[D];
+    if (D == 2) {
+      nrm.get2(a.rng, i, z);
+    } else {
 #pragma unroll
-    for (int k = 0; k < D; ++k) z[k] = nrm.get(a.rng, hx, i, k);
+      for (int k = 0; k < D; ++k) z[k] = nrm.get(a.rng, i, k);
+    }
     R lv_w = 0;
     if (MODEL == TQF_MODEL_LOCAL_VOL) {
       const R t = ld_uniform(a.t_next, i);
@@ -382,11 +390,10 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   a.x0 = (const R*)x0;
   a.x0_stride = x0_stride;
 
-  const bool sobol = r->kind == TQF_RNG_SOBOL || r->kind == TQF_RNG_SOBOL_SHIFTED;
-  size_t lds = sobol ? (size_t)sobol_lds_words(D) * 4 : 0;
+  size_t lds = (size_t)rng_lds_bytes(r->kind, D);
   if (m->kind == TQF_MODEL_LOCAL_VOL) {
     const int64_t ne = (int64_t)m->lv_nt * m->lv_nx;
-    a.lv_in_lds = ne <= kMaxLvLds;
+    a.lv_in_lds = ne * (int64_t)sizeof(R) <= kMaxLvLdsBytes;
     if (a.lv_in_lds) lds += (size_t)ne * sizeof(R);
   }
   lds = (lds + 15) & ~(size_t)15;
@@ -397,8 +404,11 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
 
   const unsigned n_blocks = (unsigned)((n_paths + kBlock - 1) / kBlock);
   void (*kern)(EulerArgs<R>) = nullptr;
+  const int kind = (m->kind == TQF_MODEL_HESTON_FT && m->p[5] != 0.0)
+                       ? kModelHestonOwn
+                       : m->kind;
 #define TQF_PICK(MODEL, RNG) \
-  if (m->kind == MODEL && r->kind == RNG) kern = euler_kernel<R, MODEL, RNG>;
+  if (kind == MODEL && r->kind == RNG) kern = euler_kernel<R, MODEL, RNG>;
 #define TQF_PICK_RNG(MODEL)                       \
   TQF_PICK(MODEL, TQF_RNG_DRAWS)                  \
   TQF_PICK(MODEL, TQF_RNG_PHILOX)                 \
@@ -408,6 +418,7 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   TQF_PICK_RNG(TQF_MODEL_GBM)
   TQF_PICK_RNG(TQF_MODEL_CONST)
   TQF_PICK_RNG(TQF_MODEL_HESTON_FT)
+  TQF_PICK_RNG(kModelHestonOwn)
   TQF_PICK_RNG(TQF_MODEL_LOCAL_VOL)
   TQF_PICK_RNG(TQF_MODEL_HESTON_QE)
 #undef TQF_PICK_RNG

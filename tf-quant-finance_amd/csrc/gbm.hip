@@ -37,7 +37,6 @@ struct GbmArgs {
 template <typename R, int RNG, int MAXD>
 __global__ void __launch_bounds__(kPathBlock) gbm_exact_kernel(GbmArgs<R> a) {
   extern __shared__ __align__(16) unsigned char lds[];
-  uint32_t* hx = (uint32_t*)lds;
   const int64_t path = (int64_t)blockIdx.x * kPathBlock + threadIdx.x;
   const bool active = path < a.n_paths;
   const int64_t pc = active ? path : a.n_paths - 1;
@@ -50,7 +49,7 @@ __global__ void __launch_bounds__(kPathBlock) gbm_exact_kernel(GbmArgs<R> a) {
     s0[c] = c < d ? a.s0[pc * a.s0_stride + c] : (R)0;
   }
   Normals<R, RNG> nrm;
-  nrm.init(a.rng, pc, d);
+  nrm.init(a.rng, pc, d, lds);
 
   auto record = [&](int slot, bool initial) {
     if (!active) return;
@@ -69,11 +68,11 @@ __global__ void __launch_bounds__(kPathBlock) gbm_exact_kernel(GbmArgs<R> a) {
 
   const int n_dims = a.n_steps * d;
   for (int i = 0; i < a.n_steps; ++i) {
-    nrm.begin_step(a.rng, hx, i, n_dims);
+    nrm.begin_step(a.rng, i, n_dims);
     const R dt = ld_uniform(a.dt, i), sdt = ld_uniform(a.sqrt_dt, i);
     R z[MAXD];
 #pragma unroll
-    for (int c = 0; c < MAXD; ++c) z[c] = c < d ? nrm.get(a.rng, hx, i, c) : (R)0;
+    for (int c = 0; c < MAXD; ++c) z[c] = c < d ? nrm.get(a.rng, i, c) : (R)0;
     if (a.chol) {
 #pragma unroll
       for (int r = MAXD - 1; r >= 0; --r) {
@@ -141,8 +140,8 @@ int launch(const tqf_gbm* g, const tqf_grid* grid, const tqf_rng* r,
   a.mode = mode;
   a.out = (R*)out;
   a.n_paths = n_paths;
-  const bool sobol = r->kind == TQF_RNG_SOBOL || r->kind == TQF_RNG_SOBOL_SHIFTED;
-  const size_t lds = sobol ? (size_t)sobol_lds_words(g->dim) * 4 : 16;
+  size_t lds = (size_t)rng_lds_bytes(r->kind, g->dim);
+  if (lds == 0) lds = 16;
   const unsigned blocks = (unsigned)((n_paths + kPathBlock - 1) / kPathBlock);
   if (g->dim <= 1) return launch_d<R, 1>(a, r->kind, blocks, lds, s);
   if (g->dim <= 4) return launch_d<R, 4>(a, r->kind, blocks, lds, s);

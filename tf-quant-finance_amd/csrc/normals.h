@@ -14,6 +14,7 @@
 #pragma once
 
 #include "common.h"
+#include "icdf_table.h"
 
 namespace tqf {
 
@@ -53,10 +54,71 @@ inline void fill_rng_args(RngArgs<R>& a, const tqf_rng* r, int64_t n_paths) {
 constexpr int sobol_lds_words(int D) { return 2 * kSobolChunk * D; }
 
 
+// ---- float32 shifted-Sobol inverse CDF (piecewise cubic, table in LDS) ----
+// z = Phi^-1((m + 1/2) 2^-24) of the top 24 bits m of a 32-bit point.  The
+// point folds onto the lower half (t = m, or 2^24 - 1 - m when the top bit
+// is set, which also selects the negated copy of the table); the float32
+// value of t' * 128 with t' = 2t + 1 (exact) indexes a segment by its
+// exponent and top 3 mantissa bits, and its low 20 mantissa bits minus 2^20
+// are the cubic's variable (csrc/fit_icdf.py derives the table; exhaustive
+// float32 error: 4.8e-7 absolute, 3.2e-7 relative for |z| >= 1e-3).
+//
+// LDS image: segment s sits at byte ((s + 1072) & 255) * 16, which is bits
+// 20..27 of the float's bit pattern times 16 (one AND of its upper half):
+// segments 0..191 land on bytes [768, 3840); the negated copy is 4096 bytes
+// above (bit 12, set from the sign mask with one AND-OR).
+constexpr int kIcdfHalf = 4096;
+constexpr int kIcdfSlot0 = (134 << kIcdfSegBits) & 255;  // 48
+constexpr int kIcdfLdsBytes = 2 * kIcdfHalf;
+static_assert(kIcdfSegBits == 3, "the LDS addressing assumes 8 sub-segments");
+static_assert((kIcdfSlot0 + kIcdfSegs) * 16 <= kIcdfHalf,
+              "a table half must fit below bit 12");
+
+// The table's block-shared copy (static LDS of every kernel that uses it).
+__device__ __forceinline__ unsigned char* icdf_lds() {
+  __shared__ __align__(16) unsigned char t[kIcdfLdsBytes];
+  return t;
+}
+
+// Cooperative copy of both halves into LDS (caller synchronises).
+__device__ __forceinline__ void icdf_stage(unsigned char* lds) {
+  float4* t = (float4*)lds;
+  for (int e = threadIdx.x; e < kIcdfSegs; e += blockDim.x) {
+    const float4 c = make_float4(kIcdfCoef[4 * e], kIcdfCoef[4 * e + 1],
+                                 kIcdfCoef[4 * e + 2], kIcdfCoef[4 * e + 3]);
+    t[kIcdfSlot0 + e] = c;
+    t[kIcdfHalf / 16 + kIcdfSlot0 + e] = make_float4(-c.x, -c.y, -c.z, -c.w);
+  }
+}
+
+__device__ __forceinline__ float icdf_shifted_f32(uint32_t v,
+                                                  const unsigned char* lds) {
+  const uint32_t s = (uint32_t)((int32_t)v >> 31);
+  const uint32_t w = ((v ^ s) & 0xFFFFFF80u) | 0x80u;  // t' * 128
+  const uint32_t fi = __float_as_uint((float)w);       // exact
+  // t' * 128 lies in [2^7, 2^31): exponent 134..157, so bits 20..27 of fi
+  // are 48..239 (no wrap) and the offset stays inside its half
+  const uint32_t off = (fi >> 16) & 0xFF0u;
+  const float g = (float)(int32_t)(fi | 0xFFF00000u);  // [-2^20, -1]
+  const float4 c = *(const float4*)(lds + (off | (s & (uint32_t)kIcdfHalf)));
+  return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.w, g, c.z), g, c.y),
+                        g, c.x);
+}
+
+// Bytes of dynamic LDS the normal source of a kernel needs (the Sobol
+// high-bit words; the ICDF table is static LDS).
+__host__ __device__ constexpr int rng_lds_bytes(int rng_kind, int D) {
+  return (rng_kind == TQF_RNG_SOBOL || rng_kind == TQF_RNG_SOBOL_SHIFTED)
+             ? sobol_lds_words(D) * 4
+             : 0;
+}
+
 template <typename R, int RNG>
 struct Normals {
   static constexpr bool kSobol =
       RNG == TQF_RNG_SOBOL || RNG == TQF_RNG_SOBOL_SHIFTED;
+  static constexpr bool kIcdf =
+      RNG == TQF_RNG_SOBOL_SHIFTED && sizeof(R) == sizeof(float);
   R c0, c1, c2, c3;  // Philox cache
   uint32_t ctr0, ctr3;
   R sign;
@@ -64,12 +126,22 @@ struct Normals {
   uint32_t lo_idx, hi_sel;
   int chunk_base;
   int D;  // normals per step
+  unsigned char* icdf;
+  uint32_t* hx;
 
-  __device__ void init(const RngArgs<R>& a, int64_t path, int d) {
+  // Every thread of the block calls init (it stages LDS tables; the first
+  // begin_step synchronises before any use).
+  __device__ void init(const RngArgs<R>& a, int64_t path, int d,
+                       unsigned char* lds) {
     D = d;
     p = path;
     sign = (R)1;
     chunk_base = 0;
+    hx = (uint32_t*)lds;
+    if constexpr (kIcdf) {
+      icdf = icdf_lds();
+      icdf_stage(icdf);
+    }
     if (RNG == TQF_RNG_PHILOX_ANTITHETIC) {
       const int64_t half = a.n_paths / 2;
       if (path >= half) {
@@ -90,8 +162,8 @@ struct Normals {
 
   // Call at the top of step i with every thread of the block (it may
   // synchronise).  n_dims bounds the Sobol dimensions that exist.
-  __device__ __forceinline__ void begin_step(const RngArgs<R>& a, uint32_t* hx,
-                                             int i, int n_dims) {
+  __device__ __forceinline__ void begin_step(const RngArgs<R>& a, int i,
+                                             int n_dims) {
     if (kSobol && (i % kSobolChunk) == 0) {
       chunk_base = i;
       __syncthreads();
@@ -117,8 +189,17 @@ struct Normals {
     }
   }
 
-  __device__ __forceinline__ R get(const RngArgs<R>& a, const uint32_t* hx,
-                                   int i, int k) {
+  __device__ __forceinline__ R sobol_value(const RngArgs<R>& a, uint32_t v) {
+    if constexpr (RNG == TQF_RNG_SOBOL) {
+      return sobol_ref_normal<R>(v, a.sscale);
+    } else if constexpr (kIcdf) {
+      return (R)icdf_shifted_f32(v, icdf);
+    } else {
+      return sobol_shifted_normal(v, R(0));
+    }
+  }
+
+  __device__ __forceinline__ R get(const RngArgs<R>& a, int i, int k) {
     const int j = i * D + k;
     if (RNG == TQF_RNG_DRAWS) {
       return a.draws[p * a.stride_path + (int64_t)i * a.stride_step + k];
@@ -137,8 +218,23 @@ struct Normals {
       // low-byte table interleaved by step: [step][256][D]
       const uint32_t v = a.slo[((int64_t)i * 256 + lo_idx) * D + k] ^
                          hx[hi_sel * (kSobolChunk * D) + jj];
-      if (RNG == TQF_RNG_SOBOL) return sobol_ref_normal<R>(v, a.sscale);
-      return sobol_shifted_normal(v, R(0));
+      return sobol_value(a, v);
+    }
+  }
+
+  // Both normals of a step with two (D == 2): the Sobol words come in one
+  // 8-byte global load and one 8-byte LDS read.
+  __device__ __forceinline__ void get2(const RngArgs<R>& a, int i, R* z) {
+    if (kSobol) {
+      const int jj = (i - chunk_base) * 2;
+      const uint32_t* row = a.slo + (int64_t)i * 512;  // uniform
+      const uint2 lo = *(const uint2*)(row + lo_idx * 2);
+      const uint2 hi = *(const uint2*)(hx + hi_sel * (kSobolChunk * 2) + jj);
+      z[0] = sobol_value(a, lo.x ^ hi.x);
+      z[1] = sobol_value(a, lo.y ^ hi.y);
+    } else {
+      z[0] = get(a, i, 0);
+      z[1] = get(a, i, 1);
     }
   }
 };

@@ -6,6 +6,7 @@
 // Gray-code, order as tff/math/random_ops/sobol/sobol_impl.py:139-157 and
 // tff/math/qmc/digital_net.py:379-399), XOR the digital shift.
 #include "common.h"
+#include "normals.h"
 
 using namespace tqf;
 
@@ -96,14 +97,30 @@ sobol_normals_kernel(const uint32_t* __restrict__ v, int dim, R scale,
                      int64_t n, int64_t first,
                      const uint32_t* __restrict__ shift, int mode,
                      R* __restrict__ out) {
+  // float32 shifted transform: the same LDS table as the fused steppers
+  constexpr bool kF32 = sizeof(R) == sizeof(float);
+  unsigned char* icdf = nullptr;
+  if constexpr (kF32) {
+    icdf = icdf_lds();
+    if (mode == 1) {
+      icdf_stage(icdf);
+      __syncthreads();
+    }
+  }
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n * (int64_t)dim) return;
   const int64_t i = g / dim;
   const int d = (int)(g - i * dim);
   const uint32_t x = sobol_point(v + (int64_t)d * 32, (uint64_t)(first + i),
                                  shift ? shift[d] : 0u);
-  out[g] = mode == 0 ? sobol_ref_normal<R>(x, scale)
-                     : sobol_shifted_normal(x, R(0));
+  R z;
+  if (mode == 0)
+    z = sobol_ref_normal<R>(x, scale);
+  else if constexpr (kF32)
+    z = (R)icdf_shifted_f32(x, icdf);
+  else
+    z = sobol_shifted_normal(x, R(0));
+  out[g] = z;
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
