@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   for (int c = 0; c < D; ++c) x[c] = a.x0[pc * a.x0_stride + c];
 
   Normals<R, RNG> nrm;
-  nrm.init(a.rng, pc, D, lds);
+  nrm.init(a.rng, pc, D, lds, a.n_steps);
 
   if (a.init_slot >= 0) record<R, MODEL>(a, lds_pay, a.init_slot, x, path, active, n_active);
 

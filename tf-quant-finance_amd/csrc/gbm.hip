@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kPathBlock) gbm_exact_kernel(GbmArgs<R> a) {
     s0[c] = c < d ? a.s0[pc * a.s0_stride + c] : (R)0;
   }
   Normals<R, RNG> nrm;
-  nrm.init(a.rng, pc, d, lds);
+  nrm.init(a.rng, pc, d, lds, a.n_steps);
 
   auto record = [&](int slot, bool initial) {
     if (!active) return;

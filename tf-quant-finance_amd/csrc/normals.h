@@ -19,6 +19,9 @@
 namespace tqf {
 
 constexpr int kPathBlock = 256;   // threads (= paths) per block
+// Word 3 of a raw (untyped) buffer descriptor on gfx9: data format 32.
+constexpr int kRawBufferWord3 = 0x00020000;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 constexpr int kSobolChunk = 128;  // steps per LDS refill of the high-bit XOR
 
 template <typename R>
@@ -28,6 +31,7 @@ struct RngArgs {
   int64_t stride_path, stride_step;
   const uint32_t* sv;      // [ndims][32] generating columns
   const uint32_t* slo;     // [ndims / D][256][D] low-byte XOR table
+  uint32_t slo_bytes;      // its size (bounds the buffer loads)
   const uint32_t* sshift;  // [ndims] or null
   int64_t sfirst;          // sequence index of path 0
   R sscale;                // 2^-nd
@@ -44,6 +48,8 @@ inline void fill_rng_args(RngArgs<R>& a, const tqf_rng* r, int64_t n_paths) {
   a.stride_step = r->stride_step;
   a.sv = r->sobol_v;
   a.slo = r->sobol_lo;
+  const int64_t lo_bytes = (int64_t)r->sobol_ndims * 256 * 4;
+  a.slo_bytes = (uint32_t)(lo_bytes < 0xffffffffll ? lo_bytes : 0xffffffffll);
   a.sshift = r->sobol_shift;
   a.sfirst = r->sobol_first_index;
   a.sscale = (R)std::ldexp(1.0, -r->sobol_nd);
@@ -98,9 +104,13 @@ __device__ __forceinline__ float icdf_shifted_f32(uint32_t v,
   const uint32_t fi = __float_as_uint((float)w);       // exact
   // t' * 128 lies in [2^7, 2^31): exponent 134..157, so bits 20..27 of fi
   // are 48..239 (no wrap) and the offset stays inside its half
-  const uint32_t off = (fi >> 16) & 0xFF0u;
+  uint32_t off;  // (fi >> 16) & 0xFF0: one AND reading the upper half-word
+  asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD "
+      "src0_sel:WORD_1 src1_sel:DWORD"
+      : "=v"(off)
+      : "v"(fi), "v"(0xFF0u));
   const float g = (float)(int32_t)(fi | 0xFFF00000u);  // [-2^20, -1]
-  const float4 c = *(const float4*)(lds + (off | (s & (uint32_t)kIcdfHalf)));
+  const float4 c = *(const float4*)(lds + ((s & (uint32_t)kIcdfHalf) | off));
   return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.w, g, c.z), g, c.y),
                         g, c.x);
 }
@@ -128,12 +138,16 @@ struct Normals {
   int D;  // normals per step
   unsigned char* icdf;
   uint32_t* hx;
+  __amdgpu_buffer_rsrc_t lo_rsrc;  // low-byte table (raw buffer, bounds-checked)
+  u32x2 lo_next;                   // get2: the next step's low-byte words
+  int last_step;
 
   // Every thread of the block calls init (it stages LDS tables; the first
   // begin_step synchronises before any use).
   __device__ void init(const RngArgs<R>& a, int64_t path, int d,
-                       unsigned char* lds) {
+                       unsigned char* lds, int n_steps) {
     D = d;
+    last_step = n_steps > 0 ? n_steps - 1 : 0;
     p = path;
     sign = (R)1;
     chunk_base = 0;
@@ -157,7 +171,17 @@ struct Normals {
           (uint64_t)(a.sfirst + (int64_t)blockIdx.x * kPathBlock) >> 8;
       lo_idx = (uint32_t)(idx & 255);
       hi_sel = (uint32_t)((idx >> 8) - hi0);
+      lo_rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.slo, (short)0,
+                                                  (int)a.slo_bytes, kRawBufferWord3);
+      if (D == 2 && n_steps > 0) lo_next = load_lo2(0);
     }
+  }
+
+  // Step i's two low-byte words: voffset = the lane's entry, soffset = the
+  // step's row (uniform).
+  __device__ __forceinline__ u32x2 load_lo2(int i) const {
+    return __builtin_amdgcn_raw_buffer_load_b64(lo_rsrc, lo_idx * 8u,
+                                                (uint32_t)i * 2048u, 0);
   }
 
   // Call at the top of step i with every thread of the block (it may
@@ -175,12 +199,20 @@ struct Normals {
         const int j = i * D + (t - c * n);
         uint32_t x = 0;
         if (j < n_dims) {
+          // columns 8..31 of dimension j in six 16-byte loads issued
+          // together, then masked by the index bits 8..31
+          const uint32_t hb = (uint32_t)((hi0 + c) & 0xffffffull);
+          const uint4* col = (const uint4*)(a.sv + (int64_t)j * 32 + 8);
+          uint4 cv[6];
+#pragma unroll
+          for (int q = 0; q < 6; ++q) cv[q] = col[q];
           x = a.sshift ? a.sshift[j] : 0u;
-          uint64_t hb = (hi0 + c) & 0xffffffull;  // index bits 8..31
-          const uint32_t* v = a.sv + (int64_t)j * 32 + 8;
-          while (hb) {
-            x ^= v[__builtin_ctzll(hb)];
-            hb &= hb - 1;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) {
+            x ^= cv[q].x & (0u - ((hb >> (4 * q)) & 1u));
+            x ^= cv[q].y & (0u - ((hb >> (4 * q + 1)) & 1u));
+            x ^= cv[q].z & (0u - ((hb >> (4 * q + 2)) & 1u));
+            x ^= cv[q].w & (0u - ((hb >> (4 * q + 3)) & 1u));
           }
         }
         hx[t] = x;
@@ -227,8 +259,9 @@ struct Normals {
   __device__ __forceinline__ void get2(const RngArgs<R>& a, int i, R* z) {
     if (kSobol) {
       const int jj = (i - chunk_base) * 2;
-      const uint32_t* row = a.slo + (int64_t)i * 512;  // uniform
-      const uint2 lo = *(const uint2*)(row + lo_idx * 2);
+      // software pipeline: this step's words were loaded one step ago
+      const u32x2 lo = lo_next;
+      lo_next = load_lo2(i < last_step ? i + 1 : last_step);
       const uint2 hi = *(const uint2*)(hx + hi_sel * (kSobolChunk * 2) + jj);
       z[0] = sobol_value(a, lo.x ^ hi.x);
       z[1] = sobol_value(a, lo.y ^ hi.y);
