@@ -97,6 +97,91 @@ def accuracy_check(prices, ses):
   }
 
 
+def contract_loop(contracts, seed=11, h=1e-3):
+  """Returns run(greeks) -> seconds for pricing `contracts` one tqf_euler
+  call each (uniform grid, Philox sub-stream per contract, payoff mode), the
+  loop a caller without the batched kernel would write.  With greeks=True
+  each Greek of the batched kernel is added by central differences with
+  common random numbers (relative bump h).  Every launch is prepared before
+  the timed loop."""
+  import ctypes
+  import torch
+  from tqf import _native
+  from tqf.engine import euler as engine
+  dev = torch.device('cuda', 0)
+  base, bumped = [], []
+  for i, c in enumerate(contracts):
+    n, T, r = c['n_steps'], c['maturity'], c['rate']
+    all_times = np.linspace(0.0, T, n + 1).astype(np.float32)
+    keep = np.zeros(n + 1, bool)
+    keep[-1] = True
+    rng = engine.RngSpec('philox', seed=seed, stream_id=c.get('stream_id', i))
+    if c['kind'] == 0:
+      spec = ('const', r - 0.5 * c['sigma']**2, c['sigma'])
+      x0 = [np.log(c['s0'])]
+    else:
+      spec = ('heston', r, c['kappa'], c['theta'], c['xi'], c['rho'], 0.0)
+      x0 = [np.log(c['s0']), c['v0']]
+    L = engine.prepare(spec, all_times, keep, 1, x0, c['n_paths'], rng,
+                       torch.float32, mode='payoff', strikes=[c['strike']],
+                       discount=[np.exp(-r * T)], dev=dev)
+    base.append(L)
+    # bumped variants share the grid, stream and workspace; each has its
+    # own model parameters or initial state and its own outputs
+    variants = []
+    if c['kind'] == 0:
+      sig = c['sigma']
+      bumps = [('x0', 0), ('p', ('sigma',)), ('p', ('r',))]
+    else:
+      bumps = [('x0', 0), ('x0', 1), ('p', ('r',)), ('p', (1,)), ('p', (2,)),
+               ('p', (3,)), ('p', (4,))]
+    for kind, what in bumps:
+      for sgn in (1.0, -1.0):
+        m = _native.Model.from_buffer_copy(L.m)
+        x0_t = L.x0_t
+        if kind == 'x0':
+          x = np.asarray(x0, np.float64).copy()
+          if what == 0:
+            x[0] += np.log1p(sgn * h)
+          else:
+            x[1] *= 1 + sgn * h
+          x0_t = torch.as_tensor(x, dtype=torch.float32, device=dev)
+        elif c['kind'] == 0:
+          sig2 = c['sigma'] * (1 + sgn * h) if what == ('sigma',) else c['sigma']
+          r2 = r * (1 + sgn * h) if what == ('r',) else r
+          m.p[0], m.p[1] = r2 - 0.5 * sig2 * sig2, sig2
+        else:
+          k = 0 if what == ('r',) else what[0]
+          m.p[k] = m.p[k] * (1 + sgn * h)
+        o = _native.Output.from_buffer_copy(L.o)
+        outs = torch.empty(4, dtype=torch.float64, device=dev)
+        o.prices, o.stderrs, o.sums = (outs.data_ptr(), outs.data_ptr() + 8,
+                                       0)
+        variants.append((m, o, x0_t, outs))
+    bumped.append(variants)
+
+  def call(L, m, o, x0_t):
+    _native.call('tqf_euler', ctypes.byref(m), ctypes.byref(L.g),
+                 ctypes.byref(L.r), ctypes.byref(o),
+                 ctypes.c_int64(L.num_samples),
+                 ctypes.c_void_p(x0_t.data_ptr()),
+                 ctypes.c_int64(L.x0_stride), ctypes.c_int32(0),
+                 _native.stream_ptr(dev))
+
+  def run(greeks):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for L, variants in zip(base, bumped):
+      L.launch()
+      if greeks:
+        for m, o, x0_t, _ in variants:
+          call(L, m, o, x0_t)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+  run(False)  # warm-up
+  return run
+
+
 def secondary_metrics():
   """One timed run each of the other configurations (not the headline)."""
   import torch
@@ -138,6 +223,31 @@ def secondary_metrics():
                        'seconds': t,
                        'path_steps': plan.total_path_steps,
                        'path_steps_per_s': plan.total_path_steps / t}
+  # the same contracts through the plain fused kernel, one call per contract
+  # (price only, and price + central-difference Greeks with common random
+  # numbers: 2 more calls per Greek)
+  loop = contract_loop(cs)
+  t_price = loop(False)
+  t_greeks = loop(True)
+  out['batch_4096']['loop_price_only_seconds'] = t_price
+  out['batch_4096']['loop_fd_greeks_seconds'] = t_greeks
+  out['batch_4096']['speedup_vs_loop_fd_greeks'] = t_greeks / t
+  out['batch_4096']['speedup_vs_loop_price_only'] = t_price / t
+  # learned LSM at full size: 5-asset Bermudan max-call, 2^20 training +
+  # 2^20 pricing paths, 9 dates x 20 steps, MLP vs degree-2 polynomial
+  from tqf.engine import mlp_lsm
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  paths, df = mlp_lsm.bermudan_max_call_paths(2**21)
+  res = mlp_lsm.mlp_lsm_price(paths, list(range(1, 10)), 100.0, df, 2**20)
+  torch.cuda.synchronize()
+  out['learned_lsm_5asset'] = {
+      'seconds': time.perf_counter() - t0,
+      'mlp_price': res['mlp_price'], 'mlp_se': res['mlp_se'],
+      'poly_price': res['poly_price'], 'poly_se': res['poly_se'],
+      'gain': res['gain'], 'gain_se': res['gain_se'],
+      'accepted': res['accepted']}
+  del paths
   # 16-asset basket with adjoint Greeks
   d = 16
   rs = np.random.RandomState(3)

@@ -271,8 +271,6 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   }
 }
 
-// One workgroup per output: strided partial sums over the blocks, then a
-// fixed-shape LDS tree (deterministic), then mean / standard error.
 // Level 1 of the block-partial reduction: thread t of workgroup (c, y) sums
 // entry y*256 + t (of the 2 * n_out interleaved sums) over chunk c of
 // kReduceChunk consecutive blocks; consecutive threads read consecutive
@@ -292,6 +290,8 @@ payoff_reduce_kernel(const double* __restrict__ partials, int64_t n_blocks,
   chunks[(int64_t)blockIdx.x * n_out2 + e] = s;
 }
 
+// Level 2: one workgroup per output sums the chunk sums with a fixed-shape
+// LDS tree (deterministic), then forms the mean and standard error.
 __global__ void __launch_bounds__(256)
 payoff_finalize_kernel(const double* __restrict__ partials, int64_t n_blocks,
                        int n_out, int n_strikes, int64_t n_paths,
