@@ -75,8 +75,8 @@ int tqf_sobol_lo_table(const uint32_t* v, int32_t dim, int32_t group,
  * out[i*dim + d] in `dtype`.  mode 0: the reference's transform of
  * sobol.sample (u = cast(int) 2^-nd, tff/math/random_ops/
  * multivariate_normal.py:356-424); mode 1: the engine's shifted transform
- * (u = (top24 + 1/2) 2^-24 for float32, evaluated with a piecewise-cubic
- * table within 4.8e-7 absolute / 3.2e-7 relative for |z| >= 1e-3, the same
+ * (u = (top25 + 1/2) 2^-25 for float32, evaluated with a piecewise-cubic
+ * table within 5.2e-7 absolute / 3.3e-7 relative for |z| >= 1e-3, the same
  * code the fused steppers run; (int + 1/2) 2^-32 for float64). */
 int tqf_sobol_normals(const uint32_t* v, int32_t dim, int32_t nd, int64_t n,
                       int64_t first_index, const uint32_t* shift,
@@ -132,7 +132,11 @@ typedef struct tqf_model {
                              constants: 0 e = exp(-kappa dt), 1 c1, 2 c2
                              (s^2 = v c1 + c2), 3..7 k0..k4 of the log-spot
                              update, 8 theta, 9 update flag (0 when
-                             dt <= tolerance), 10-11 unused */
+                             dt <= tolerance), 10-11 unused.
+                             HESTON_FT: device [n_steps][4] = {-dt/2,
+                             kappa theta dt, -kappa dt, r t_{i+1}} (r =
+                             p[0]); the kernel carries log S - r t and adds
+                             r t back where it records or prices. */
 } tqf_model;
 
 typedef struct tqf_grid {

@@ -3,8 +3,8 @@
   * quasi draws: z = erfinv((u - 0.5) * 2) * sqrt(2)
     (tff/math/random_ops/multivariate_normal.py:420), evaluated here in
     float64 with scipy.special.erfinv.
-  * the engine's shifted-Sobol transform: float32 uses the 24-bit uniform
-    u = (m + 1/2) 2^-24 of the top 24 bits m of a 32-bit point, float64 the
+  * the engine's shifted-Sobol transform: float32 uses the 25-bit uniform
+    u = (n + 1/2) 2^-25 of the top 25 bits n of a 32-bit point, float64 the
     32-bit u = (v + 1/2) 2^-32; q = 2u - 1 is formed exactly and
     z = sqrt(2) erfinv(q).
 
@@ -22,11 +22,11 @@ def quasi_normals(u):
 
 
 def shifted_sobol_normals(ints_u32, dtype=np.float32):
-  """float32 kernels use the top 24 bits, float64 kernels all 32 bits."""
+  """float32 kernels use the top 25 bits, float64 kernels all 32 bits."""
   v = np.asarray(ints_u32, np.uint64)
   if dtype == np.float64:
     q = (v.astype(np.float64) + 0.5) * 2.0**-31 - 1.0
   else:
-    m = (v >> np.uint64(8)).astype(np.float64)
-    q = (2 * m + 1 - 2.0**24) * 2.0**-24
+    n = (v >> np.uint64(7)).astype(np.float64)
+    q = (2 * n + 1 - 2.0**25) * 2.0**-25
   return special.erfinv(q) * np.sqrt(2.0)

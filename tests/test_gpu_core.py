@@ -327,13 +327,13 @@ def test_gbm_exact_moments_sobol_and_philox():
 
 def test_shifted_normal_transform_exhaustive():
   # dimension 0 with 32 digits is the bit reversal of the index: indices
-  # 0 .. 2^24 - 1 put every value of the top 24 bits through the kernel
+  # 0 .. 2^25 - 1 put every value of the top 25 bits through the kernel
   import ctypes
   from tqf import _native
-  n = 2**24
+  n = 2**25
   v = tsobol.device_columns(1, 32)
   want = None
-  for shift in (None, 0x5A):  # the low 8 bits must not matter
+  for shift in (None, 0x5A):  # the low 7 bits must not matter
     out = torch.empty(n, dtype=torch.float32, device='cuda')
     sh = None
     if shift is not None:
@@ -347,7 +347,7 @@ def test_shifted_normal_transform_exhaustive():
     if want is None:
       k = np.arange(n, dtype=np.uint64)
       rev = np.zeros(n, np.uint64)
-      for b in range(24):
+      for b in range(25):
         rev |= ((k >> np.uint64(b)) & np.uint64(1)) << np.uint64(31 - b)
       want = onormal.shifted_sobol_normals(rev.astype(np.uint32))
       ref_bits = got
@@ -355,7 +355,7 @@ def test_shifted_normal_transform_exhaustive():
       np.testing.assert_array_equal(got, ref_bits)
     err = np.abs(got - want)
     big = np.abs(want) >= 1e-3
-    assert err.max() <= 5e-7, err.max()
+    assert err.max() <= 5.5e-7, err.max()
     assert (err[big] / np.abs(want[big])).max() <= 3.5e-7
     assert err[~big].max() <= 2e-10
 

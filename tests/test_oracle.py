@@ -113,8 +113,8 @@ def test_normal_transforms():
                              rtol=1e-13, atol=1e-15)
   ints = np.array([0, 2**31, 2**32 - 1, 12345678], np.uint64)
   z = normal.shifted_sobol_normals(ints)
-  m = (ints >> np.uint64(8)).astype(np.float64)
-  np.testing.assert_allclose(z, special.ndtri((m + 0.5) / 2**24), rtol=1e-12)
+  n = (ints >> np.uint64(7)).astype(np.float64)
+  np.testing.assert_allclose(z, special.ndtri((n + 0.5) / 2**25), rtol=1e-12)
 
 
 # ---------------------------------------------------------------- grids ----

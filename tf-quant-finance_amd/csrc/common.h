@@ -124,11 +124,9 @@ __device__ __forceinline__ R sobol_ref_normal(uint32_t v, R scale) {
   return erfinv_any((u - (R)0.5) * (R)2) * (R)1.4142135623730951;
 }
 
-// Engine transform of a 32-digit shifted point: q = 2u - 1 formed exactly.
-__device__ __forceinline__ float sobol_shifted_normal(uint32_t v, float) {
-  const int m2 = (int)((v >> 8) << 1) + 1 - (1 << 24);
-  return erfinv_f32((float)m2 * 0x1p-24f) * 1.41421356f;
-}
+// Engine transform of a 32-digit shifted point in float64: q = 2u - 1
+// formed exactly from u = (v + 1/2) 2^-32.  (float32 uses the LDS table of
+// normals.h.)
 __device__ __forceinline__ double sobol_shifted_normal(uint32_t v, double) {
   const double q = ((double)v + 0.5) * 0x1p-31 - 1.0;
   return erfinv(q) * 1.4142135623730951;

@@ -39,7 +39,8 @@ struct EulerArgs {
   R p[8];
   const R* lv_vol;
   const R* lv_t;
-  const R* coef;  // HESTON_QE per-step constants [n_steps][12]
+  const R* coef;  // per-step constants: HESTON_QE [n_steps][12], HESTON_FT
+                  // [n_steps][4]
   R lv_x0, lv_inv_dx;
   int lv_nt, lv_nx, lv_in_lds;
   // grid
@@ -57,7 +58,8 @@ struct EulerArgs {
   int64_t n_paths;
   const R* x0;
   int64_t x0_stride;
-  int lds_pay_off;  // byte offset of the payoff area in dynamic LDS
+  int lds_pay_off;   // byte offset of the payoff area in dynamic LDS
+  int lds_spot_off;  // byte offset of the per-wave spot rows from it
 };
 
 // Kernel-internal model code: HESTON_FT with the variant flag p[5] != 0.
@@ -111,14 +113,18 @@ struct Stepper {
       //   143-173): raw v in the drift, sqrt(|v|) in the vol.
       // x' = (x + dt a(x)) + b(x) (z sqrt_dt), with sqrt_dt folded into the
       // volatility factor and each "+ dt a" / "+ b z" a single FMA.
+      // With the per-step table {-dt/2, kappa theta dt, -kappa dt, r t'}
+      // the log-spot state carries x - r t (the deterministic drift is
+      // added back where the state is recorded: spot_offset below).
       const R vp = MODEL == TQF_MODEL_HESTON_FT ? pos_part(x[1]) : x[1];
       const R sq = fast_sqrt(MODEL == TQF_MODEL_HESTON_FT ? vp : fabs(x[1]));
       const R ssq = sq * sdt;
-      const R a0 = fma_r(vp, (R)-0.5, a.p[0]);
-      const R a1 = fma_r(vp, -a.p[1], a.p[1] * a.p[2]);
+      const R* c = a.coef + (int64_t)i * 4;
+      x[0] = fma_r(vp, ld_uniform(c, 0), x[0]);
+      x[1] = fma_r(vp, ld_uniform(c, 2), x[1] + ld_uniform(c, 1));
       const R c1 = fma_r(a.p[4], z[1], a.p[3] * z[0]);
-      x[0] = fma_r(ssq, z[0], fma_r(dt, a0, x[0]));
-      x[1] = fma_r(ssq, c1, fma_r(dt, a1, x[1]));
+      x[0] = fma_r(ssq, z[0], x[0]);
+      x[1] = fma_r(ssq, c1, x[1]);
     } else if (MODEL == TQF_MODEL_HESTON_QE) {
       // Andersen QE (heston_model.py:522-572), psi_c = 1.5, gamma = 1/2.
       R c[10];
@@ -153,17 +159,25 @@ struct Stepper {
   }
 };
 
-// Payoff of one recorded slot for the calling wave (see file comment).
+// Payoff of one recorded slot for the calling wave (see file comment).  The
+// wave's 64 spots go through a per-wave LDS row and come back as broadcast
+// reads (every lane the same address), so a (path, strike) pair costs the
+// four arithmetic instructions and no lane read.
 template <typename R, int MODEL>
 __device__ void wave_payoff(const EulerArgs<R>& a, R* lds_pay, int slot,
                             R spot, int n_active) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  R* row = (R*)((unsigned char*)lds_pay + a.lds_spot_off) + wave * kWave;
+  row[lane] = spot;
+  // one wave writes and reads its own row: LDS keeps a wave's accesses in
+  // order, the barrier only stops the compiler from moving them
+  __builtin_amdgcn_wave_barrier();
   for (int c = 0; c < a.n_strikes; c += kWave) {
     const int ks = c + lane;
     const R k = ks < a.n_strikes ? (R)a.strikes[ks] : (R)0;
     R s1 = 0, s2 = 0;
     for (int j = 0; j < n_active; ++j) {
-      const R sj = read_lane(spot, j);
+      const R sj = row[j];
       R pay = sj - k;
       pay = pay > (R)0 ? pay : (R)0;
       s1 += pay;
@@ -175,25 +189,40 @@ __device__ void wave_payoff(const EulerArgs<R>& a, R* lds_pay, int slot,
       o[1] = s2;
     }
   }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Deterministic part of component 0 the state leaves out after step i
+// (Heston FT with its per-step table: r t_{i+1}); 0 otherwise.
+template <typename R, int MODEL>
+__device__ __forceinline__ R spot_offset(const EulerArgs<R>& a, int i) {
+  if ((MODEL == TQF_MODEL_HESTON_FT || MODEL == kModelHestonOwn) && i >= 0)
+    return ld_uniform(a.coef, (int64_t)i * 4 + 3);
+  return (R)0;
 }
 
 template <typename R, int MODEL>
 __device__ __forceinline__ void record(const EulerArgs<R>& a, R* lds_pay,
-                                       int slot, const R* x, int64_t path,
-                                       bool active, int n_active) {
+                                       int slot, const R* x, R off,
+                                       int64_t path, bool active,
+                                       int n_active) {
   constexpr int D = ModelDim<MODEL>::D;
+  R y[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) y[c] = x[c];
+  y[0] += off;
   if (a.mode == TQF_OUT_PAYOFF) {
-    const R spot = MODEL == TQF_MODEL_GBM ? x[0] : fast_exp(x[0]);
+    const R spot = MODEL == TQF_MODEL_GBM ? y[0] : fast_exp(y[0]);
     wave_payoff<R, MODEL>(a, lds_pay, slot, spot, n_active);
   } else if (active) {
     if (a.mode == TQF_OUT_PATH_MAJOR) {
 #pragma unroll
       for (int c = 0; c < D; ++c)
-        a.paths[(path * a.n_slots + slot) * D + c] = x[c];
+        a.paths[(path * a.n_slots + slot) * D + c] = y[c];
     } else {
 #pragma unroll
       for (int c = 0; c < D; ++c)
-        a.paths[((int64_t)slot * D + c) * a.n_paths + path] = x[c];
+        a.paths[((int64_t)slot * D + c) * a.n_paths + path] = y[c];
     }
   }
 }
@@ -227,7 +256,8 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   Normals<R, RNG> nrm;
   nrm.init(a.rng, pc, D, lds, a.n_steps);
 
-  if (a.init_slot >= 0) record<R, MODEL>(a, lds_pay, a.init_slot, x, path, active, n_active);
+  if (a.init_slot >= 0)
+    record<R, MODEL>(a, lds_pay, a.init_slot, x, (R)0, path, active, n_active);
 
   int lv_row = 0;
   const int n_dims = a.n_steps * D;
@@ -253,7 +283,9 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
     }
     Stepper<R, MODEL>::step(a, x, z, dt, sdt, tab, lv_row, lv_w, i);
     const int s = ld_uniform(a.slot, i);
-    if (s >= 0) record<R, MODEL>(a, lds_pay, s, x, path, active, n_active);
+    if (s >= 0)
+      record<R, MODEL>(a, lds_pay, s, x, spot_offset<R, MODEL>(a, i), path,
+                       active, n_active);
   }
 
   if (a.mode == TQF_OUT_PAYOFF) {
@@ -390,16 +422,25 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   a.x0 = (const R*)x0;
   a.x0_stride = x0_stride;
 
+  // dynamic LDS: Sobol high-bit words | local-vol table | payoff sums and
+  // spot rows; the float32 inverse-CDF table is static LDS (kIcdfLdsBytes)
+  size_t pay = 0;
+  if (a.mode == TQF_OUT_PAYOFF) {
+    const size_t sums = (size_t)kWaves * g->n_slots * out->n_strikes * 2 * sizeof(R);
+    a.lds_spot_off = (int)((sums + 15) & ~(size_t)15);
+    pay = (size_t)a.lds_spot_off + (size_t)kWaves * kWave * sizeof(R);
+  }
   size_t lds = (size_t)rng_lds_bytes(r->kind, D);
   if (m->kind == TQF_MODEL_LOCAL_VOL) {
-    const int64_t ne = (int64_t)m->lv_nt * m->lv_nx;
-    a.lv_in_lds = ne * (int64_t)sizeof(R) <= kMaxLvLdsBytes;
-    if (a.lv_in_lds) lds += (size_t)ne * sizeof(R);
+    const int64_t lv = (int64_t)m->lv_nt * m->lv_nx * (int64_t)sizeof(R);
+    const int64_t total = (int64_t)((lds + lv + 15) & ~(size_t)15) +
+                          (int64_t)pay + kIcdfLdsBytes;
+    a.lv_in_lds = lv <= kMaxLvLdsBytes && total <= 64 * 1024;
+    if (a.lv_in_lds) lds += (size_t)lv;
   }
   lds = (lds + 15) & ~(size_t)15;
   a.lds_pay_off = (int)lds;
-  if (a.mode == TQF_OUT_PAYOFF)
-    lds += (size_t)kWaves * g->n_slots * out->n_strikes * 2 * sizeof(R);
+  lds += pay;
   if (lds == 0) lds = 16;
 
   const unsigned n_blocks = (unsigned)((n_paths + kBlock - 1) / kBlock);
@@ -457,9 +498,9 @@ extern "C" int tqf_euler(const tqf_model* m, const tqf_grid* g,
               "unknown model kind %d", m->kind);
   const int D =
       (m->kind == TQF_MODEL_HESTON_FT || m->kind == TQF_MODEL_HESTON_QE) ? 2 : 1;
-  if (m->kind == TQF_MODEL_HESTON_QE) {
+  if (m->kind == TQF_MODEL_HESTON_QE || m->kind == TQF_MODEL_HESTON_FT) {
     TQF_REQUIRE(m->step_coef != nullptr || g->n_steps == 0,
-                "Heston QE needs per-step constants");
+                "Heston models need their per-step constants (step_coef)");
   }
   TQF_REQUIRE(m->dim == D, "model dim %d does not match kind (%d)", m->dim, D);
   TQF_REQUIRE(g->n_steps >= 0 && g->n_slots >= 1, "bad grid sizes");

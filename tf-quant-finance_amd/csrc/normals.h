@@ -61,18 +61,20 @@ constexpr int sobol_lds_words(int D) { return 2 * kSobolChunk * D; }
 
 
 // ---- float32 shifted-Sobol inverse CDF (piecewise cubic, table in LDS) ----
-// z = Phi^-1((m + 1/2) 2^-24) of the top 24 bits m of a 32-bit point.  The
-// point folds onto the lower half (t = m, or 2^24 - 1 - m when the top bit
-// is set, which also selects the negated copy of the table); the float32
-// value of t' * 128 with t' = 2t + 1 (exact) indexes a segment by its
-// exponent and top 3 mantissa bits, and its low 20 mantissa bits minus 2^20
-// are the cubic's variable (csrc/fit_icdf.py derives the table; exhaustive
-// float32 error: 4.8e-7 absolute, 3.2e-7 relative for |z| >= 1e-3).
+// z = Phi^-1((n + 1/2) 2^-25) of the top 25 bits n of a 32-bit point.  The
+// point folds onto the lower half with its sign mask s (m = (v ^ s) >> 7 in
+// [0, 2^24); the upper half, s all ones, reads the negated copy of the
+// table); the float32 value of m * 128 (exact, formed by masking the low 7
+// bits) indexes a segment by its exponent and top 3 mantissa bits, and its
+// low 20 mantissa bits minus 2^20 are the cubic's variable (csrc/
+// fit_icdf.py derives the table; exhaustive float32 error: 5.2e-7
+// absolute, 3.3e-7 relative for |z| >= 1e-3).
 //
-// LDS image: segment s sits at byte ((s + 1072) & 255) * 16, which is bits
+// LDS image: segment k sits at byte ((k + 1072) & 255) * 16, which is bits
 // 20..27 of the float's bit pattern times 16 (one AND of its upper half):
-// segments 0..191 land on bytes [768, 3840); the negated copy is 4096 bytes
-// above (bit 12, set from the sign mask with one AND-OR).
+// segments 0..191 land on bytes [768, 3840), and m = 0 (the float 0) on
+// byte 0, which holds the constant kIcdfZero.  The negated copy is 4096
+// bytes above (bit 12, set from the sign mask with one AND-OR).
 constexpr int kIcdfHalf = 4096;
 constexpr int kIcdfSlot0 = (134 << kIcdfSegBits) & 255;  // 48
 constexpr int kIcdfLdsBytes = 2 * kIcdfHalf;
@@ -95,15 +97,19 @@ __device__ __forceinline__ void icdf_stage(unsigned char* lds) {
     t[kIcdfSlot0 + e] = c;
     t[kIcdfHalf / 16 + kIcdfSlot0 + e] = make_float4(-c.x, -c.y, -c.z, -c.w);
   }
+  if (threadIdx.x == 0) {
+    t[0] = make_float4(kIcdfZero, 0.f, 0.f, 0.f);
+    t[kIcdfHalf / 16] = make_float4(-kIcdfZero, 0.f, 0.f, 0.f);
+  }
 }
 
 __device__ __forceinline__ float icdf_shifted_f32(uint32_t v,
                                                   const unsigned char* lds) {
   const uint32_t s = (uint32_t)((int32_t)v >> 31);
-  const uint32_t w = ((v ^ s) & 0xFFFFFF80u) | 0x80u;  // t' * 128
-  const uint32_t fi = __float_as_uint((float)w);       // exact
-  // t' * 128 lies in [2^7, 2^31): exponent 134..157, so bits 20..27 of fi
-  // are 48..239 (no wrap) and the offset stays inside its half
+  const uint32_t w = (v ^ s) & 0xFFFFFF80u;       // m * 128
+  const uint32_t fi = __float_as_uint((float)w);  // exact
+  // m * 128 is 0 or in [2^7, 2^31): exponent 134..157, so bits 20..27 of
+  // fi are 48..239 (no wrap), or 0 for m = 0; the offset stays in its half
   uint32_t off;  // (fi >> 16) & 0xFF0: one AND reading the upper half-word
   asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD "
       "src0_sel:WORD_1 src1_sel:DWORD"

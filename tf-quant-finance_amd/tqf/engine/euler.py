@@ -117,7 +117,7 @@ _RNG_CODES = {'draws': _native.RNG_DRAWS, 'philox': _native.RNG_PHILOX,
               'sobol_shifted': _native.RNG_SOBOL_SHIFTED}
 
 
-def _model_struct(spec, dtype, dev, keep):
+def _model_struct(spec, dtype, dev, keep, step_times=None):
   m = _native.Model()
   d = model_dim(spec)
   m.dim = d
@@ -131,6 +131,19 @@ def _model_struct(spec, dtype, dev, keep):
     m.kind = _native.MODEL_HESTON_FT
     for i in range(6):
       m.p[i] = float(spec[1 + i])
+    if step_times is not None:
+      # per-step constants {-dt/2, kappa theta dt, -kappa dt, r t_{i+1}}
+      r, kappa, theta = float(spec[1]), float(spec[2]), float(spec[3])
+      npd = utils.np_dtype(dtype)
+      t = np.asarray(step_times, np.float64)
+      dt = (t[1:] - t[:-1]).astype(npd).astype(np.float64)
+      coef = np.stack([-0.5 * dt, kappa * theta * dt, -kappa * dt,
+                       r * t[1:]], -1)
+      if coef.shape[0] == 0:
+        coef = np.zeros((1, 4))
+      ct = torch.as_tensor(np.ascontiguousarray(coef), device=dev).to(dtype)
+      keep.append(ct)
+      m.step_coef = ct.data_ptr()
   elif spec[0] == 'heston_qe':
     m.kind = _native.MODEL_HESTON_QE
     coef = torch.as_tensor(np.ascontiguousarray(spec[1]), dtype=dtype,
@@ -253,7 +266,8 @@ def prepare(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
   g.init_slot = init_slot
   g.n_rng_steps = max(n_rng_steps, executed)
 
-  m = _model_struct(spec, dtype, dev, keep)
+  m = _model_struct(spec, dtype, dev, keep,
+                    step_times=all_times[:max(executed, 1) + 1])
 
   r = _native.Rng()
   r.kind = _RNG_CODES[rng.kind]
