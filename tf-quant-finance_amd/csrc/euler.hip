@@ -165,10 +165,11 @@ struct Stepper {
 // four arithmetic instructions and no lane read.
 template <typename R, int MODEL>
 __device__ void wave_payoff(const EulerArgs<R>& a, R* lds_pay, int slot,
-                            R spot, int n_active) {
+                            R spot, bool active) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   R* row = (R*)((unsigned char*)lds_pay + a.lds_spot_off) + wave * kWave;
-  row[lane] = spot;
+  // lanes past the last path hold -inf: their payoff max(-inf - K, 0) is 0
+  row[lane] = active ? spot : -(R)INFINITY;
   // one wave writes and reads its own row: LDS keeps a wave's accesses in
   // order, the barrier only stops the compiler from moving them
   __builtin_amdgcn_wave_barrier();
@@ -176,7 +177,8 @@ __device__ void wave_payoff(const EulerArgs<R>& a, R* lds_pay, int slot,
     const int ks = c + lane;
     const R k = ks < a.n_strikes ? (R)a.strikes[ks] : (R)0;
     R s1 = 0, s2 = 0;
-    for (int j = 0; j < n_active; ++j) {
+#pragma unroll 16
+    for (int j = 0; j < kWave; ++j) {
       const R sj = row[j];
       R pay = sj - k;
       pay = pay > (R)0 ? pay : (R)0;
@@ -204,8 +206,7 @@ __device__ __forceinline__ R spot_offset(const EulerArgs<R>& a, int i) {
 template <typename R, int MODEL>
 __device__ __forceinline__ void record(const EulerArgs<R>& a, R* lds_pay,
                                        int slot, const R* x, R off,
-                                       int64_t path, bool active,
-                                       int n_active) {
+                                       int64_t path, bool active) {
   constexpr int D = ModelDim<MODEL>::D;
   R y[D];
 #pragma unroll
@@ -213,7 +214,7 @@ __device__ __forceinline__ void record(const EulerArgs<R>& a, R* lds_pay,
   y[0] += off;
   if (a.mode == TQF_OUT_PAYOFF) {
     const R spot = MODEL == TQF_MODEL_GBM ? y[0] : fast_exp(y[0]);
-    wave_payoff<R, MODEL>(a, lds_pay, slot, spot, n_active);
+    wave_payoff<R, MODEL>(a, lds_pay, slot, spot, active);
   } else if (active) {
     if (a.mode == TQF_OUT_PATH_MAJOR) {
 #pragma unroll
@@ -236,9 +237,6 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   const int64_t path = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const bool active = path < a.n_paths;
   const int64_t pc = active ? path : a.n_paths - 1;
-  const int64_t wave_base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63);
-  const int64_t rem = a.n_paths - wave_base;
-  const int n_active = rem >= kWave ? kWave : (rem > 0 ? (int)rem : 0);
 
   // Local-vol table: stage in LDS when small.
   const R* tab = a.lv_vol;
@@ -257,7 +255,7 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   nrm.init(a.rng, pc, D, lds, a.n_steps);
 
   if (a.init_slot >= 0)
-    record<R, MODEL>(a, lds_pay, a.init_slot, x, (R)0, path, active, n_active);
+    record<R, MODEL>(a, lds_pay, a.init_slot, x, (R)0, path, active);
 
   int lv_row = 0;
   const int n_dims = a.n_steps * D;
@@ -285,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
     const int s = ld_uniform(a.slot, i);
     if (s >= 0)
       record<R, MODEL>(a, lds_pay, s, x, spot_offset<R, MODEL>(a, i), path,
-                       active, n_active);
+                       active);
   }
 
   if (a.mode == TQF_OUT_PAYOFF) {

@@ -200,28 +200,34 @@ struct Normals {
       const uint64_t hi0 =
           (uint64_t)(a.sfirst + (int64_t)blockIdx.x * kPathBlock) >> 8;
       const int n = kSobolChunk * D;
-      for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) {
-        const int c = t / n;
-        const int j = i * D + (t - c * n);
-        uint32_t x = 0;
-        if (j < n_dims) {
-          // columns 8..31 of dimension j in six 16-byte loads issued
-          // together, then masked by the index bits 8..31
-          const uint32_t hb = (uint32_t)((hi0 + c) & 0xffffffull);
-          const uint4* col = (const uint4*)(a.sv + (int64_t)j * 32 + 8);
-          uint4 cv[6];
+      // row c holds index bits 8..31 of hi0 + c; the second row is needed
+      // only when the block's 256 paths straddle a multiple of 256
+      const int rows =
+          ((a.sfirst + (int64_t)blockIdx.x * kPathBlock) & 255) != 0 ? 2 : 1;
+      for (int c = 0; c < rows; ++c) {
+        // uniform per pass: the column masks live in scalar registers
+        const uint32_t hb = (uint32_t)((hi0 + c) & 0xffffffull);
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+          const int j = i * D + t;
+          uint32_t x = 0;
+          if (j < n_dims) {
+            // columns 8..31 of dimension j in six 16-byte loads issued
+            // together, then masked by the index bits 8..31
+            const uint4* col = (const uint4*)(a.sv + (int64_t)j * 32 + 8);
+            uint4 cv[6];
 #pragma unroll
-          for (int q = 0; q < 6; ++q) cv[q] = col[q];
-          x = a.sshift ? a.sshift[j] : 0u;
+            for (int q = 0; q < 6; ++q) cv[q] = col[q];
+            x = a.sshift ? a.sshift[j] : 0u;
 #pragma unroll
-          for (int q = 0; q < 6; ++q) {
-            x ^= cv[q].x & (0u - ((hb >> (4 * q)) & 1u));
-            x ^= cv[q].y & (0u - ((hb >> (4 * q + 1)) & 1u));
-            x ^= cv[q].z & (0u - ((hb >> (4 * q + 2)) & 1u));
-            x ^= cv[q].w & (0u - ((hb >> (4 * q + 3)) & 1u));
+            for (int q = 0; q < 6; ++q) {
+              x ^= cv[q].x & (0u - ((hb >> (4 * q)) & 1u));
+              x ^= cv[q].y & (0u - ((hb >> (4 * q + 1)) & 1u));
+              x ^= cv[q].z & (0u - ((hb >> (4 * q + 2)) & 1u));
+              x ^= cv[q].w & (0u - ((hb >> (4 * q + 3)) & 1u));
+            }
           }
+          hx[c * n + t] = x;
         }
-        hx[t] = x;
       }
       __syncthreads();
     }
