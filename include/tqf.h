@@ -133,10 +133,13 @@ typedef struct tqf_model {
                              (s^2 = v c1 + c2), 3..7 k0..k4 of the log-spot
                              update, 8 theta, 9 update flag (0 when
                              dt <= tolerance), 10-11 unused.
-                             HESTON_FT: device [n_steps][4] = {-dt/2,
-                             kappa theta dt, -kappa dt, r t_{i+1}} (r =
-                             p[0]); the kernel carries log S - r t and adds
-                             r t back where it records or prices. */
+                             HESTON_FT: device [n_steps][8] = {-dt/2,
+                             kappa theta dt, -kappa dt, r t_{i+1},
+                             sqrt(dt), slot[i] (the grid's record slot,
+                             stored as the integer's bit pattern), 0, 0}
+                             with r = p[0]; the kernel carries log S - r t
+                             and adds r t back where it records or
+                             prices. */
 } tqf_model;
 
 typedef struct tqf_grid {

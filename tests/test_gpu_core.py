@@ -383,6 +383,80 @@ def test_euler_shifted_sobol_parity():
   _close(_np(got), want, 5e-5)
 
 
+# ---------------------------------------------------------- local vol
+
+def _lv_surface(nt=4, nx=9):
+  times = np.linspace(0.0, 1.2, nt)
+  x = np.linspace(np.log(40.), np.log(250.), nx)
+  vols = (0.25 - 0.06 * (x[None, :] - np.log(100.)) +
+          0.04 * np.sin(3 * x[None, :]) + 0.05 * times[:, None])
+  return times, x, vols
+
+
+@pytest.mark.parametrize('nx', [9, 900])  # LDS-staged and global table
+@pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12),
+                                       (torch.float32, 2e-5)])
+def test_local_vol_parity(dtype, tol, nx):
+  from oracle import local_vol as olv
+  npd = np.float64 if dtype == torch.float64 else np.float32
+  times, x, vols = _lv_surface(7 if nx > 100 else 4, nx)
+  surf = tqf.models.LocalVolSurface(times, x[0], x[1] - x[0], vols)
+  model = tqf.models.LocalVolatilityModel(1, risk_free_rate=0.03,
+                                          dividend_yield=0.01,
+                                          local_volatility_fn=surf,
+                                          dtype=dtype)
+  rs = np.random.RandomState(21)
+  n, steps = 3000, 40
+  z = rs.standard_normal((n, steps, 1)).astype(npd)
+  got = _np(model.sample_paths([0.5, 1.0], initial_state=[100.0],
+                               num_time_steps=steps, normal_draws=z))
+  fns = olv.lv_fns(times, x[0], x[1] - x[0], vols, 0.03, 0.01)
+  want = oeuler.sample(1, fns[0], fns[1], np.asarray([0.5, 1.0], npd),
+                       num_samples=n,
+                       initial_state=np.asarray([np.log(100.)], npd),
+                       normal_draws=z, num_time_steps=steps, dtype=npd)
+  _close(got, np.exp(want), tol)
+  # the same surface as an arbitrary callable takes the generic loop
+  generic = tqf.models.LocalVolatilityModel(
+      1, risk_free_rate=0.03, dividend_yield=0.01,
+      local_volatility_fn=lambda t, s: surf(t, s), dtype=dtype)
+  got2 = _np(generic.sample_paths([0.5, 1.0], initial_state=[100.0],
+                                  num_time_steps=steps, normal_draws=z))
+  _close(got2, got, 10 * tol)
+
+
+def test_local_vol_flat_surface_is_black_scholes():
+  times = np.array([0.0, 1.0])
+  x = np.linspace(np.log(20.), np.log(500.), 5)
+  surf = tqf.models.LocalVolSurface(times, x[0], x[1] - x[0],
+                                    np.full((2, 5), 0.25))
+  model = tqf.models.LocalVolatilityModel(1, risk_free_rate=0.04,
+                                          local_volatility_fn=surf,
+                                          dtype=torch.float64)
+  s = _np(model.sample_paths([1.0], initial_state=[100.0], num_time_steps=20,
+                             num_samples=200000,
+                             random_type=mvn.RandomType.PSEUDO, seed=5))
+  st = s[:, 0, 0]
+  for k in (80.0, 100.0, 120.0):
+    pay = np.exp(-0.04) * np.maximum(st - k, 0)
+    bs = closed_form.bs_price(100.0, k, 1.0, 0.04, 0.25)
+    assert abs(pay.mean() - bs) <= 3 * pay.std() / np.sqrt(st.size), k
+
+
+def test_local_vol_precompute_iv_tabulates_and_rejects_grids():
+  surf_fn = lambda t, s: 0.2 + 0.0 * s
+  model = tqf.models.LocalVolatilityModel(
+      1, risk_free_rate=0.02, local_volatility_fn=surf_fn,
+      times_grid=np.linspace(0.0, 1.0, 11), spot_grid=[50.0, 200.0],
+      precompute_iv=True, dtype=torch.float64)
+  assert isinstance(model.local_volatility_fn(), tqf.models.LocalVolSurface)
+  with pytest.raises(ValueError):
+    model.sample_paths([1.0], initial_state=[100.0], time_step=0.1)
+  p = _np(model.sample_paths([1.0], initial_state=[100.0], num_samples=1000,
+                             random_type=mvn.RandomType.PSEUDO, seed=1))
+  assert p.shape == (1000, 1, 1) and np.all(p > 0)
+
+
 # ---------------------------------------------------------------- Heston QE
 
 @pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12),

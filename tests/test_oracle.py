@@ -364,3 +364,20 @@ def test_mlp_oracle_gradient_matches_finite_differences():
     fd = (mlp.loss_and_grad(p + e, x, t, w, 1 / 40)[0] -
           mlp.loss_and_grad(p - e, x, t, w, 1 / 40)[0]) / (2 * h)
     np.testing.assert_allclose(g[i], fd, rtol=1e-5, atol=1e-9)
+
+
+def test_local_vol_table_is_exact_on_bilinear_surfaces():
+  from oracle import local_vol
+  times = np.array([0.0, 0.5, 1.5])
+  x = np.linspace(np.log(50.), np.log(200.), 7)
+  a, b, c = 0.2, -0.03, 0.05
+  vols = a + b * (x[None, :] - x[0]) + c * times[:, None]
+  for t in (0.0, 0.2, 0.5, 1.0, 1.5):
+    xs = np.linspace(x[0], x[-1], 11)
+    got = local_vol.table_sigma(times, x[0], x[1] - x[0], vols, t, xs)
+    np.testing.assert_allclose(got, a + b * (xs - x[0]) + c * t, rtol=1e-13)
+  # flat outside the grid
+  got = local_vol.table_sigma(times, x[0], x[1] - x[0], vols, 3.0,
+                              np.array([x[0] - 1, x[-1] + 1]))
+  np.testing.assert_allclose(got, [vols[-1, 0], vols[-1, -1]], rtol=1e-13)
+

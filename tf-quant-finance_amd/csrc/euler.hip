@@ -40,7 +40,7 @@ struct EulerArgs {
   const R* lv_vol;
   const R* lv_t;
   const R* coef;  // per-step constants: HESTON_QE [n_steps][12], HESTON_FT
-                  // [n_steps][4]
+                  // [n_steps][8] (StepRow)
   R lv_x0, lv_inv_dx;
   int lv_nt, lv_nx, lv_in_lds;
   // grid
@@ -93,13 +93,39 @@ __device__ __forceinline__ R lv_sigma(const EulerArgs<R>& a, const R* tab,
 }
 
 // ---- per-step update -------------------------------------------------------
+// Heston FT per-step row (the model's step_coef): {-dt/2, kappa theta dt,
+// -kappa dt, r t_{i+1}, sqrt(dt), record slot (int bits), 0, 0}; one scalar
+// load per step brings everything the step needs.
+template <typename R>
+struct StepRow {
+  R c[8];
+};
+template <int MODEL>
+constexpr bool kHasStepRow =
+    MODEL == TQF_MODEL_HESTON_FT || MODEL == kModelHestonOwn;
+
+// Scalar load of row i (the compiler merges the eight into one s_load).
+template <typename R>
+__device__ __forceinline__ StepRow<R> ld_row(const R* coef, int64_t i) {
+  StepRow<R> r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.c[k] = ld_uniform(coef, i * 8 + k);
+  return r;
+}
+
+__device__ __forceinline__ int row_slot(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int row_slot(double v) {
+  return (int)__double_as_longlong(v);
+}
+
 template <typename R, int MODEL>
 struct Stepper {
   static constexpr int D = ModelDim<MODEL>::D;
   __device__ __forceinline__ static void step(const EulerArgs<R>& a, R* x,
                                               const R* z, R dt, R sdt,
                                               const R* tab, int lv_row,
-                                              R lv_w, int i) {
+                                              R lv_w, int i,
+                                              const StepRow<R>& row) {
     if (MODEL == TQF_MODEL_GBM) {
       const R drift = a.p[0] * x[0];
       const R vol = a.p[1] * x[0];
@@ -113,15 +139,14 @@ struct Stepper {
       //   143-173): raw v in the drift, sqrt(|v|) in the vol.
       // x' = (x + dt a(x)) + b(x) (z sqrt_dt), with sqrt_dt folded into the
       // volatility factor and each "+ dt a" / "+ b z" a single FMA.
-      // With the per-step table {-dt/2, kappa theta dt, -kappa dt, r t'}
+      // With the per-step row {-dt/2, kappa theta dt, -kappa dt, r t', ...}
       // the log-spot state carries x - r t (the deterministic drift is
-      // added back where the state is recorded: spot_offset below).
+      // added back where the state is recorded).
       const R vp = MODEL == TQF_MODEL_HESTON_FT ? pos_part(x[1]) : x[1];
       const R sq = fast_sqrt(MODEL == TQF_MODEL_HESTON_FT ? vp : fabs(x[1]));
       const R ssq = sq * sdt;
-      const R* c = a.coef + (int64_t)i * 4;
-      x[0] = fma_r(vp, ld_uniform(c, 0), x[0]);
-      x[1] = fma_r(vp, ld_uniform(c, 2), x[1] + ld_uniform(c, 1));
+      x[0] = fma_r(vp, row.c[0], x[0]);
+      x[1] = fma_r(vp, row.c[2], x[1] + row.c[1]);
       const R c1 = fma_r(a.p[4], z[1], a.p[3] * z[0]);
       x[0] = fma_r(ssq, z[0], x[0]);
       x[1] = fma_r(ssq, c1, x[1]);
@@ -194,15 +219,6 @@ __device__ void wave_payoff(const EulerArgs<R>& a, R* lds_pay, int slot,
   __builtin_amdgcn_wave_barrier();
 }
 
-// Deterministic part of component 0 the state leaves out after step i
-// (Heston FT with its per-step table: r t_{i+1}); 0 otherwise.
-template <typename R, int MODEL>
-__device__ __forceinline__ R spot_offset(const EulerArgs<R>& a, int i) {
-  if ((MODEL == TQF_MODEL_HESTON_FT || MODEL == kModelHestonOwn) && i >= 0)
-    return ld_uniform(a.coef, (int64_t)i * 4 + 3);
-  return (R)0;
-}
-
 template <typename R, int MODEL>
 __device__ __forceinline__ void record(const EulerArgs<R>& a, R* lds_pay,
                                        int slot, const R* x, R off,
@@ -259,31 +275,61 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
 
   int lv_row = 0;
   const int n_dims = a.n_steps * D;
-  for (int i = 0; i < a.n_steps; ++i) {
-    nrm.begin_step(a.rng, i, n_dims);
-    const R dt = ld_uniform(a.dt, i);
-    const R sdt = ld_uniform(a.sqrt_dt, i);
-    R z[D];
+  auto fetch = [&](int i, R* z) {
     if (D == 2) {
       nrm.get2(a.rng, i, z);
     } else {
 #pragma unroll
       for (int k = 0; k < D; ++k) z[k] = nrm.get(a.rng, i, k);
     }
-    R lv_w = 0;
-    if (MODEL == TQF_MODEL_LOCAL_VOL) {
-      const R t = ld_uniform(a.t_next, i);
-      while (lv_row + 1 < a.lv_nt && a.lv_t[lv_row + 1] <= t) ++lv_row;
-      if (lv_row + 1 < a.lv_nt) {
-        const R t0 = a.lv_t[lv_row], t1 = a.lv_t[lv_row + 1];
-        lv_w = t <= t0 ? (R)0 : (t - t0) / (t1 - t0);
+  };
+  // Chunks of kSobolChunk steps (the Sobol high-bit words are rebuilt per
+  // chunk).  Inside a chunk the normals of step i+1 are produced during
+  // step i -- they do not depend on the state -- with no branch between
+  // them and the update, so their LDS and memory latency overlaps it.
+  for (int c0 = 0; c0 < a.n_steps; c0 += kSobolChunk) {
+    nrm.refill(a.rng, c0, n_dims);
+    const int c1 = c0 + kSobolChunk < a.n_steps ? c0 + kSobolChunk : a.n_steps;
+    R z_next[D];
+    fetch(c0, z_next);
+    StepRow<R> row_next;
+    if constexpr (kHasStepRow<MODEL>)
+      row_next = ld_row(a.coef, c0);
+    for (int i = c0; i < c1; ++i) {
+      R z[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) z[k] = z_next[k];
+      fetch(i + 1 < c1 ? i + 1 : i, z_next);  // the last one is redundant
+      // per-step scalars: the Heston FT row is loaded one step ahead, so the
+      // update never waits on a scalar load issued behind the LDS reads
+      StepRow<R> row;
+      R dt, sdt;
+      int s;
+      if constexpr (kHasStepRow<MODEL>) {
+        row = row_next;
+        row_next = ld_row(a.coef, i + 1 < a.n_steps ? i + 1 : i);
+        dt = (R)0;
+        sdt = row.c[4];
+        s = row_slot(row.c[5]);
+      } else {
+        dt = ld_uniform(a.dt, i);
+        sdt = ld_uniform(a.sqrt_dt, i);
+        s = ld_uniform(a.slot, i);
       }
+      R lv_w = 0;
+      if (MODEL == TQF_MODEL_LOCAL_VOL) {
+        const R t = ld_uniform(a.t_next, i);
+        while (lv_row + 1 < a.lv_nt && a.lv_t[lv_row + 1] <= t) ++lv_row;
+        if (lv_row + 1 < a.lv_nt) {
+          const R t0 = a.lv_t[lv_row], t1 = a.lv_t[lv_row + 1];
+          lv_w = t <= t0 ? (R)0 : (t - t0) / (t1 - t0);
+        }
+      }
+      Stepper<R, MODEL>::step(a, x, z, dt, sdt, tab, lv_row, lv_w, i, row);
+      if (s >= 0)
+        record<R, MODEL>(a, lds_pay, s, x,
+                         kHasStepRow<MODEL> ? row.c[3] : (R)0, path, active);
     }
-    Stepper<R, MODEL>::step(a, x, z, dt, sdt, tab, lv_row, lv_w, i);
-    const int s = ld_uniform(a.slot, i);
-    if (s >= 0)
-      record<R, MODEL>(a, lds_pay, s, x, spot_offset<R, MODEL>(a, i), path,
-                       active);
   }
 
   if (a.mode == TQF_OUT_PAYOFF) {

@@ -194,7 +194,14 @@ struct Normals {
   // synchronise).  n_dims bounds the Sobol dimensions that exist.
   __device__ __forceinline__ void begin_step(const RngArgs<R>& a, int i,
                                              int n_dims) {
-    if (kSobol && (i % kSobolChunk) == 0) {
+    if (kSobol && (i % kSobolChunk) == 0) refill(a, i, n_dims);
+  }
+
+  // Builds the block's high-bit words for steps [i, i + kSobolChunk) (Sobol
+  // only; every thread of the block calls it, it synchronises).
+  __device__ __forceinline__ void refill(const RngArgs<R>& a, int i,
+                                         int n_dims) {
+    if (kSobol) {
       chunk_base = i;
       __syncthreads();
       const uint64_t hi0 =

@@ -48,7 +48,7 @@ class LocalVolTable:
   """sigma(t, S) on a (t_i, ln S_j) grid with a uniform log-spot axis.
 
   Lookups are bilinear in (t, ln S) with flat extrapolation; the kernel
-  stages the table in LDS when it has at most 8192 entries.
+  stages the table in LDS when it fits (up to 32 KB).
   """
 
   def __init__(self, times, log_spot_min, log_spot_step, vols, drift_rate):
@@ -117,7 +117,7 @@ _RNG_CODES = {'draws': _native.RNG_DRAWS, 'philox': _native.RNG_PHILOX,
               'sobol_shifted': _native.RNG_SOBOL_SHIFTED}
 
 
-def _model_struct(spec, dtype, dev, keep, step_times=None):
+def _model_struct(spec, dtype, dev, keep, step_times=None, step_slots=None):
   m = _native.Model()
   d = model_dim(spec)
   m.dim = d
@@ -132,16 +132,26 @@ def _model_struct(spec, dtype, dev, keep, step_times=None):
     for i in range(6):
       m.p[i] = float(spec[1 + i])
     if step_times is not None:
-      # per-step constants {-dt/2, kappa theta dt, -kappa dt, r t_{i+1}}
+      # per-step rows {-dt/2, kappa theta dt, -kappa dt, r t_{i+1},
+      # sqrt(dt), slot bits, 0, 0}
       r, kappa, theta = float(spec[1]), float(spec[2]), float(spec[3])
       npd = utils.np_dtype(dtype)
       t = np.asarray(step_times, np.float64)
-      dt = (t[1:] - t[:-1]).astype(npd).astype(np.float64)
-      coef = np.stack([-0.5 * dt, kappa * theta * dt, -kappa * dt,
-                       r * t[1:]], -1)
-      if coef.shape[0] == 0:
-        coef = np.zeros((1, 4))
-      ct = torch.as_tensor(np.ascontiguousarray(coef), device=dev).to(dtype)
+      dt = (t[1:] - t[:-1]).astype(npd)
+      sdt = np.sqrt(dt).astype(npd)
+      dt = dt.astype(np.float64)
+      n = dt.shape[0]
+      coef = np.zeros((max(n, 1), 8), npd)
+      coef[:n, 0] = -0.5 * dt
+      coef[:n, 1] = kappa * theta * dt
+      coef[:n, 2] = -kappa * dt
+      coef[:n, 3] = r * t[1:]
+      coef[:n, 4] = sdt
+      ib = np.int32 if npd == np.float32 else np.int64
+      bits = coef[:, 5].view(ib)
+      bits[:] = -1
+      bits[:n] = np.asarray(step_slots[:n], ib)
+      ct = torch.from_numpy(np.ascontiguousarray(coef)).to(dev)
       keep.append(ct)
       m.step_coef = ct.data_ptr()
   elif spec[0] == 'heston_qe':
@@ -267,7 +277,7 @@ def prepare(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
   g.n_rng_steps = max(n_rng_steps, executed)
 
   m = _model_struct(spec, dtype, dev, keep,
-                    step_times=all_times[:max(executed, 1) + 1])
+                    step_times=all_times[:executed + 1], step_slots=slots)
 
   r = _native.Rng()
   r.kind = _RNG_CODES[rng.kind]

@@ -5,6 +5,9 @@ from tqf.models import utils
 from tqf.models.geometric_brownian_motion import GeometricBrownianMotion
 from tqf.models.geometric_brownian_motion import MultivariateGeometricBrownianMotion
 from tqf.models.heston import HestonModel
+from tqf.models.local_volatility import LocalVolatilityModel
+from tqf.models.local_volatility import LocalVolSurface
 
 __all__ = ['euler_sampling', 'longstaff_schwartz', 'utils', 'GeometricBrownianMotion',
-           'MultivariateGeometricBrownianMotion', 'HestonModel']
+           'MultivariateGeometricBrownianMotion', 'HestonModel',
+           'LocalVolatilityModel', 'LocalVolSurface']
