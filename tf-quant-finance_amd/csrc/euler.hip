@@ -290,16 +290,12 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   for (int c0 = 0; c0 < a.n_steps; c0 += kSobolChunk) {
     nrm.refill(a.rng, c0, n_dims);
     const int c1 = c0 + kSobolChunk < a.n_steps ? c0 + kSobolChunk : a.n_steps;
-    R z_next[D];
-    fetch(c0, z_next);
     StepRow<R> row_next;
-    if constexpr (kHasStepRow<MODEL>)
-      row_next = ld_row(a.coef, c0);
-    for (int i = c0; i < c1; ++i) {
-      R z[D];
-#pragma unroll
-      for (int k = 0; k < D; ++k) z[k] = z_next[k];
-      fetch(i + 1 < c1 ? i + 1 : i, z_next);  // the last one is redundant
+    if constexpr (kHasStepRow<MODEL>) row_next = ld_row(a.coef, c0);
+    // one step: consumes zc (its normals, fetched a step earlier) and the
+    // prefetched row, and fetches step i+1's normals into zn
+    auto one_step = [&](int i, const R* zc, R* zn) {
+      fetch(i + 1 < c1 ? i + 1 : i, zn);  // the chunk's last one is redundant
       // per-step scalars: the Heston FT row is loaded one step ahead, so the
       // update never waits on a scalar load issued behind the LDS reads
       StepRow<R> row;
@@ -325,11 +321,21 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
           lv_w = t <= t0 ? (R)0 : (t - t0) / (t1 - t0);
         }
       }
-      Stepper<R, MODEL>::step(a, x, z, dt, sdt, tab, lv_row, lv_w, i, row);
+      Stepper<R, MODEL>::step(a, x, zc, dt, sdt, tab, lv_row, lv_w, i, row);
       if (s >= 0)
         record<R, MODEL>(a, lds_pay, s, x,
                          kHasStepRow<MODEL> ? row.c[3] : (R)0, path, active);
+    };
+    // two steps per iteration with the normals alternating between zA and
+    // zB, so the pipelining needs no register copies
+    R zA[D], zB[D];
+    fetch(c0, zA);
+    int i = c0;
+    for (; i + 1 < c1; i += 2) {
+      one_step(i, zA, zB);
+      one_step(i + 1, zB, zA);
     }
+    if (i < c1) one_step(i, zA, zB);
   }
 
   if (a.mode == TQF_OUT_PAYOFF) {
