@@ -27,7 +27,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = ('Heston path-steps/s on 1 GPU; options+Greeks/s at 1e-3 rel err vs '
           'FP64')
-FLOP_PER_PATH_STEP = 73  # algorithmic count, FP32
+FLOP_PER_PATH_STEP = 73  # algorithmic count per path-step, FP32
+# HBM bytes per launch of the headline kernel at 2^20 paths: FETCH_SIZE and
+# WRITE_SIZE of two rocprofv3 --pmc passes, each scaled by the same counter
+# on a 256 MiB device copy (profiles/euler_heston_ft_sobol_pmc.md).
+TRAFFIC_BYTES_2POW20 = 23.87e6
 PEAK_FP32 = 157.3e12
 PEAK_HBM = 8.0e12
 
@@ -278,8 +282,8 @@ def main():
                        'known-size kernel that scales HBM counters)')
   args = ap.parse_args()
   if args.gpus != 1:
-    print('bench.py measures one GPU: this build runs on a single MI355X and '
-          'has no multi-GPU path (see DESIGN.md, "More than one GPU").',
+    print('bench.py measures one GPU: the engine runs on a single MI355X '
+          'and has no multi-GPU path; run it with --gpus 1.',
           file=sys.stderr)
     sys.exit(2)
 
@@ -375,7 +379,9 @@ def main():
                  'rng': 'sobol (Joe-Kuo, 32-digit shift, seed 7)'},
       'roofline': {'bound': 'valu', 'achieved': achieved / 1e12,
                    'peak': PEAK_FP32 / 1e12, 'unit': 'TFLOP/s',
-                   'frac': achieved / PEAK_FP32, 'traffic': None,
+                   'frac': achieved / PEAK_FP32,
+                   'traffic': (TRAFFIC_BYTES_2POW20 if args.paths == 2**20
+                               else None),
                    'kernel': 'euler_kernel<float, HESTON_FT, SOBOL_SHIFTED>',
                    'kernel_avg_ms': avg * 1e3,
                    'flop_per_path_step': FLOP_PER_PATH_STEP},
