@@ -82,6 +82,25 @@ int tqf_sobol_normals(const uint32_t* v, int32_t dim, int32_t nd, int64_t n,
                       int64_t first_index, const uint32_t* shift,
                       int32_t mode, int32_t dtype, void* out, void* stream);
 
+/* ---- Halton ------------------------------------------------------------ */
+
+/* Device.  Halton points: element i is the radical inverse of
+ * (seq_indices ? seq_indices[i] : first_index + i) + 1 in base bases[d],
+ * truncated to max_digits[d] digits; out[i*out_stride + d].  perms (device
+ * [max(max_digits)][radix_sum], or NULL) permutes digit j of axis d through
+ * perms[j*radix_sum + offsets[d] + digit] and zero_correction (device
+ * [dim] or NULL) is added (Owen's randomisation).  out_kind 1/2: float32/64
+ * uniforms; 3/4: float32/64 normals sqrt(2) erfinv(2u - 1).  bases,
+ * max_digits, offsets: device [dim] int32.  Replaces halton.sample
+ * (tff/math/random_ops/halton/halton_impl.py:59-285) and the Halton branch
+ * of the quasi normal draws (multivariate_normal.py:393-424). */
+int tqf_halton_points(const int32_t* bases, const int32_t* max_digits,
+                      const int32_t* offsets, int32_t dim, int64_t n,
+                      int64_t first_index, const int64_t* seq_indices,
+                      const int32_t* perms, int32_t radix_sum,
+                      const double* zero_correction, int32_t out_kind,
+                      void* out, int64_t out_stride, void* stream);
+
 /* ---- Philox normals ---------------------------------------------------- */
 
 /* Device.  out[p*n_draws + j] = normal j of path p of the engine's Philox
@@ -248,7 +267,8 @@ typedef struct tqf_lsm {
   int32_t degree;           /* make_polynomial_basis(degree),
                                (degree + 1) * dim <= 64                       */
   int32_t n_basis;          /* (degree + 1)^dim, at most 255                 */
-  int32_t pad0;
+  int32_t basis_log;        /* 0: basis on the paths' values; 1: on their
+                               logarithms (the payoff always uses values)    */
   const uint8_t* exponents; /* device [n_basis][dim] exponent table in the
                                reference's meshgrid order (lsm.py:116-118)   */
   double strike;

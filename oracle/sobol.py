@@ -9,6 +9,12 @@ Restates, with Python ints / numpy uint64 (no TF):
   * `digital_net_sample`         tff/math/qmc/digital_net.py:205-419
   * `sobol_sample`               tff/math/qmc/sobol.py:32-129
   * `tent_transform`             tff/math/qmc/utils.py:124-141
+  * `scramble_generating_matrices` tff/math/qmc/digital_net.py:422-527,
+    restated as the GF(2) product L_d g (digit vectors, most significant
+    digit first) with L_d unit lower triangular, column s of L_d being the
+    digits of S[d, s] >> s -- pinned by the reference's no-op case
+    (digital_net_test.py:274-306: every S = 2^(nd-1) gives L = I) and by the
+    net property surviving the scramble (L nonsingular).
 
 Pinned by: the reference golden tables (random_ops/sobol/sobol_test.py:28-98,
 qmc/sobol_test.py:63-193, qmc/digital_net_test.py:112-246) and by
@@ -187,3 +193,26 @@ def gray_code_ints(dim, count, nd=32):
       x[d] ^= v[d][c]
     out[n - 1] = x
   return out
+
+
+def scramble_generating_matrices(generating_matrices, scrambling_matrices,
+                                 num_digits):
+  g = np.asarray(generating_matrices, np.int64)
+  sm = np.asarray(scrambling_matrices, np.int64)
+  nd = int(num_digits)
+  dim, ncol = g.shape
+  out = np.zeros_like(g)
+  for d in range(dim):
+    # L[r, s] = digit r (MSB first) of S[d, s] >> s
+    lmat = np.zeros((nd, nd), np.int64)
+    for c in range(nd):
+      v = int(sm[d, c]) >> c
+      for r in range(nd):
+        lmat[r, c] = (v >> (nd - 1 - r)) & 1
+    for j in range(ncol):
+      digits = np.array([(int(g[d, j]) >> (nd - 1 - r)) & 1
+                         for r in range(nd)])
+      res = lmat.dot(digits) % 2
+      out[d, j] = int(sum(int(b) << (nd - 1 - r) for r, b in enumerate(res)))
+  return out
+

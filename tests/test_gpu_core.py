@@ -476,3 +476,87 @@ def test_heston_qe_parity(dtype, tol):
   _close(got[..., 0], want[..., 0], tol)
   # the variance has exact zeros on the psi > 1.5 branch: absolute scale
   _close(got[..., 1], want[..., 1], tol, scale=1e3 * np.max(want[..., 1]))
+
+
+# --------------------------------------------------------------- Halton
+
+@pytest.mark.parametrize('dtype,rtol', [(torch.float32, 1e-6),
+                                        (torch.float64, 1e-13)])
+def test_halton_matches_oracle(dtype, rtol):
+  from oracle import halton as ohalton
+  from tqf.math.random import halton
+  npd = np.float32 if dtype == torch.float32 else np.float64
+  got, params = halton.sample(2, num_results=5, randomized=False, dtype=dtype)
+  assert params is None
+  np.testing.assert_allclose(
+      _np(got), [[1. / 2, 1. / 3], [1. / 4, 2. / 3], [3. / 4, 1. / 9],
+                 [1. / 8, 4. / 9], [5. / 8, 7. / 9]], rtol=1e-6)
+  for dim, idx in ((17, np.arange(3000)),
+                   (40, np.array([0, 5, 2**20 + 3, 12345678]))):
+    got, _ = halton.sample(dim, sequence_indices=idx, randomized=False,
+                           dtype=dtype)
+    want = ohalton.sample(dim, sequence_indices=idx, dtype=npd)
+    np.testing.assert_allclose(_np(got), want, rtol=rtol, atol=1e-30)
+  # randomized: the oracle given the same permutations and correction
+  got, params = halton.sample(7, num_results=2000, seed=11, dtype=dtype)
+  want = ohalton.sample(7, num_results=2000,
+                        perms=_np(params.perms),
+                        zero_correction=_np(params.zero_correction),
+                        dtype=npd)
+  np.testing.assert_allclose(_np(got), want, rtol=rtol, atol=1e-30)
+  # the same seed gives the same points; params reproduce them by index
+  again, _ = halton.sample(7, num_results=2000, seed=11, dtype=dtype)
+  np.testing.assert_array_equal(_np(again), _np(got))
+  part, _ = halton.sample(7, sequence_indices=np.arange(1000, 2000),
+                          randomization_params=params, dtype=dtype)
+  np.testing.assert_array_equal(_np(part), _np(got)[1000:])
+
+
+def test_halton_randomized_integration_and_normal_draws():
+  from oracle import halton as ohalton
+  from tqf.math.random import halton
+  # E[x1 x2^2 x3^3] over the unit cube = 1/24 (halton_test.py:139-154)
+  est = []
+  for seed in range(10):
+    x, _ = halton.sample(3, num_results=1000, seed=seed,
+                         dtype=torch.float64)
+    x = _np(x)
+    est.append(np.mean(x[:, 0] * x[:, 1]**2 * x[:, 2]**3))
+  assert abs(np.mean(est) - 1 / 24) <= 2 * np.std(est) + 1e-4
+  # Halton normal draws: sqrt(2) erfinv(2u - 1) of the points, quasi layout
+  z = _np(mvn.multivariate_normal([500], mean=np.zeros(3),
+                                  random_type=mvn.RandomType.HALTON,
+                                  skip=10, dtype=torch.float64))
+  u = ohalton.sample(3, sequence_indices=np.arange(10, 510),
+                     dtype=np.float64)
+  np.testing.assert_allclose(z, onormal.quasi_normals(u), rtol=1e-12,
+                             atol=1e-12)
+
+
+def test_euler_halton_draws_parity():
+  from oracle import halton as ohalton
+  n, steps = 1024, 16
+  spec, fns, x0 = _models()['heston_ft']
+  u = ohalton.sample(2 * steps, sequence_indices=np.arange(n),
+                     dtype=np.float64)
+  z = onormal.quasi_normals(u).reshape(n, steps, 2)
+  want = oeuler.sample(2, fns[0], fns[1], np.asarray([1.0]), num_samples=n,
+                       initial_state=np.asarray(x0), normal_draws=z,
+                       num_time_steps=steps, dtype=np.float64)
+  drift = engine.tag(lambda t, x: None, spec)
+  got = tqf.models.euler_sampling.sample(
+      2, drift, drift, np.asarray([1.0]), num_time_steps=steps,
+      num_samples=n, initial_state=np.asarray(x0),
+      random_type=mvn.RandomType.HALTON, dtype=torch.float64)
+  _close(_np(got), want, 1e-11)
+
+
+def test_qmc_scrambled_digital_net():
+  nd, n = 12, 4096
+  g = qmc.sobol_generating_matrices(5, n, nd)
+  sm = qmc.random_scrambling_matrices(5, nd, seed=(2, 3))
+  got = _np(qmc.digital_net_sample(g, n, nd, scrambling_matrices=sm,
+                                   dtype=torch.float64))
+  want = osobol.digital_net_sample_int(
+      osobol.scramble_generating_matrices(g, sm, nd), n, nd) / 2.0**nd
+  np.testing.assert_array_equal(got, want)

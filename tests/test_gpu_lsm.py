@@ -5,7 +5,7 @@ import pytest
 import torch
 
 from conftest import load_golden
-from oracle import gbm as ogbm
+from oracle import closed_form, gbm as ogbm
 from oracle import lsm as olsm
 
 pytestmark = pytest.mark.gpu
@@ -148,3 +148,72 @@ def test_reference_pinv_cutoff_loses_the_fit_on_ill_conditioned_bases():
                             discount_factors=df, dtype=np.float64)
   np.testing.assert_allclose(_np(got), exact, rtol=2e-6)
   assert raw[0] < exact[0]
+
+
+def _crr_american(s0, k, t, r, sigma, call, steps=4000):
+  """Cox-Ross-Rubinstein binomial tree with early exercise (independent
+  check of the American value)."""
+  dt = t / steps
+  u = np.exp(sigma * np.sqrt(dt))
+  d = 1 / u
+  p = (np.exp(r * dt) - d) / (u - d)
+  disc = np.exp(-r * dt)
+  j = np.arange(steps + 1)
+  s = s0 * u**(steps - j) * d**j
+  v = np.maximum(s - k, 0) if call else np.maximum(k - s, 0)
+  for n in range(steps - 1, -1, -1):
+    s = s[:-1] / u
+    v = disc * (p * v[:-1] + (1 - p) * v[1:])
+    v = np.maximum(v, np.maximum(s - k, 0) if call else np.maximum(k - s, 0))
+  return v[0]
+
+
+def test_bs_lsm_price_american_options():
+  from tqf.experimental import american_option
+  spots = np.array([100.0, 90.0, 100.0])
+  strikes = np.array([110.0, 100.0, 95.0])
+  expiries = np.array([1.0, 0.5, 1.0])
+  vols = np.array([0.25, 0.3, 0.2])
+  r = 0.05
+  df = np.exp(-r * expiries)
+  call = np.array([False, False, True])
+  got = _np(american_option.bs_lsm_price(
+      spots, expiries, strikes, vols, df, num_samples=100000,
+      num_exercise_times=50, is_call_option=call, seed=(1, 2)))
+  for i in range(3):
+    want = _crr_american(spots[i], strikes[i], expiries[i], r, vols[i],
+                         call[i])
+    # LSM on 50 dates with 1e5 paths: low-biased by its sub-optimal rule
+    # and by the discrete exercise grid, plus Monte Carlo noise
+    assert abs(got[i] - want) <= 0.015 * want + 0.02, (i, got[i], want)
+  # no early exercise for a call without dividends: Black-Scholes
+  bs = closed_form.bs_price(spots[2], strikes[2], 1.0, r, 0.2)
+  assert abs(got[2] - bs) <= 0.015 * bs
+
+
+def test_bs_lsm_price_matches_oracle_on_the_same_paths():
+  # the GPU LSM (basis on ln S, payoff on S) against the oracle's
+  # restatement of least_square_mc on the engine's own paths
+  import tqf
+  from tqf.experimental import american_option
+  from tqf.math.random.multivariate_normal import RandomType
+  n, m, s0, k, t, sig, r = 20000, 20, 100.0, 105.0, 0.75, 0.3, 0.04
+  got = float(_np(american_option.bs_lsm_price(
+      [s0], [t], [k], [sig], [np.exp(-r * t)], num_samples=n,
+      num_exercise_times=m, is_call_option=False, seed=(3, 4)))[0])
+  times = np.linspace(0.0, 1.0, m)
+  unit = _np(tqf.models.GeometricBrownianMotion(
+      0.0, 1.0, dtype=torch.float64).sample_paths(
+          times, initial_state=1.0, num_samples=n, seed=(3, 4),
+          random_type=RandomType.STATELESS_ANTITHETIC))[..., 0]
+  w = np.log(unit) + 0.5 * times
+  log_s = sig * np.sqrt(t) * w + (r - 0.5 * sig**2) * times * t
+  kk = k / s0
+
+  def payoff(x, ti):
+    return np.maximum(kk - np.exp(x[:, ti, 0]), 0)[:, None]
+  want = olsm.least_square_mc(log_s[..., None], list(range(m)), payoff,
+                              olsm.make_polynomial_basis(2, True),
+                              discount_factors=np.exp(-r * times * t))[0]
+  # same exercise decisions up to rounding of the regression
+  np.testing.assert_allclose(got, want * s0, rtol=2e-6)

@@ -207,3 +207,26 @@ def test_icdf_table_matches_inverse_normal_exhaustively():
   assert err[~big].max() <= 2e-10
   # symmetry z(n) = -z(2^25 - 1 - n) holds exactly
   np.testing.assert_array_equal(p, -p[::-1])
+
+
+def test_sobol_scrambling_host_matches_oracle_and_keeps_the_net():
+  from tqf.math import qmc
+  nd, n = 10, 1024
+  g = qmc.sobol_generating_matrices(6, n, nd)
+  sm = qmc.random_scrambling_matrices(6, nd, seed=(2, 3))
+  assert sm.shape == (6, nd)
+  assert np.all(sm >= 2**(nd - 1)) and np.all(sm < 2**nd)
+  got = qmc.scramble_generating_matrices(g, sm, nd)
+  np.testing.assert_array_equal(got,
+                                osobol.scramble_generating_matrices(g, sm, nd))
+  # minimum scrambling matrices are the identity (digital_net_test.py:274)
+  ident = np.full_like(sm, 2**(nd - 1))
+  np.testing.assert_array_equal(qmc.scramble_generating_matrices(g, ident, nd),
+                                g)
+  # the scrambled net is still a (0, m, 1)-net in every dimension: each
+  # aligned block of 2^m points takes every value j / 2^m once
+  pts = osobol.digital_net_sample_int(got, n, nd)
+  for d in range(6):
+    for m in (1, 4, 10):
+      blk = (pts[:2**m, d] >> (nd - m)).astype(np.int64)
+      assert sorted(blk) == list(range(2**m))

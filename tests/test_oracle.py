@@ -381,3 +381,36 @@ def test_local_vol_table_is_exact_on_bilinear_surfaces():
                               np.array([x[0] - 1, x[-1] + 1]))
   np.testing.assert_allclose(got, [vols[-1, 0], vols[-1, -1]], rtol=1e-13)
 
+
+
+def test_halton_oracle_known_values_and_radical_inverse():
+  from fractions import Fraction
+  from oracle import halton
+  # halton_test.py:30-37
+  want = np.array([[1. / 2, 1. / 3], [1. / 4, 2. / 3], [3. / 4, 1. / 9],
+                   [1. / 8, 4. / 9], [5. / 8, 7. / 9]])
+  np.testing.assert_allclose(halton.sample(2, num_results=5), want, rtol=1e-6)
+  p = halton.primes(1000)
+  assert list(p[:10]) == [2, 3, 5, 7, 11, 13, 17, 19, 23, 29]
+  assert p[-1] == 7919  # the 1000th prime
+  # radical inverse with exact fractions, float64 truncation irrelevant here
+  idx = np.array([0, 1, 7, 100, 12345, 999999])
+  got = halton.sample(6, sequence_indices=idx, dtype=np.float64)
+  for i, k in enumerate(idx):
+    for d, b in enumerate(p[:6]):
+      n, f, w = int(k) + 1, Fraction(0), Fraction(1)
+      while n:
+        w /= int(b)
+        f += (n % int(b)) * w
+        n //= int(b)
+      assert abs(got[i, d] - float(f)) <= 1e-15
+  np.testing.assert_array_equal(
+      halton.sample(3, num_results=50),
+      halton.sample(3, sequence_indices=np.arange(50)))
+  # identity permutations and no correction leave the sequence unchanged
+  bases = p[:3]
+  ident = np.concatenate([np.arange(b) for b in bases])
+  perms = np.tile(ident, (24, 1))
+  np.testing.assert_array_equal(
+      halton.sample(3, num_results=40, perms=perms),
+      halton.sample(3, num_results=40))

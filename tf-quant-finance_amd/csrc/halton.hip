@@ -1,0 +1,103 @@
+// Halton sequences (radical inverses in prime bases), plain or randomised
+// by digit permutations, as uniforms or as normal draws.
+//
+// Element k of the sequence is the radical inverse of the integer k + 1 in
+// the prime base b_d of each axis, truncated to max_digits[d] base-b digits
+// (tff/math/random_ops/halton/halton_impl.py:59-285, indices 1-based at
+// :386-408).  Randomised (Owen 2017, :252-285, :288-330): digit j of axis d
+// goes through perms[j * radix_sum + offset[d] + digit] -- the zeros past
+// the integer's length included, up to max_digits[d] -- and
+// zero_correction[d] is added.  One thread per (point, axis); the value is
+// accumulated in float64 and rounded once to the output type.
+#include "common.h"
+
+using namespace tqf;
+
+namespace {
+
+struct HaltonArgs {
+  const int32_t* bases;
+  const int32_t* max_digits;
+  const int32_t* offsets;
+  int dim;
+  int64_t n;
+  int64_t first;
+  const int64_t* seq;
+  const int32_t* perms;
+  int radix_sum;
+  const double* zero_correction;
+  int kind;  // 1 float32, 2 float64 uniforms; 3, 4 the same as normals
+  void* out;
+  int64_t stride;
+};
+
+__global__ void __launch_bounds__(256) halton_kernel(HaltonArgs a) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n * (int64_t)a.dim) return;
+  const int64_t i = g / a.dim;
+  const int d = (int)(g - i * a.dim);
+  uint64_t k = (uint64_t)((a.seq ? a.seq[i] : a.first + i) + 1);
+  const uint64_t b = (uint64_t)a.bases[d];
+  const int nd = a.max_digits[d];
+  double acc = 0.0, w = 1.0;
+  if (a.perms) {
+    const int32_t* pd = a.perms + a.offsets[d];
+    for (int j = 0; j < nd; ++j) {
+      const uint64_t q = k / b;
+      const int c = (int)(k - q * b);
+      k = q;
+      w *= (double)b;
+      acc += (double)pd[(int64_t)j * a.radix_sum + c] / w;
+    }
+  } else {
+    for (int j = 0; j < nd && k; ++j) {
+      const uint64_t q = k / b;
+      const int c = (int)(k - q * b);
+      k = q;
+      w *= (double)b;
+      acc += (double)c / w;
+    }
+  }
+  if (a.zero_correction) acc += a.zero_correction[d];
+  const int64_t o = i * a.stride + d;
+  if (a.kind == 1) {
+    ((float*)a.out)[o] = (float)acc;
+  } else if (a.kind == 2) {
+    ((double*)a.out)[o] = acc;
+  } else if (a.kind == 3) {
+    // the reference's transform in float32: erfinv((u - 0.5) * 2) sqrt 2
+    const float u = (float)acc;
+    ((float*)a.out)[o] = erfinv_f32((u - 0.5f) * 2.0f) * 1.41421356f;
+  } else {
+    ((double*)a.out)[o] = erfinv((acc - 0.5) * 2.0) * 1.4142135623730951;
+  }
+}
+
+}  // namespace
+
+extern "C" int tqf_halton_points(const int32_t* bases,
+                                 const int32_t* max_digits,
+                                 const int32_t* offsets, int32_t dim,
+                                 int64_t n, int64_t first_index,
+                                 const int64_t* seq_indices,
+                                 const int32_t* perms, int32_t radix_sum,
+                                 const double* zero_correction,
+                                 int32_t out_kind, void* out,
+                                 int64_t out_stride, void* stream) {
+  clear_error();
+  TQF_REQUIRE(dim >= 1 && dim <= 1000,
+              "`dim` should be in [1, 1000], got %d", dim);
+  TQF_REQUIRE(n >= 0 && first_index >= 0, "bad sizes");
+  TQF_REQUIRE(bases && max_digits && out, "null argument");
+  TQF_REQUIRE(perms == nullptr || (offsets != nullptr && radix_sum >= 2),
+              "randomised Halton needs offsets and radix_sum");
+  TQF_REQUIRE(out_kind >= 1 && out_kind <= 4, "bad out_kind %d", out_kind);
+  TQF_REQUIRE(out_stride >= dim, "out_stride < dim");
+  if (n == 0) return TQF_OK;
+  HaltonArgs a{bases, max_digits, offsets, dim, n, first_index, seq_indices,
+               perms, radix_sum, zero_correction, out_kind, out, out_stride};
+  const int64_t total = n * (int64_t)dim;
+  hipLaunchKernelGGL(halton_kernel, dim3((unsigned)((total + 255) / 256)),
+                     dim3(256), 0, (hipStream_t)stream, a);
+  return check_launch("tqf_halton_points");
+}

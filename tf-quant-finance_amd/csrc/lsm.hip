@@ -44,7 +44,16 @@ struct LsmArgs {
   const uint8_t* expo;      // [K][d] exponents
   int K;
   int deg;
+  int basis_log;            // basis built on ln(x) (payoff still on x)
 };
+
+// The variable the basis is built on: x, or ln x when the caller regresses
+// on log prices (the pricing platform's American options regress on
+// ln S: tff/experimental/pricing_platform/framework/equity_instruments/
+// american_option/utils.py:150-167).
+__device__ __forceinline__ double basis_input(const LsmArgs& a, double x) {
+  return a.basis_log ? log(x) : x;
+}
 
 __device__ __forceinline__ double payoff_value(const LsmArgs& a,
                                                const double* x) {
@@ -78,8 +87,9 @@ lsm_prepare_kernel(LsmArgs a, double df_ratio, const double* __restrict__ cash,
     y[p] = df_ratio * (cash[p] + vals[p]);
   }
   for (int j = 0; j < a.d; ++j) {
-    red[j][threadIdx.x] = x[j];
-    red[kMaxAssets + j][threadIdx.x] = x[j] * x[j];
+    const double b = p < a.n_paths ? basis_input(a, x[j]) : 0.0;
+    red[j][threadIdx.x] = b;
+    red[kMaxAssets + j][threadIdx.x] = b * b;
   }
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -164,7 +174,7 @@ lsm_gram_kernel(LsmArgs a, const double* __restrict__ exv,
       if (p < p_end) {
         const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
         for (int j = 0; j < a.d; ++j) {
-          const double c = (row[j] - a.mean[j]) * a.scale[j];
+          const double c = (basis_input(a, row[j]) - a.mean[j]) * a.scale[j];
           double v = 1.0;
           for (int e = 0; e <= a.deg; ++e) {
             pw[s][j * deg1 + e] = v;
@@ -338,7 +348,7 @@ lsm_update_kernel(LsmArgs a, double df_ratio, const double* __restrict__ exv,
   double pw[kMaxPw];
   const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
   for (int j = 0; j < a.d; ++j) {
-    const double c = (row[j] - a.mean[j]) * a.scale[j];
+    const double c = (basis_input(a, row[j]) - a.mean[j]) * a.scale[j];
     double v = 1.0;
     for (int e = 0; e <= a.deg; ++e) {
       pw[j * deg1 + e] = v;
@@ -423,7 +433,8 @@ lsm_features_kernel(LsmArgs a, const double* __restrict__ exv,
   const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
   float* xr = x + p * kMlpStride;
   for (int j = 0; j < kMlpIn - 1; ++j)
-    xr[j] = j < a.d ? (float)((row[j] - a.mean[j]) * a.scale[j]) : 0.f;
+    xr[j] = j < a.d ? (float)((basis_input(a, row[j]) - a.mean[j]) * a.scale[j])
+                    : 0.f;
   xr[kMlpIn - 1] = (float)(exv[p] / a.strike);
   xr[6] = xr[7] = 0.f;
   tgt[p] = (float)(y[p] / a.strike);
@@ -466,6 +477,7 @@ LsmArgs make_args(const tqf_lsm* l, int t_idx) {
   a.expo = l->exponents;
   a.K = l->n_basis;
   a.deg = l->degree;
+  a.basis_log = l->basis_log;
   return a;
 }
 

@@ -349,6 +349,24 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
     partial[(int64_t)blockIdx.x * (kMlpParams + 1) + e] = G[e];
 }
 
+// Level 1 of the gradient reduction: workgroup (x, c) adds the per-block
+// partials of blocks [c * per, (c + 1) * per) for entries x*256 + t, so the
+// reduction runs on many CUs instead of one row of workgroups.
+constexpr int kGradChunks = 32;
+
+__global__ void __launch_bounds__(256)
+mlp_partial_reduce_kernel(const float* __restrict__ partial, int n_blocks,
+                          int per, float* __restrict__ chunks) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e > kMlpParams) return;
+  const int b0 = blockIdx.y * per;
+  const int b1 = b0 + per < n_blocks ? b0 + per : n_blocks;
+  float g = 0.f;
+  for (int b = b0; b < b1; ++b) g += partial[(int64_t)b * (kMlpParams + 1) + e];
+  chunks[(int64_t)blockIdx.y * (kMlpParams + 1) + e] = g;
+}
+
+// Level 2 (the chunk sums, fixed order) and the Adam update.
 __global__ void __launch_bounds__(256)
 mlp_adam_kernel(float* __restrict__ params, float* __restrict__ m,
                 float* __restrict__ v, const float* __restrict__ partial,
@@ -393,9 +411,15 @@ int mlp_grad_step(float* params, float* m, float* v, const float* x,
                   hipStream_t s) {
   hipLaunchKernelGGL(mlp_grad_kernel, n_blocks, 256, 0, s, params, x, tgt, w,
                      first, n, inv_norm, partial);
-  hipLaunchKernelGGL(mlp_adam_kernel, (kMlpParams + 1 + 255) / 256, 256, 0, s,
-                     params, m, v, partial, n_blocks, lr, 0.9f, 0.999f, 1e-8f,
-                     step, grad_out, loss_out);
+  const int n_chunks = n_blocks < kGradChunks ? n_blocks : kGradChunks;
+  const int per = (n_blocks + n_chunks - 1) / n_chunks;
+  float* chunks = partial + (int64_t)n_blocks * (kMlpParams + 1);
+  const unsigned tiles = (kMlpParams + 1 + 255) / 256;
+  hipLaunchKernelGGL(mlp_partial_reduce_kernel, dim3(tiles, n_chunks), 256, 0,
+                     s, partial, n_blocks, per, chunks);
+  hipLaunchKernelGGL(mlp_adam_kernel, tiles, 256, 0, s, params, m, v, chunks,
+                     n_chunks, lr, 0.9f, 0.999f, 1e-8f, step, grad_out,
+                     loss_out);
   return check_launch("mlp_grad_step");
 }
 
@@ -413,7 +437,8 @@ extern "C" int tqf_mlp_forward(const float* params, const float* x, int64_t n,
 }
 
 extern "C" int64_t tqf_mlp_workspace_size(int32_t n_blocks) {
-  return (int64_t)n_blocks * (kMlpParams + 1) * 4;
+  const int64_t chunks = n_blocks < kGradChunks ? n_blocks : kGradChunks;
+  return ((int64_t)n_blocks + chunks) * (kMlpParams + 1) * 4;
 }
 
 extern "C" int tqf_mlp_train_step(float* params, float* adam_m, float* adam_v,

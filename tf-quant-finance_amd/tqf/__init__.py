@@ -9,11 +9,15 @@ Mirrors of the reference call surface:
   tqf.models.MultivariateGeometricBrownianMotion
   tqf.models.HestonModel                  <- tff.models.HestonModel
   tqf.models.longstaff_schwartz           <- tff.models.longstaff_schwartz
+  tqf.models.LocalVolatilityModel         <- tff.experimental.local_volatility
+  tqf.experimental.american_option.bs_lsm_price
+                                          <- pricing platform american_option
 All compute runs in hand-written HIP kernels (libtqf.so) on the GPU.
 """
 from tqf import errors
 from tqf import math
 from tqf import models
 from tqf import engine
+from tqf import experimental
 
-__all__ = ['errors', 'math', 'models', 'engine']
+__all__ = ['errors', 'math', 'models', 'engine', 'experimental']

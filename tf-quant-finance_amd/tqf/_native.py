@@ -63,7 +63,7 @@ class Gbm(ctypes.Structure):
 class Lsm(ctypes.Structure):
   _fields_ = [('paths', _c_void_p), ('n_paths', _i64), ('n_times', _i32),
               ('dim', _i32), ('payoff', _i32), ('degree', _i32),
-              ('n_basis', _i32), ('pad0', _i32), ('exponents', _c_void_p),
+              ('n_basis', _i32), ('basis_log', _i32), ('exponents', _c_void_p),
               ('strike', _f64), ('num_calibration', _i64), ('rcond', _f64)]
 
 
@@ -94,6 +94,9 @@ _SIGS = {
                                  _i32, _i32, _c_void_p, _c_void_p]),
     'tqf_philox_normals': (_i32, [_u64, _u32, _i64, _i64, _i32, _i32,
                                   _c_void_p, _c_void_p]),
+    'tqf_halton_points': (_i32, [_c_void_p, _c_void_p, _c_void_p, _i32, _i64,
+                                 _i64, _c_void_p, _c_void_p, _i32, _c_void_p,
+                                 _i32, _c_void_p, _i64, _c_void_p]),
     'tqf_euler_workspace_size': (_i64, [_i64, _i32, _i32]),
     'tqf_euler': (_i32, [ctypes.POINTER(Model), ctypes.POINTER(Grid),
                          ctypes.POINTER(Rng), ctypes.POINTER(Output), _i64,

@@ -1,0 +1,95 @@
+"""American options by Longstaff-Schwartz under Black-Scholes (mirror of the
+pricing platform's `bs_lsm_price`, tff/experimental/pricing_platform/
+framework/equity_instruments/american_option/utils.py:26-169).
+
+The reference draws one set of unit GBM paths exp(-t/2 + W(t)) on the
+normalised exercise grid t = linspace(0, 1, num_exercise_times) and maps
+them to every option of the batch as spot-scaled paths
+S/S0 = exp(sigma sqrt(T) W(t) + (r - sigma^2/2) t T) (utils.py:103-136),
+with r = -ln(df)/T; each option is then priced by least_square_mc with the
+degree-2 polynomial basis on ln S and the call/put payoff on S
+(utils.py:137-167), and the result is scaled back by the spot.
+
+Here the unit paths come from the exact GBM kernel (tqf_gbm_exact) with
+the engine's antithetic Philox stream (the reference's
+STATELESS_ANTITHETIC), the per-option mapping is one elementwise pass on
+the GPU, and the regression runs in the FP64 LSM kernels (tqf_lsm_price,
+basis on ln S).  A user basis_fn falls back to the torch LSM with the same
+log-path / payoff split.
+"""
+import numpy as np
+import torch
+
+from tqf import _native
+from tqf.math.random.multivariate_normal import RandomType
+from tqf.models import GeometricBrownianMotion
+from tqf.models import longstaff_schwartz as lsm_lib
+from tqf.models import utils
+
+__all__ = ['bs_lsm_price']
+
+
+def _vec(v, n, dtype):
+  a = np.asarray(utils.to_numpy(v), np.float64).reshape(-1)
+  return np.broadcast_to(a, [n]).astype(np.float64)
+
+
+def bs_lsm_price(spots, expiry_times, strikes, volatility, discount_factors,
+                 num_samples=100000, num_exercise_times=100, basis_fn=None,
+                 seed=(1, 2), is_call_option=True,
+                 num_calibration_samples=None, dtype=None, name=None):
+  """American option prices [batch] (same arguments as the reference)."""
+  del name
+  dtype = dtype or torch.float64
+  if not isinstance(dtype, torch.dtype):
+    dtype = torch.float64 if np.dtype(dtype) == np.float64 else torch.float32
+  dev = _native.device()
+  spots_np = np.asarray(utils.to_numpy(spots), np.float64).reshape(-1)
+  n_opt = spots_np.shape[0]
+  expiry = _vec(expiry_times, n_opt, dtype)
+  strike = _vec(strikes, n_opt, dtype) / spots_np
+  vol = _vec(volatility, n_opt, dtype)
+  df = _vec(discount_factors, n_opt, dtype)
+  call = np.broadcast_to(np.asarray(utils.to_numpy(is_call_option),
+                                    bool).reshape(-1), [n_opt])
+  rate = -np.log(df) / expiry
+  times = np.linspace(0.0, 1.0, int(num_exercise_times))
+  gbm = GeometricBrownianMotion(0.0, 1.0, dtype=torch.float64)
+  # unit paths exp(-t/2 + W(t)) on the exercise grid, antithetic
+  unit = gbm.sample_paths(times, initial_state=1.0,
+                          num_samples=int(num_samples), seed=seed,
+                          random_type=RandomType.STATELESS_ANTITHETIC)
+  t = torch.as_tensor(times, dtype=torch.float64, device=dev).reshape(1, -1)
+  w = torch.log(unit[..., 0]) + 0.5 * t  # W(t), [N, T]
+  ex = list(range(int(num_exercise_times)))
+  if basis_fn is None:
+    basis_fn = lsm_lib.make_polynomial_basis(2)
+  bspec = getattr(basis_fn, '_tqf_basis', None)
+  prices = []
+  for b in range(n_opt):
+    tt = t * expiry[b]
+    log_s = (vol[b] * np.sqrt(expiry[b])) * w + (rate[b] - 0.5 * vol[b]**2) * tt
+    dfs = np.exp(-rate[b] * times * expiry[b])
+    kind = lsm_lib.PAYOFF_MAX_CALL if call[b] else lsm_lib.PAYOFF_BASKET_PUT
+    if bspec is not None and bspec[0] == 'poly':
+      paths = torch.exp(log_s).unsqueeze(-1).contiguous()  # [N, T, 1]
+      p = lsm_lib._native_lsm(
+          paths, ex, (kind, np.array([strike[b]])), bspec[1],
+          np.concatenate([[1.0], dfs]).reshape(-1, 1, 1),
+          num_calibration_samples,
+          torch.float64, dev, basis_log=True)
+      prices.append(float(p[0]))
+    else:
+      k, is_call = strike[b], bool(call[b])
+
+      def payoff_fn(log_paths, time_index, k=k, is_call=is_call):
+        s = torch.exp(log_paths[:, time_index, 0])
+        v = torch.relu(s - k) if is_call else torch.relu(k - s)
+        return v.unsqueeze(-1)
+      p = lsm_lib.least_square_mc(log_s.unsqueeze(-1), ex, payoff_fn,
+                                  basis_fn, discount_factors=dfs,
+                                  num_calibration_samples=num_calibration_samples,
+                                  dtype=torch.float64)
+      prices.append(float(p.reshape(-1)[0]))
+  out = torch.as_tensor(np.asarray(prices) * spots_np, dtype=dtype, device=dev)
+  return out

@@ -5,6 +5,8 @@
   sobol_sample               tff/math/qmc/sobol.py:32-129
   random_digital_shift       tff/math/qmc/digital_net.py:45-95 (values drawn
                              from the engine's Philox stream, not TF's)
+  random_scrambling_matrices tff/math/qmc/digital_net.py:98-152 (likewise)
+  scramble_generating_matrices tff/math/qmc/digital_net.py:422-527
   tent_transform             tff/math/qmc/utils.py:124-141
 
 Points are produced by the `tqf_sobol_points` kernel (index-0 sequence or
@@ -22,7 +24,8 @@ from tqf.math.random import sobol as sobol_lib
 from tqf.math.random.multivariate_normal import seed_to_key
 
 __all__ = ['sobol_generating_matrices', 'digital_net_sample', 'sobol_sample',
-           'random_digital_shift', 'tent_transform']
+           'random_digital_shift', 'random_scrambling_matrices',
+           'scramble_generating_matrices', 'tent_transform']
 
 
 def _log2_ceil_f32(n):
@@ -56,10 +59,13 @@ def digital_net_sample(generating_matrices, num_results, num_digits,
                        validate_args=False, dtype=None, name=None):
   """Samples a digital net on the GPU, shape [num_samples, dim]."""
   del name
-  if scrambling_matrices is not None:
-    raise NotImplementedError('scrambled digital nets are not supported')
   g = np.asarray(generating_matrices.detach().cpu() if isinstance(
       generating_matrices, torch.Tensor) else generating_matrices, np.int64)
+  if scrambling_matrices is not None:
+    # digital_net.py:349-353: the net of the scrambled matrices
+    g = scramble_generating_matrices(g, scrambling_matrices, num_digits,
+                                     validate_args=validate_args,
+                                     dtype=np.int64)
   if validate_args and g.ndim != 2:
     raise errors.InvalidArgumentError('generating_matrices must have rank 2')
   dim, ncols = g.shape
@@ -147,6 +153,56 @@ def random_digital_shift(dim, num_digits, seed, validate_args=False,
   vals = lo + (words.astype(np.uint64) % np.uint64(span)).astype(np.int64)
   return np.asarray(vals, np.int64 if dtype in (np.int64, torch.int64)
                     else np.int64)
+
+
+def random_scrambling_matrices(dim, num_digits, seed, validate_args=False,
+                               dtype=None, name=None):
+  """[dim, num_digits] integers uniform in [2^(nd-1), 2^nd): the rows of
+  the lower-triangular scrambling matrices, from the engine's Philox
+  stream (sub-stream 0x5343); the reference draws them from TF's stateless
+  uniform (digital_net.py:98-152), not reproducible here."""
+  del validate_args, name
+  if num_digits < 1 or num_digits > 32:
+    raise ValueError('num_digits must be in [1, 32]')
+  key = seed_to_key(seed)
+  words = philox_words(int(dim) * int(num_digits), key, stream_id=0x5343)
+  lo = 1 << (num_digits - 1)
+  vals = lo + (words.astype(np.uint64) % np.uint64(lo)).astype(np.int64)
+  np_dtype = np.int64 if dtype in (np.int64, torch.int64, 'int64') else np.int32
+  return vals.reshape(int(dim), int(num_digits)).astype(np_dtype)
+
+
+def scramble_generating_matrices(generating_matrices, scrambling_matrices,
+                                 num_digits, validate_args=False, dtype=None,
+                                 name=None):
+  """Linear matrix scrambling (digital_net.py:422-527): entry (d, j) of the
+  result is the XOR over s < num_digits of (S[d, s] >> s) where bit
+  num_digits - 1 - s of G[d, j] is set.  Host integers (the matrices are
+  [dim, log2 N])."""
+  del name
+  g = np.asarray(generating_matrices.detach().cpu() if isinstance(
+      generating_matrices, torch.Tensor) else generating_matrices, np.int64)
+  sm = np.asarray(scrambling_matrices.detach().cpu() if isinstance(
+      scrambling_matrices, torch.Tensor) else scrambling_matrices, np.int64)
+  nd = int(num_digits)
+  if validate_args:
+    if g.ndim != sm.ndim:
+      raise errors.InvalidArgumentError(
+          'input matrices must have the same rank')
+    if nd <= 0:
+      raise errors.InvalidArgumentError('num_digits must be positive')
+  out = np.zeros_like(g)
+  for s in range(nd):
+    col = (sm[:, s:s + 1] >> s)  # [dim, 1]
+    bit = (g >> (nd - 1 - s)) & 1
+    out ^= np.where(bit == 1, col, 0)
+  if dtype is None:
+    np_dtype = (generating_matrices.dtype if isinstance(
+        generating_matrices, np.ndarray) else np.int32)
+  else:
+    np_dtype = np.int64 if dtype in (np.int64, torch.int64, 'int64') \
+        else np.int32
+  return out.astype(np_dtype)
 
 
 def philox_words(n, key, stream_id=0):

@@ -153,10 +153,26 @@ def multivariate_normal(sample_shape, mean=None, covariance_matrix=None,
     nb, ns = len(batch_shape), len(sample_shape)
     perm = list(range(nb, nb + ns)) + list(range(nb - 1, -1, -1))
     z = zt.permute(perm)
+  elif random_type in (RandomType.HALTON, RandomType.HALTON_RANDOMIZED):
+    # _mvnormal_quasi (multivariate_normal.py:393-424): Halton points of
+    # sequence indices skip .. skip + n - 1, the reference transform
+    from tqf.math.random import halton
+    skip = int(kwargs.get('skip', 0) or 0)
+    batch_rev = batch_shape[::-1]
+    out_shape_t = batch_rev + sample_shape
+    num_samples = int(np.prod(out_shape_t)) // dim
+    zs, _ = halton.sample(
+        dim, sequence_indices=np.arange(skip, skip + num_samples),
+        randomized=random_type == RandomType.HALTON_RANDOMIZED,
+        randomization_params=kwargs.get('randomization_params'), seed=seed,
+        dtype=dtype, _normal=True)
+    zt = zs.transpose(0, 1).reshape(out_shape_t)
+    nb, ns = len(batch_shape), len(sample_shape)
+    perm = list(range(nb, nb + ns)) + list(range(nb - 1, -1, -1))
+    z = zt.permute(perm)
   else:
     raise NotImplementedError(
-        'Only STATELESS, PSEUDO, PSEUDO_ANTITHETIC, STATELESS_ANTITHETIC and '
-        'SOBOL random types are supported. Supplied: {}'.format(random_type))
+        'random type {} is not supported'.format(random_type))
   if scale is not None:
     z = torch.einsum('...ij,...j->...i', scale, z)
   if mean_t is not None:

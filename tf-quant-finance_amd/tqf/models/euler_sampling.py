@@ -94,6 +94,14 @@ def sample(dim, drift_fn, volatility_fn, times, time_step=None,
     raise NotImplementedError('batch_shape in initial_state is not supported')
 
   spec = engine.spec_of(drift_fn, volatility_fn)
+  if (spec is not None and draws is None and random_type in
+      (RandomType.HALTON, RandomType.HALTON_RANDOMIZED)):
+    # Halton draws are produced up front ([N, steps, d], the reference's
+    # quasi layout) and read by the fused kernel as injected draws
+    draws = utils.generate_mc_normal_draws(
+        num_normal_draws=dim, num_time_steps=steps,
+        num_sample_paths=num_samples, random_type=random_type, seed=seed,
+        skip=skip, dtype=dtype).permute(1, 0, 2)
   if spec is not None and watch_params is None and engine.model_dim(spec) == dim:
     rng = engine.RngSpec.from_random_type(random_type, seed, skip, draws)
     return engine.run(spec, all_times, keep_mask, k, x0, num_samples, rng,

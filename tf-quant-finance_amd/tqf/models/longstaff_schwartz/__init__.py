@@ -130,7 +130,7 @@ def least_square_mc(sample_paths, exercise_times, payoff_fn, basis_fn,
 
 
 def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
-                return_details=False):
+                return_details=False, basis_log=False):
   kind, strikes = pspec
   paths_b = x.unsqueeze(0) if x.dim() == 3 else x  # [Bp, N, T, d]
   bp = paths_b.shape[0]
@@ -159,6 +159,7 @@ def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
     l.payoff = kind
     l.degree = degree
     l.n_basis = k
+    l.basis_log = int(bool(basis_log))
     l.exponents = expo_t.data_ptr()
     l.strike = float(strikes[b if strikes.shape[0] > 1 else 0])
     l.num_calibration = int(num_calib or 0)

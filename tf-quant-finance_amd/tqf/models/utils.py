@@ -148,6 +148,15 @@ def generate_mc_normal_draws(num_normal_draws, num_time_steps,
                        'PSEUDO_ANTITHETIC random type')
     flat = mvn.philox_normals(n, steps * d, mvn.seed_to_key(seed, random_type),
                               dtype, antithetic=anti)
+  elif random_type in (mvn.RandomType.HALTON,
+                       mvn.RandomType.HALTON_RANDOMIZED):
+    # quasi draws: sequence index skip + p, dimension step * d + k
+    from tqf.math.random import halton
+    flat, _ = halton.sample(
+        steps * d, sequence_indices=np.arange(int(skip or 0),
+                                              int(skip or 0) + n),
+        randomized=random_type == mvn.RandomType.HALTON_RANDOMIZED,
+        seed=seed, dtype=dtype, _normal=True)
   else:
     raise NotImplementedError('random type {} is not supported'.format(
         random_type))
