@@ -70,10 +70,9 @@ def cpu_baseline(n_paths, chunk=2**14):
 def accuracy_check(prices, ses):
   """FP32 2^20-path prices against the engine's float64 run of the same
   scheme at 2^24 paths (the float64 path is pinned to the numpy oracle at
-  1e-12 by the parity tests) and against the semi-analytic Heston price
-  (which also sees the Euler discretisation bias)."""
+  1e-12 by the parity tests, and to the semi-analytic Heston price by
+  tests/test_gpu_core.py)."""
   import torch
-  from oracle import heston as oheston
   from tqf.engine import pricing
   ref = pricing.heston_ft_calls(num_paths=2**24, num_time_steps=NUM_STEPS,
                                 maturities=MATURITIES, grid='reference',
@@ -81,12 +80,6 @@ def accuracy_check(prices, ses):
                                 dtype=torch.float64, **PARAMS)
   rp = ref['prices'].cpu().numpy()
   rse = ref['stderrs'].cpu().numpy()
-  strikes = np.linspace(60, 140, 64)
-  sa = np.array([[oheston.european_call(PARAMS['s0'], k, t, PARAMS['r'],
-                                        PARAMS['v0'], PARAMS['kappa'],
-                                        PARAMS['theta'], PARAMS['xi'],
-                                        PARAMS['rho'])
-                  for k in strikes] for t in MATURITIES])
   big = rp > 1.0
   err = np.abs(prices - rp)
   comb = np.sqrt(ses**2 + rse**2)
@@ -97,8 +90,39 @@ def accuracy_check(prices, ses):
       'max_err_in_combined_se': float(np.max(err / comb)),
       'frac_within_3se': float(np.mean(err <= 3 * comb)),
       'max_stderr': float(np.max(ses)),
-      'fp64_vs_semi_analytic_max_abs': float(np.max(np.abs(rp - sa))),
   }
+
+
+def batch_accuracy(contracts, batch_prices, seed=11, every=256):
+  """Batched FP32 prices against a float64 run of the same scheme on the
+  same Philox streams (the plain fused kernel, per contract) for every
+  `every`-th contract: the FP32 arithmetic error, free of Monte Carlo noise.
+  """
+  import torch
+  from tqf.engine import euler as engine
+  dev = torch.device('cuda', 0)
+  errs = []
+  for i in range(0, len(contracts), every):
+    c = contracts[i]
+    n, T, r = c['n_steps'], c['maturity'], c['rate']
+    all_times = np.linspace(0.0, T, n + 1)
+    keep = np.zeros(n + 1, bool)
+    keep[-1] = True
+    rng = engine.RngSpec('philox', seed=seed, stream_id=c.get('stream_id', i))
+    if c['kind'] == 0:
+      spec = ('const', r - 0.5 * c['sigma']**2, c['sigma'])
+      x0 = [np.log(c['s0'])]
+    else:
+      spec = ('heston', r, c['kappa'], c['theta'], c['xi'], c['rho'], 0.0)
+      x0 = [np.log(c['s0']), c['v0']]
+    res = engine.run(spec, all_times, keep, 1, x0, c['n_paths'], rng,
+                     torch.float64, mode='payoff', strikes=[c['strike']],
+                     discount=[np.exp(-r * T)], dev=dev)
+    ref = float(res['prices'].cpu()[0, 0])
+    errs.append(abs(float(batch_prices[i]) - ref) / max(abs(ref), 1e-2))
+  return {'vs': 'float64 plain kernel, same Philox streams, %d contracts'
+                % len(errs),
+          'max_rel_err_price': float(np.max(errs))}
 
 
 def contract_loop(contracts, seed=11, h=1e-3):
@@ -220,13 +244,18 @@ def secondary_metrics():
                           'seconds': t}
   # batched heterogeneous contracts with Greeks
   cs = batch_lib.random_contracts(4096)
-  plan = batch_lib.BatchPlan(cs, tile=1024)
+  cfg, tune = batch_lib.autotune(cs, cache_path=None)
+  plan = batch_lib.BatchPlan(cs, **cfg)
   g = plan.capture()
   t, _ = timed(lambda: g.replay(), 2)
+  acc = batch_accuracy(cs, plan.out[:, 0].cpu().numpy())
   out['batch_4096'] = {'value': 4096 / t, 'unit': 'options+Greeks/s',
-                       'seconds': t,
+                       'seconds': t, 'check': acc,
                        'path_steps': plan.total_path_steps,
-                       'path_steps_per_s': plan.total_path_steps / t}
+                       'path_steps_per_s': plan.total_path_steps / t,
+                       'autotuned': cfg,
+                       'candidates_s': {'%d/%d' % (r['tile'], r['n_blocks']):
+                                        round(r['seconds'], 4) for r in tune}}
   # the same contracts through the plain fused kernel, one call per contract
   # (price only, and price + central-difference Greeks with common random
   # numbers: 2 more calls per Greek)

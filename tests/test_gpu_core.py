@@ -560,3 +560,48 @@ def test_qmc_scrambled_digital_net():
   want = osobol.digital_net_sample_int(
       osobol.scramble_generating_matrices(g, sm, nd), n, nd) / 2.0**nd
   np.testing.assert_array_equal(got, want)
+
+
+def test_heston_ft_fp64_engine_vs_semi_analytic():
+  # the headline scheme in float64 (2^20 shifted-Sobol paths, the reference
+  # grid of 365 steps) against the independent Lewis-integral price: the
+  # difference is the Euler bias plus Monte Carlo error
+  strikes = np.linspace(60, 140, 64)
+  mats = (0.25, 0.5, 0.75, 1.0)
+  res = pricing.heston_ft_calls(num_paths=2**20, num_time_steps=365,
+                                maturities=mats, grid='reference',
+                                random='sobol_shifted', seed=7,
+                                dtype=torch.float64, s0=100.0, v0=0.04,
+                                kappa=2.0, theta=0.04, xi=0.3, rho=-0.7,
+                                r=0.03)
+  p, se = _np(res['prices']), _np(res['stderrs'])
+  sa = np.array([[oheston.european_call(100.0, k, t, 0.03, 0.04, 2.0, 0.04,
+                                        0.3, -0.7) for k in strikes]
+                 for t in mats])
+  assert np.all(np.abs(p - sa) <= 3 * se + 4e-3), np.max(np.abs(p - sa))
+
+
+def test_batch_float32_matches_float64_plain_kernel_on_the_same_streams():
+  from tqf.engine import batch as batch_lib
+  cs = batch_lib.random_contracts(64, seed=5)
+  cs = [dict(c, n_paths=min(c['n_paths'], 8192)) for c in cs[:12]]
+  plan = batch_lib.BatchPlan(cs, tile=1024, seed=11)
+  out = _np(plan.launch())
+  for i, c in enumerate(cs):
+    n, t, r = c['n_steps'], c['maturity'], c['rate']
+    grid = np.linspace(0.0, t, n + 1)
+    keep = np.zeros(n + 1, bool)
+    keep[-1] = True
+    if c['kind'] == 0:
+      spec = ('const', r - 0.5 * c['sigma']**2, c['sigma'])
+      x0 = [np.log(c['s0'])]
+    else:
+      spec = ('heston', r, c['kappa'], c['theta'], c['xi'], c['rho'], 0.0)
+      x0 = [np.log(c['s0']), c['v0']]
+    ref = engine.run(spec, grid, keep, 1, x0, c['n_paths'],
+                     engine.RngSpec('philox', seed=11, stream_id=i),
+                     torch.float64, mode='payoff', strikes=[c['strike']],
+                     discount=[np.exp(-r * t)])
+    want = float(_np(ref['prices'])[0, 0])
+    assert abs(out[i, 0] - want) <= 1e-3 * max(want, 1e-1), (i, out[i, 0],
+                                                             want)
