@@ -253,6 +253,9 @@ def secondary_metrics():
                        'seconds': t, 'check': acc,
                        'path_steps': plan.total_path_steps,
                        'path_steps_per_s': plan.total_path_steps / t,
+                       'flop': plan.total_flop,
+                       'achieved_tflops': plan.total_flop / t / 1e12,
+                       'frac_fp32_peak': plan.total_flop / t / PEAK_FP32,
                        'autotuned': cfg,
                        'candidates_s': {'%d/%d' % (r['tile'], r['n_blocks']):
                                         round(r['seconds'], 4) for r in tune}}

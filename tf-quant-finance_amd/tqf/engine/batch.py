@@ -18,7 +18,7 @@ import torch
 from tqf import _native
 
 __all__ = ['BatchPlan', 'autotune', 'random_contracts', 'GREEK_NAMES',
-           'KIND_GBM', 'KIND_HESTON']
+           'KIND_GBM', 'KIND_HESTON', 'FLOP_PER_PATH_STEP']
 
 KIND_GBM, KIND_HESTON = 0, 1
 GREEK_NAMES = {
@@ -26,6 +26,19 @@ GREEK_NAMES = {
     KIND_HESTON: ['price', 'delta', 'dv0', 'rho', 'dkappa', 'dtheta', 'dxi',
                   'drho'],
 }
+# FLOP per path-step of batch_kernel (csrc/batch.hip), counted with add/
+# sub/mul/max/select 1, FMA 2, sqrt/log/exp/sin/cos/div 1, integer work 0,
+# a Box-Muller normal 5 (10 per pair: 2 scale, log, mul, sqrt, sin, cos,
+# 2 mul, amortised):
+#   GBM     normal 5 + dw 1 + drift add 1 + sigma*dw FMA 2 + vega sum 1 = 10
+#   Heston  forward 30 (2 normals 10; max, sqrt, 2 dw; log-spot 6;
+#           variance 10) + the same 30 again when each checkpoint segment
+#           is recomputed + reverse sweep 39 (max, sqrt, 2 dw 4; sqrt-bar 6;
+#           v-bar 7; rate-bar 2; kappa-bar 4; theta-bar 2; xi-bar 7;
+#           rho-bar 5; v update 2) = 99
+# Per-path payoff and Greek assembly (a few dozen FLOP per path) is left
+# out: at >= 50 steps it is under 1% of the total.
+FLOP_PER_PATH_STEP = {KIND_GBM: 10, KIND_HESTON: 99}
 _FIELDS = ['kind', 'n_steps', 'n_paths', 's0', 'strike', 'maturity', 'rate',
            'sigma', 'v0', 'kappa', 'theta', 'xi', 'rho']
 CACHE_PATH = os.environ.get(
@@ -115,6 +128,9 @@ class BatchPlan:
     self.out = torch.empty((n, 16), dtype=torch.float64, device=self.dev)
     self.total_path_steps = int(sum(c['n_paths'] * c['n_steps']
                                     for c in self.contracts))
+    self.total_flop = int(sum(c['n_paths'] * c['n_steps'] *
+                              FLOP_PER_PATH_STEP[c['kind']]
+                              for c in self.contracts))
 
   def launch(self):
     _native.call('tqf_batch_price', ctypes.c_void_p(self._contracts.data_ptr()),
