@@ -139,9 +139,14 @@ typedef struct tqf_model {
                    HESTON_FT: r, kappa, theta, xi, rho, variant
                    (variant 0: full truncation; 1: HestonModel's own
                    coefficients, raw v drift and sqrt|v| vol).
-                   LOCAL_VOL: drift rate (r - q).
+                   LOCAL_VOL: p[0] drift rate (r - q); p[1] != 0 selects
+                   edge mode: each table row is [value below column 1,
+                   nx - 2 interior columns, value above], lv_x0 is the
+                   log-spot of column 1, and outside the interior nodes
+                   the edge value is used as it is.
                    HESTON_QE: theta (per-step values in step_coef). */
-  const void* lv_vol;   /* LOCAL_VOL: device [nt][nx] sigma table          */
+  const void* lv_vol;   /* LOCAL_VOL: device [nt][nx] sigma table (bilinear
+                           in t and ln S, flat outside; see p[1])          */
   const void* lv_t;     /* device [nt] table times (increasing)            */
   double lv_x0;         /* log-spot of table column 0                      */
   double lv_dx;         /* uniform log-spot spacing                        */

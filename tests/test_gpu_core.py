@@ -443,13 +443,15 @@ def test_local_vol_flat_surface_is_black_scholes():
     assert abs(pay.mean() - bs) <= 3 * pay.std() / np.sqrt(st.size), k
 
 
-def test_local_vol_precompute_iv_tabulates_and_rejects_grids():
+def test_local_vol_precompute_iv_steps_on_times_grid_and_rejects_grids():
+  # a user callable is kept as it is; precompute_iv only makes the model
+  # step on times_grid (local_volatility_model.py:425-436, 521-529)
   surf_fn = lambda t, s: 0.2 + 0.0 * s
   model = tqf.models.LocalVolatilityModel(
       1, risk_free_rate=0.02, local_volatility_fn=surf_fn,
       times_grid=np.linspace(0.0, 1.0, 11), spot_grid=[50.0, 200.0],
       precompute_iv=True, dtype=torch.float64)
-  assert isinstance(model.local_volatility_fn(), tqf.models.LocalVolSurface)
+  assert model.local_volatility_fn() is surf_fn
   with pytest.raises(ValueError):
     model.sample_paths([1.0], initial_state=[100.0], time_step=0.1)
   p = _np(model.sample_paths([1.0], initial_state=[100.0], num_samples=1000,

@@ -43,6 +43,7 @@ struct EulerArgs {
                   // [n_steps][8] (StepRow)
   R lv_x0, lv_inv_dx;
   int lv_nt, lv_nx, lv_in_lds;
+  int lv_edge;  // rows carry [below, interior..., above] values
   // grid
   const R* dt;
   const R* sqrt_dt;
@@ -78,17 +79,30 @@ struct ModelDim {
 template <typename R>
 __device__ __forceinline__ R lv_sigma(const EulerArgs<R>& a, const R* tab,
                                       int row, R wt, R x) {
-  R fx = (x - a.lv_x0) * a.lv_inv_dx;
-  fx = fx < (R)0 ? (R)0 : fx;
-  const R fmax = (R)(a.lv_nx - 1);
+  // interior columns [e, nx - e) with e = 1 in edge mode; column 0 and
+  // nx - 1 then hold the values below the first and above the last node
+  const int e = a.lv_edge;
+  const int ni = a.lv_nx - 2 * e;
+  const R fraw = (x - a.lv_x0) * a.lv_inv_dx;
+  const R fmax = (R)(ni - 1);
+  R fx = fraw < (R)0 ? (R)0 : fraw;
   fx = fx > fmax ? fmax : fx;
   int j = (int)fx;
-  j = j > a.lv_nx - 2 ? a.lv_nx - 2 : j;
+  j = j > ni - 2 ? ni - 2 : j;
   const R wx = fx - (R)j;
   const R* r0 = tab + (int64_t)row * a.lv_nx;
   const R* r1 = row + 1 < a.lv_nt ? r0 + a.lv_nx : r0;
-  const R s0 = r0[j] + wx * (r0[j + 1] - r0[j]);
-  const R s1 = r1[j] + wx * (r1[j + 1] - r1[j]);
+  int c0 = e + j, c1 = e + j + 1;
+  R w = wx;
+  if (e) {
+    // outside the nodes: the edge column, no interpolation in S
+    const bool lo = fraw < (R)0, hi = fraw > fmax;
+    c0 = lo ? 0 : hi ? a.lv_nx - 1 : c0;
+    c1 = lo ? 0 : hi ? a.lv_nx - 1 : c1;
+    w = (lo || hi) ? (R)0 : w;
+  }
+  const R s0 = r0[c0] + w * (r0[c1] - r0[c0]);
+  const R s1 = r1[c0] + w * (r1[c1] - r1[c0]);
   return s0 + wt * (s1 - s0);
 }
 
@@ -455,6 +469,7 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   a.lv_inv_dx = m->lv_dx > 0 ? (R)(1.0 / m->lv_dx) : (R)0;
   a.lv_nt = m->lv_nt;
   a.lv_nx = m->lv_nx;
+  a.lv_edge = (m->kind == TQF_MODEL_LOCAL_VOL && m->p[1] != 0.0) ? 1 : 0;
   a.dt = (const R*)g->dt;
   a.sqrt_dt = (const R*)g->sqrt_dt;
   a.t_next = (const R*)g->t_next;
@@ -587,6 +602,9 @@ extern "C" int tqf_euler(const tqf_model* m, const tqf_grid* g,
     TQF_REQUIRE(m->lv_vol && m->lv_t && m->lv_nt >= 1 && m->lv_nx >= 2 &&
                     m->lv_dx > 0,
                 "bad local volatility table");
+    TQF_REQUIRE(m->p[1] == 0.0 || m->lv_nx >= 4,
+                "an edge-mode local volatility table needs two interior "
+                "columns and the two edge columns");
     TQF_REQUIRE(g->t_next != nullptr, "null t_next");
   }
   TQF_REQUIRE(out->mode >= TQF_OUT_PATH_MAJOR &&

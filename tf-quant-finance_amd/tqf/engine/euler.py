@@ -47,23 +47,41 @@ def model_dim(spec):
 class LocalVolTable:
   """sigma(t, S) on a (t_i, ln S_j) grid with a uniform log-spot axis.
 
-  Lookups are bilinear in (t, ln S) with flat extrapolation; the kernel
-  stages the table in LDS when it fits (up to 32 KB).
+  Lookups are bilinear in (t, ln S).  Outside the nodes the value is flat,
+  or, with `outside` ([nt, 2]: below the first node, above the last), those
+  values are used as they are (a surface that jumps at the ends of its
+  spot range).  The kernel stages the table in LDS when it fits.
   """
 
-  def __init__(self, times, log_spot_min, log_spot_step, vols, drift_rate):
+  def __init__(self, times, log_spot_min, log_spot_step, vols, drift_rate,
+               outside=None):
     self.times = np.asarray(times, np.float64)
     self.x0 = float(log_spot_min)
     self.dx = float(log_spot_step)
     self.vols = np.asarray(vols, np.float64)
     self.drift_rate = float(drift_rate)
+    self.outside = None if outside is None else np.asarray(outside,
+                                                           np.float64)
     if self.vols.shape != (self.times.shape[0], self.vols.shape[1]):
       raise ValueError('vols must have shape [len(times), n_spots]')
+    if self.outside is not None and self.outside.shape != (
+        self.times.shape[0], 2):
+      raise ValueError('outside must have shape [len(times), 2]')
+
+  def device_rows(self):
+    """The kernel's row layout: [nt][nx] or, with edge values,
+    [nt][1 + nx + 1]."""
+    if self.outside is None:
+      return self.vols
+    return np.concatenate([self.outside[:, :1], self.vols,
+                           self.outside[:, 1:]], 1)
 
   def sigma(self, t, x):
     """Host evaluation of the table (same interpolation as the kernel)."""
     nt, nx = self.vols.shape
-    fx = np.clip((np.asarray(x) - self.x0) / self.dx, 0, nx - 1)
+    x = np.asarray(x, np.float64)
+    fraw = (x - self.x0) / self.dx
+    fx = np.clip(fraw, 0, nx - 1)
     j = np.minimum(fx.astype(np.int64), nx - 2)
     wx = fx - j
     row = max(int(np.searchsorted(self.times, t, side='right')) - 1, 0)
@@ -73,6 +91,11 @@ class LocalVolTable:
       wt = (t - self.times[row]) / (self.times[row + 1] - self.times[row])
     s0 = self.vols[row, j] + wx * (self.vols[row, j + 1] - self.vols[row, j])
     s1 = self.vols[r1, j] + wx * (self.vols[r1, j + 1] - self.vols[r1, j])
+    if self.outside is not None:
+      lo, hi = fraw < 0, fraw > nx - 1
+      o = self.outside
+      s0 = np.where(lo, o[row, 0], np.where(hi, o[row, 1], s0))
+      s1 = np.where(lo, o[r1, 0], np.where(hi, o[r1, 1], s1))
     return s0 + wt * (s1 - s0)
 
 
@@ -164,12 +187,14 @@ def _model_struct(spec, dtype, dev, keep, step_times=None, step_slots=None):
     tab = spec[1]
     m.kind = _native.MODEL_LOCAL_VOL
     m.p[0] = tab.drift_rate
-    vol = torch.as_tensor(tab.vols, dtype=dtype, device=dev).contiguous()
+    m.p[1] = 0.0 if tab.outside is None else 1.0
+    rows = tab.device_rows()
+    vol = torch.as_tensor(rows, dtype=dtype, device=dev).contiguous()
     tt = torch.as_tensor(tab.times, dtype=dtype, device=dev).contiguous()
     keep.extend([vol, tt])
     m.lv_vol, m.lv_t = vol.data_ptr(), tt.data_ptr()
     m.lv_x0, m.lv_dx = tab.x0, tab.dx
-    m.lv_nt, m.lv_nx = tab.vols.shape
+    m.lv_nt, m.lv_nx = rows.shape
   else:
     raise NotImplementedError('unknown model spec %r' % (spec[0],))
   return m
