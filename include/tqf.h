@@ -119,6 +119,8 @@ int tqf_philox_normals(uint64_t seed, uint32_t stream_id, int64_t n_paths,
 #define TQF_MODEL_HESTON_FT 2 /* (log S, v), full truncation, drift r      */
 #define TQF_MODEL_LOCAL_VOL 3 /* log S, sigma(t, S) from a bilinear table   */
 #define TQF_MODEL_HESTON_QE 4 /* (log S, v), Andersen QE (HestonModel)      */
+#define TQF_MODEL_HULL_WHITE 5 /* short rate r = x + f(0, t), exact Gaussian
+                                  step of x (HullWhiteModel1F)              */
 
 #define TQF_RNG_DRAWS 0            /* injected normals                      */
 #define TQF_RNG_PHILOX 1
@@ -157,6 +159,12 @@ typedef struct tqf_model {
                              (s^2 = v c1 + c2), 3..7 k0..k4 of the log-spot
                              update, 8 theta, 9 update flag (0 when
                              dt <= tolerance), 10-11 unused.
+                             HULL_WHITE: device [n_steps][8] = {e^{-a dt},
+                             conditional mean, conditional standard
+                             deviation, f(0, t_{i+1}), 1, slot[i] (bits as
+                             for HESTON_FT), 0, 0}: x' = e x + m + s z and
+                             the recorded rate is x + f(0, t); p[0] =
+                             f(0, t_0) for a record at the first time.
                              HESTON_FT: device [n_steps][8] = {-dt/2,
                              kappa theta dt, -kappa dt, r t_{i+1},
                              sqrt(dt), slot[i] (the grid's record slot,

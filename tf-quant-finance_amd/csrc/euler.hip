@@ -115,8 +115,9 @@ struct StepRow {
   R c[8];
 };
 template <int MODEL>
-constexpr bool kHasStepRow =
-    MODEL == TQF_MODEL_HESTON_FT || MODEL == kModelHestonOwn;
+constexpr bool kHasStepRow = MODEL == TQF_MODEL_HESTON_FT ||
+                             MODEL == kModelHestonOwn ||
+                             MODEL == TQF_MODEL_HULL_WHITE;
 
 // Scalar load of row i (the compiler merges the eight into one s_load).
 template <typename R>
@@ -164,6 +165,10 @@ struct Stepper {
       const R c1 = fma_r(a.p[4], z[1], a.p[3] * z[0]);
       x[0] = fma_r(ssq, z[0], x[0]);
       x[1] = fma_r(ssq, c1, x[1]);
+    } else if (MODEL == TQF_MODEL_HULL_WHITE) {
+      // exact Gaussian step of x = r - f(0, t) (vector_hull_white.py:
+      // 738-760): x' = (e^{-a dt} x + m) + s z, row {e, m, s, f(0, t'), ...}
+      x[0] = fma_r(row.c[2], z[0], fma_r(row.c[0], x[0], row.c[1]));
     } else if (MODEL == TQF_MODEL_HESTON_QE) {
       // Andersen QE (heston_model.py:522-572), psi_c = 1.5, gamma = 1/2.
       R c[10];
@@ -285,7 +290,9 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   nrm.init(a.rng, pc, D, lds, a.n_steps);
 
   if (a.init_slot >= 0)
-    record<R, MODEL>(a, lds_pay, a.init_slot, x, (R)0, path, active);
+    record<R, MODEL>(a, lds_pay, a.init_slot, x,
+                     MODEL == TQF_MODEL_HULL_WHITE ? a.p[0] : (R)0, path,
+                     active);
 
   int lv_row = 0;
   const int n_dims = a.n_steps * D;
@@ -527,6 +534,7 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   TQF_PICK_RNG(kModelHestonOwn)
   TQF_PICK_RNG(TQF_MODEL_LOCAL_VOL)
   TQF_PICK_RNG(TQF_MODEL_HESTON_QE)
+  TQF_PICK_RNG(TQF_MODEL_HULL_WHITE)
 #undef TQF_PICK_RNG
 #undef TQF_PICK
   if (!kern) return set_error(TQF_ENOTIMPL, "unsupported model/rng combination");
@@ -559,13 +567,21 @@ extern "C" int tqf_euler(const tqf_model* m, const tqf_grid* g,
   TQF_REQUIRE(dtype == TQF_F32 || dtype == TQF_F64, "bad dtype %d", dtype);
   TQF_REQUIRE(n_paths >= 1, "num_samples must be positive, got %lld",
               (long long)n_paths);
-  TQF_REQUIRE(m->kind >= TQF_MODEL_GBM && m->kind <= TQF_MODEL_HESTON_QE,
+  TQF_REQUIRE(m->kind >= TQF_MODEL_GBM && m->kind <= TQF_MODEL_HULL_WHITE,
               "unknown model kind %d", m->kind);
   const int D =
       (m->kind == TQF_MODEL_HESTON_FT || m->kind == TQF_MODEL_HESTON_QE) ? 2 : 1;
   if (m->kind == TQF_MODEL_HESTON_QE || m->kind == TQF_MODEL_HESTON_FT) {
     TQF_REQUIRE(m->step_coef != nullptr || g->n_steps == 0,
                 "Heston models need their per-step constants (step_coef)");
+  }
+  if (m->kind == TQF_MODEL_HULL_WHITE) {
+    TQF_REQUIRE(m->step_coef != nullptr || g->n_steps == 0,
+                "the Hull-White model needs its per-step constants "
+                "(step_coef)");
+    TQF_REQUIRE(out->mode == TQF_OUT_PATH_MAJOR ||
+                    out->mode == TQF_OUT_STEP_MAJOR,
+                "the Hull-White model records rates, not option payoffs");
   }
   TQF_REQUIRE(m->dim == D, "model dim %d does not match kind (%d)", m->dim, D);
   TQF_REQUIRE(g->n_steps >= 0 && g->n_slots >= 1, "bad grid sizes");

@@ -5,6 +5,7 @@ A *coefficient spec* names a model whose drift/volatility the kernels know:
   ('const', a, b)                    dX = a dt + b dW
   ('heston', r, kappa, theta, xi, rho, variant)   (log S, v)
   ('local_vol', LocalVolTable)       d ln S = (r - q - s^2/2) dt + s dW
+  ('hull_white', coef_fn)            r = x + f(0, t), x' = e x + m + s z
   ('heston_qe', coef[steps, 12])     Andersen QE with per-step constants
 Callables returned by the tqf model classes carry their spec in the
 `_tqf_spec` attribute, so `euler_sampling.sample` can route them here.
@@ -183,6 +184,28 @@ def _model_struct(spec, dtype, dev, keep, step_times=None, step_slots=None):
                            device=dev).contiguous()
     keep.append(coef)
     m.step_coef = coef.data_ptr()
+  elif spec[0] == 'hull_white':
+    # spec[1](step_times) -> (decay, mean, sd, f(0, t_{i+1}), f(0, t_0)) of
+    # the exact steps between consecutive grid times
+    m.kind = _native.MODEL_HULL_WHITE
+    npd = utils.np_dtype(dtype)
+    t = np.asarray(step_times, np.float64)
+    decay, mean, sd, f_next, f_first = spec[1](t)
+    m.p[0] = float(f_first)
+    n = t.shape[0] - 1
+    coef = np.zeros((max(n, 1), 8), npd)
+    coef[:n, 0] = decay[:n]
+    coef[:n, 1] = mean[:n]
+    coef[:n, 2] = sd[:n]
+    coef[:n, 3] = f_next[:n]
+    coef[:n, 4] = 1.0
+    ib = np.int32 if npd == np.float32 else np.int64
+    bits = coef[:, 5].view(ib)
+    bits[:] = -1
+    bits[:n] = np.asarray(step_slots[:n], ib)
+    ct = torch.from_numpy(np.ascontiguousarray(coef)).to(dev)
+    keep.append(ct)
+    m.step_coef = ct.data_ptr()
   elif spec[0] == 'local_vol':
     tab = spec[1]
     m.kind = _native.MODEL_LOCAL_VOL

@@ -11,6 +11,8 @@ __all__ = ['PiecewiseConstantFunc']
 class PiecewiseConstantFunc:
   """f(x) = values[i] on (jumps[i-1], jumps[i]] (left-continuous)."""
 
+  is_piecewise_constant = True
+
   def __init__(self, jump_locations, values, dtype=None, name=None):
     del name
     self._jumps = np.asarray(jump_locations, np.float64).reshape(-1)

@@ -122,7 +122,11 @@ def least_square_mc(sample_paths, exercise_times, payoff_fn, basis_fn,
   df = _padded_discounts(discount_factors, n_ex, dtype)
   bspec = getattr(basis_fn, '_tqf_basis', None)
   pspec = getattr(payoff_fn, '_tqf_payoff', None)
-  if bspec is not None and pspec is not None and bspec[0] == 'poly':
+  # the native kernels take one discount factor per date; discount
+  # factors that vary by path run through the torch restatement
+  per_path = df.ndim == 3 and df.shape[1] > 1
+  if (bspec is not None and pspec is not None and bspec[0] == 'poly' and
+      not per_path):
     return _native_lsm(x, ex, pspec, bspec[1], df, num_calibration_samples,
                        dtype, dev)
   return _torch_lsm(x, ex, payoff_fn, basis_fn, df, num_calibration_samples,
