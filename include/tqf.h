@@ -270,6 +270,8 @@ int tqf_gbm_exact(const tqf_gbm* gbm, const tqf_grid* grid, const tqf_rng* rng,
 
 #define TQF_PAYOFF_BASKET_PUT 0  /* relu(K - mean_j S_j) (payoff_utils.py:27) */
 #define TQF_PAYOFF_MAX_CALL 1    /* relu(max_j S_j - K)                        */
+#define TQF_PAYOFF_TABLE 2       /* exercise values given per (time, path):
+                                    payoff_table[t_idx][p]                    */
 
 typedef struct tqf_lsm {
   const double* paths;      /* device [n_paths][n_times][dim] float64        */
@@ -288,6 +290,13 @@ typedef struct tqf_lsm {
   int64_t num_calibration;  /* rows used for the regression (0 = all)       */
   double rcond;             /* pivot cutoff relative to the largest scaled
                                diagonal of X^T X                            */
+  const double* payoff_table;   /* TQF_PAYOFF_TABLE: device
+                                   [n_times][n_paths] exercise values (>= 0;
+                                   in the money iff > 0)                     */
+  const double* path_discounts; /* optional device [n_ex + 1][n_paths]: the
+                                   padded discount factors of every path
+                                   (row 0 = 1); when set they replace df in
+                                   tqf_lsm_price                             */
 } tqf_lsm;
 
 int64_t tqf_lsm_workspace_size(int64_t n_paths, int32_t n_basis);
@@ -295,7 +304,9 @@ int64_t tqf_lsm_workspace_size(int64_t n_paths, int32_t n_basis);
 /* Device.  Longstaff-Schwartz price of one payoff: backward induction over
  * the exercise indices ex_idx[n_ex] (time indices into the paths), with the
  * padded discount factors df[n_ex + 1] (df[0] = 1, df[i] = discount of
- * exercise date i-1), the ITM-masked float64 normal equations built in the
+ * exercise date i-1; per path when lsm->path_discounts is set, the
+ * reference's rank-3 discount factors), the ITM-masked float64 normal
+ * equations built in the
  * kernel from the centred tensor-product basis, a column-scaled Cholesky
  * solve with pivot truncation, and the exercise update.  cash/vals are
  * device [n_paths] state buffers; result[2] = (price, standard error) over

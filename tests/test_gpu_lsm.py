@@ -217,3 +217,36 @@ def test_bs_lsm_price_matches_oracle_on_the_same_paths():
                               discount_factors=np.exp(-r * times * t))[0]
   # same exercise decisions up to rounding of the regression
   np.testing.assert_allclose(got, want * s0, rtol=2e-6)
+
+
+def test_table_payoff_and_path_discounts_native_matches_torch_and_oracle():
+  # exercise values given as a table and discount factors that differ by
+  # path (the reference's rank-3 discount_factors, lsm.py:239-256): the
+  # native kernels against the torch restatement and the oracle
+  rs = np.random.RandomState(12)
+  n, t = 30000, 6
+  z = rs.standard_normal((n, t, 1))
+  times = np.linspace(0.2, 1.2, t)
+  paths = ogbm.sample_paths(times, 0.02, 0.3, 1.0, z)  # [N, T, 1]
+  rate = 0.03 + 0.01 * rs.standard_normal((n, t))
+  dfs = np.exp(-np.cumsum(rate * 0.2, axis=1))  # [N, T]
+  strikes = np.array([1.0, 1.1])
+  table = np.maximum(strikes[None, :, None] - paths[:, None, :, 0], 0)
+  table_t = torch.as_tensor(table, device='cuda')
+  basis = lsm.make_polynomial_basis(2)
+  native = lsm.least_square_mc(paths, list(range(t)),
+                               lsm.make_table_payoff(table_t), basis,
+                               discount_factors=dfs[:, None, :],
+                               dtype=np.float64)
+  generic = lsm.least_square_mc(paths, list(range(t)),
+                                lambda x, i: table_t[:, :, i],
+                                lambda x, i: basis(x, i),
+                                discount_factors=dfs[:, None, :],
+                                dtype=np.float64)
+  want = olsm.least_square_mc(paths, list(range(t)),
+                              lambda x, i: table[:, :, i],
+                              olsm.make_polynomial_basis(2, True),
+                              discount_factors=dfs[:, None, :])
+  np.testing.assert_allclose(_np(native), want, rtol=2e-6)
+  np.testing.assert_allclose(_np(generic), want, rtol=2e-6)
+  assert native.shape == (2,)

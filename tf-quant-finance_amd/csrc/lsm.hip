@@ -45,7 +45,19 @@ struct LsmArgs {
   int K;
   int deg;
   int basis_log;            // basis built on ln(x) (payoff still on x)
+  const double* table;      // TQF_PAYOFF_TABLE: [n_times][n_paths]
+  // per-path discount rows of the current date (numerator e + 1 and
+  // denominator e of the padded factors), or null for the scalar ratio
+  const double* dnum;
+  const double* dden;
 };
+
+// Discount ratio of path p: the per-path rows when given, else the date's
+// scalar ratio.
+__device__ __forceinline__ double disc_ratio(const LsmArgs& a, double ratio,
+                                             int64_t p) {
+  return a.dnum ? a.dnum[p] / a.dden[p] : ratio;
+}
 
 // The variable the basis is built on: x, or ln x when the caller regresses
 // on log prices (the pricing platform's American options regress on
@@ -56,8 +68,10 @@ __device__ __forceinline__ double basis_input(const LsmArgs& a, double x) {
 }
 
 __device__ __forceinline__ double payoff_value(const LsmArgs& a,
-                                               const double* x) {
-  if (a.payoff == TQF_PAYOFF_BASKET_PUT) {
+                                               const double* x, int64_t p) {
+  if (a.payoff == TQF_PAYOFF_TABLE) {
+    return a.table[(int64_t)a.t_idx * a.n_paths + p];
+  } else if (a.payoff == TQF_PAYOFF_BASKET_PUT) {
     double s = 0;
     for (int j = 0; j < a.d; ++j) s += x[j];
     const double v = a.strike - s / a.d;
@@ -83,8 +97,8 @@ lsm_prepare_kernel(LsmArgs a, double df_ratio, const double* __restrict__ cash,
   if (p < a.n_paths) {
     const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
     for (int j = 0; j < a.d; ++j) x[j] = row[j];
-    exv[p] = payoff_value(a, x);
-    y[p] = df_ratio * (cash[p] + vals[p]);
+    exv[p] = payoff_value(a, x, p);
+    y[p] = disc_ratio(a, df_ratio, p) * (cash[p] + vals[p]);
   }
   for (int j = 0; j < a.d; ++j) {
     const double b = p < a.n_paths ? basis_input(a, x[j]) : 0.0;
@@ -363,7 +377,7 @@ lsm_update_kernel(LsmArgs a, double df_ratio, const double* __restrict__ exv,
     vals[p] = 0;
     cash[p] = ex;
   } else {
-    vals[p] = df_ratio * (cash[p] + vals[p]);
+    vals[p] = disc_ratio(a, df_ratio, p) * (cash[p] + vals[p]);
     cash[p] = 0;
   }
 }
@@ -377,19 +391,21 @@ lsm_terminal_kernel(LsmArgs a, double* __restrict__ cash,
   double x[kMaxAssets];
   const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
   for (int j = 0; j < a.d; ++j) x[j] = row[j];
-  cash[p] = payoff_value(a, x);
+  cash[p] = payoff_value(a, x, p);
   vals[p] = 0;
 }
 
-// Per-block sums of df0 * (cash + vals) over rows [first, n) (and squares).
+// Per-block sums of df0 * (cash + vals) over rows [first, n) (and squares);
+// d1 / d0 (optional) are the per-path discount rows 1 and 0.
 __global__ void __launch_bounds__(256)
 lsm_final_kernel(const double* __restrict__ cash, const double* __restrict__ vals,
-                 double df0, int64_t first, int64_t n,
+                 double df0, const double* __restrict__ d1,
+                 const double* __restrict__ d0, int64_t first, int64_t n,
                  double* __restrict__ block_sums) {
   __shared__ double s1[256], s2[256];
   const int64_t p = first + (int64_t)blockIdx.x * 256 + threadIdx.x;
   double v = 0;
-  if (p < n) v = df0 * (cash[p] + vals[p]);
+  if (p < n) v = (d1 ? d1[p] / d0[p] : df0) * (cash[p] + vals[p]);
   s1[threadIdx.x] = v;
   s2[threadIdx.x] = v * v;
   __syncthreads();
@@ -478,6 +494,9 @@ LsmArgs make_args(const tqf_lsm* l, int t_idx) {
   a.K = l->n_basis;
   a.deg = l->degree;
   a.basis_log = l->basis_log;
+  a.table = l->payoff_table;
+  a.dnum = nullptr;
+  a.dden = nullptr;
   return a;
 }
 
@@ -491,8 +510,11 @@ int validate(const tqf_lsm* l) {
               "basis size must be in [1, 255], got %d", l->n_basis);
   TQF_REQUIRE(l->n_paths >= 1 && l->n_times >= 1, "empty sample paths");
   TQF_REQUIRE(l->payoff == TQF_PAYOFF_BASKET_PUT ||
-                  l->payoff == TQF_PAYOFF_MAX_CALL,
+                  l->payoff == TQF_PAYOFF_MAX_CALL ||
+                  l->payoff == TQF_PAYOFF_TABLE,
               "unknown payoff %d", l->payoff);
+  TQF_REQUIRE(l->payoff != TQF_PAYOFF_TABLE || l->payoff_table,
+              "the table payoff needs payoff_table");
   TQF_REQUIRE(l->paths && l->exponents, "null LSM arrays");
   return TQF_OK;
 }
@@ -585,6 +607,10 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
     a = make_args(&l2, ex_idx[e - 1]);
     a.mean = mean;
     a.scale = scale;
+    if (l->path_discounts) {
+      a.dnum = l->path_discounts + (int64_t)(e + 1) * n;
+      a.dden = l->path_discounts + (int64_t)e * n;
+    }
     const double ratio = df[e + 1] / df[e];
     hipLaunchKernelGGL(lsm_prepare_kernel, blocks_for(n), 256, 0, s, a, ratio,
                        cash, vals, exv, y, bsum);
@@ -611,8 +637,10 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
   const int64_t count = n - first;
   TQF_REQUIRE(count >= 1, "no pricing rows after the calibration samples");
   const int64_t fb = (count + 255) / 256;
+  const double* pd = l->path_discounts;
   hipLaunchKernelGGL(lsm_final_kernel, blocks_for(count), 256, 0, s, cash,
-                     vals, df[1] / df[0], first, n, fsum);
+                     vals, df[1] / df[0], pd ? pd + n : nullptr, pd, first,
+                     n, fsum);
   hipLaunchKernelGGL(lsm_final_reduce_kernel, 1, 64, 0, s, fsum, fb, count,
                      result);
   return check_launch("tqf_lsm_price finalize");
@@ -691,7 +719,8 @@ extern "C" int tqf_lsm_final(const double* cash, const double* vals,
   const int64_t fb = (count + 255) / 256;
   const hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(lsm_final_kernel, blocks_for(count), 256, 0, s, cash,
-                     vals, df0, first, n, workspace);
+                     vals, df0, (const double*)nullptr, (const double*)nullptr,
+                     first, n, workspace);
   hipLaunchKernelGGL(lsm_final_reduce_kernel, 1, 64, 0, s, workspace, fb,
                      count, result);
   return check_launch("tqf_lsm_final");

@@ -65,7 +65,8 @@ class Lsm(ctypes.Structure):
   _fields_ = [('paths', _c_void_p), ('n_paths', _i64), ('n_times', _i32),
               ('dim', _i32), ('payoff', _i32), ('degree', _i32),
               ('n_basis', _i32), ('basis_log', _i32), ('exponents', _c_void_p),
-              ('strike', _f64), ('num_calibration', _i64), ('rcond', _f64)]
+              ('strike', _f64), ('num_calibration', _i64), ('rcond', _f64),
+              ('payoff_table', _c_void_p), ('path_discounts', _c_void_p)]
 
 
 class Contract(ctypes.Structure):

@@ -168,11 +168,9 @@ def _lsm_inputs(plan, bonds, r_t, pay, dcf, cpn, tdt):
 def _price_on_paths(plan, bonds, r_t, pay, dcf, cpn, basis, tdt):
   short_rate, payoff_table, dfs = _lsm_inputs(plan, bonds, r_t, pay, dcf,
                                               cpn, tdt)
-
-  def payoff_fn(rt, time_index):
-    del rt
-    return payoff_table[:, :, time_index]
-
+  # the table payoff and per-path discount factors keep the regression in
+  # the native LSM kernels
+  payoff_fn = lsm.make_table_payoff(payoff_table)
   return lsm.least_square_mc(short_rate, list(range(payoff_table.shape[-1])),
                              payoff_fn, basis,
                              discount_factors=dfs[:, None, :], dtype=tdt)

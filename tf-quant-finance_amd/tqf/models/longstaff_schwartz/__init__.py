@@ -5,6 +5,8 @@ tff.models.longstaff_schwartz).
   make_polynomial_basis     lsm.py:50-125
   make_basket_put_payoff    payoff_utils.py:27-97
   make_max_call_payoff      engine extension: relu(max_j S_j - K)
+  make_table_payoff         engine extension: exercise values given per
+                            (path, payoff, date), e.g. swap values
 
 With the basis and payoff built here, the whole backward induction runs in
 the tqf_lsm_price kernels (float64 normal equations built on the fly,
@@ -26,9 +28,9 @@ from tqf.models import utils
 
 __all__ = ['least_square_mc', 'make_polynomial_basis',
            'make_basket_put_payoff', 'make_max_call_payoff',
-           'exponent_grid']
+           'make_table_payoff', 'exponent_grid']
 
-PAYOFF_BASKET_PUT, PAYOFF_MAX_CALL = 0, 1
+PAYOFF_BASKET_PUT, PAYOFF_MAX_CALL, PAYOFF_TABLE = 0, 1, 2
 
 
 def exponent_grid(degree, dim):
@@ -87,6 +89,18 @@ def make_max_call_payoff(strikes, dtype=None, name=None):
   return _payoff(PAYOFF_MAX_CALL, strikes, dtype)
 
 
+def make_table_payoff(values):
+  """payoff_fn(paths, t) = values[:, :, t] for exercise values [N, B,
+  n_times] computed elsewhere (non-negative; in the money iff > 0)."""
+  table = torch.as_tensor(values)
+
+  def payoff(sample_paths, time_index):
+    del sample_paths
+    return table[:, :, int(time_index)]
+  payoff._tqf_payoff = (PAYOFF_TABLE, table)
+  return payoff
+
+
 def _padded_discounts(discount_factors, n_ex, dtype):
   """Reference padding/transposition (lsm.py:239-256) -> [n_ex + 1, *, B]."""
   if discount_factors is None:
@@ -122,11 +136,12 @@ def least_square_mc(sample_paths, exercise_times, payoff_fn, basis_fn,
   df = _padded_discounts(discount_factors, n_ex, dtype)
   bspec = getattr(basis_fn, '_tqf_basis', None)
   pspec = getattr(payoff_fn, '_tqf_payoff', None)
-  # the native kernels take one discount factor per date; discount
-  # factors that vary by path run through the torch restatement
-  per_path = df.ndim == 3 and df.shape[1] > 1
+  # discount factors are one per date, or (rank 3) one per path and date;
+  # other broadcast shapes run through the torch restatement
+  n_paths = x.shape[-3]
+  df_ok = df.ndim <= 2 or df.shape[1] in (1, n_paths)
   if (bspec is not None and pspec is not None and bspec[0] == 'poly' and
-      not per_path):
+      df_ok):
     return _native_lsm(x, ex, pspec, bspec[1], df, num_calibration_samples,
                        dtype, dev)
   return _torch_lsm(x, ex, payoff_fn, basis_fn, df, num_calibration_samples,
@@ -138,7 +153,12 @@ def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
   kind, strikes = pspec
   paths_b = x.unsqueeze(0) if x.dim() == 3 else x  # [Bp, N, T, d]
   bp = paths_b.shape[0]
+  table = None
+  if kind == PAYOFF_TABLE:
+    table = strikes.to(device=dev, dtype=torch.float64)  # [N, B, T]
+    strikes = np.ones(table.shape[1])
   n_batch = max(bp, strikes.shape[0], df.shape[-1])
+  per_path = df.ndim == 3 and df.shape[1] > 1
   d = paths_b.shape[-1]
   expo = exponent_grid(degree, d).astype(np.uint8)
   k = expo.shape[0]
@@ -168,9 +188,22 @@ def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
     l.strike = float(strikes[b if strikes.shape[0] > 1 else 0])
     l.num_calibration = int(num_calib or 0)
     l.rcond = 10 * k * np.finfo(np.float64).eps
-    dfb = np.ascontiguousarray(
-        np.broadcast_to(df, df.shape[:-1] + (n_batch,))[..., b].reshape(
-            df.shape[0], -1)[:, 0], np.float64)
+    keep = []
+    if table is not None:
+      tb = table[:, b if table.shape[1] > 1 else 0, :].T.contiguous()
+      keep.append(tb)
+      l.payoff_table = tb.data_ptr()
+    if per_path:
+      pdb = torch.as_tensor(
+          np.ascontiguousarray(df[:, :, b if df.shape[2] > 1 else 0]),
+          dtype=torch.float64, device=dev).contiguous()  # [n_ex + 1, N]
+      keep.append(pdb)
+      l.path_discounts = pdb.data_ptr()
+      dfb = np.ones(df.shape[0])
+    else:
+      dfb = np.ascontiguousarray(
+          np.broadcast_to(df, df.shape[:-1] + (n_batch,))[..., b].reshape(
+              df.shape[0], -1)[:, 0], np.float64)
     res = torch.empty(2, dtype=torch.float64, device=dev)
     ranks = torch.zeros(max(len(ex) - 1, 1), dtype=torch.int32, device=dev)
     _native.call('tqf_lsm_price', ctypes.byref(l), exi,
