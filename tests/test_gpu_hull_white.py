@@ -193,3 +193,29 @@ def test_bermudan_pipeline_matches_oracle_on_same_paths():
   # the kernels solve the standardised normal equations, the oracle uses
   # pinv: the same exercise decisions up to rounding of the regression
   np.testing.assert_allclose(got, want, rtol=2e-6)
+
+
+def test_time_dependent_mean_reversion_takes_the_euler_route():
+  # hull_white_test.py:450-463: a mean reversion with jumps is sampled with
+  # Euler (drift f'(0,t) + a f(0,t) + sigma^2 (1 - e^{-2at}) / (2a) - a r).
+  # With equal values on both sides of the jump the law is the constant-a
+  # model's: moments at t = 1 within Monte Carlo and Euler error.
+  mr = PiecewiseConstantFunc([0.5], [0.1, 0.1])
+  model = tqf.models.hull_white.HullWhiteModel1F(mr, 0.01, _flat,
+                                                 dtype=torch.float64)
+  assert model._sample_with_generic
+  with pytest.raises(ValueError):
+    model.sample_paths([1.0], num_samples=10)
+  r = _np(model.sample_paths([1.0], num_samples=200000, time_step=0.01,
+                             random_type=RandomType.STATELESS_ANTITHETIC,
+                             seed=[4, 2]))[:, 0, 0]
+  a, s = 0.1, 0.01
+  mean = 0.01 + s * s / 2 / a**2 * (1 - np.exp(-a))**2
+  var = s * s / 2 / a * (1 - np.exp(-2 * a))
+  assert abs(r.mean() - mean) < 4 * np.sqrt(var / r.size)
+  np.testing.assert_allclose(r.var(), var, rtol=2e-2)
+  # the drift at a point against the formula
+  x = torch.full((3, 1), 0.02, dtype=torch.float64)
+  d = _np(model._drift_fn(torch.tensor(0.7, dtype=torch.float64), x))
+  want = 0.0 + a * 0.01 + s * s / 2 / a * (1 - np.exp(-2 * a * 0.7)) - a * 0.02
+  np.testing.assert_allclose(d, want, rtol=1e-12)

@@ -122,23 +122,38 @@ lsm_prepare_kernel(LsmArgs a, double df_ratio, const double* __restrict__ cash,
   }
 }
 
-// mean[d] and scale[d] = 1 / std (1 when the spread is zero), fixed order.
-__global__ void lsm_mean_kernel(const double* __restrict__ block_sums,
-                                int64_t n_blocks, int d, int64_t n_paths,
-                                double* __restrict__ mean,
-                                double* __restrict__ scale) {
-  const int j = threadIdx.x;
-  if (j >= d) return;
-  double s = 0, q = 0;
-  for (int64_t b = 0; b < n_blocks; ++b) {
-    s += block_sums[b * 2 * kMaxAssets + j];
-    q += block_sums[b * 2 * kMaxAssets + kMaxAssets + j];
+// mean[d] and scale[d] = 1 / std (1 when the spread is zero).  One
+// workgroup: thread t adds blocks t, t + 256, ... in order, then a fixed
+// LDS tree (deterministic).
+__global__ void __launch_bounds__(256)
+lsm_mean_kernel(const double* __restrict__ block_sums, int64_t n_blocks,
+                int d, int64_t n_paths, double* __restrict__ mean,
+                double* __restrict__ scale) {
+  __shared__ double red[2 * kMaxAssets][256];
+  const int t = threadIdx.x;
+  double acc[2 * kMaxAssets];
+#pragma unroll
+  for (int j = 0; j < 2 * kMaxAssets; ++j) acc[j] = 0;
+  for (int64_t b = t; b < n_blocks; b += 256) {
+    const double* row = block_sums + b * 2 * kMaxAssets;
+#pragma unroll
+    for (int j = 0; j < 2 * kMaxAssets; ++j) acc[j] += row[j];
   }
+#pragma unroll
+  for (int j = 0; j < 2 * kMaxAssets; ++j) red[j][t] = acc[j];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int j = 0; j < 2 * kMaxAssets; ++j) red[j][t] += red[j][t + w];
+    __syncthreads();
+  }
+  if (t >= d) return;
   const double n = (double)n_paths;
-  const double m = s / n;
-  const double var = q / n - m * m;
-  mean[j] = m;
-  scale[j] = var > 1e-300 && sqrt(var) > 1e-12 * fabs(m) ? 1.0 / sqrt(var) : 1.0;
+  const double m = red[t][0] / n;
+  const double var = red[kMaxAssets + t][0] / n - m * m;
+  mean[t] = m;
+  scale[t] = var > 1e-300 && sqrt(var) > 1e-12 * fabs(m) ? 1.0 / sqrt(var) : 1.0;
 }
 
 // Basis column k of one path from its per-asset power table.
@@ -271,7 +286,6 @@ lsm_solve_kernel(double* __restrict__ g, int K, double rtol,
   const int kext = K + 1;
   double* dsc = work;          // [K] column scales
   double* z = work + K;        // [K] forward-substitution result
-  __shared__ double piv;
   __shared__ double maxdiag;
   __shared__ int rank;
   const int t = threadIdx.x, nt = blockDim.x;
@@ -280,12 +294,13 @@ lsm_solve_kernel(double* __restrict__ g, int K, double rtol,
     dsc[i] = d > 0 ? 1.0 / sqrt(d) : 0.0;
   }
   __syncthreads();
-  // scale G and the rhs column
-  for (int64_t e = t; e < (int64_t)K * K; e += nt) {
-    const int r = (int)(e / K), c = (int)(e % K);
+  // scale G and the rhs column (K <= 255: 32-bit index math)
+  for (int e = t; e < K * K; e += nt) {
+    const int r = e / K, c = e % K;
     g[(int64_t)r * kext + c] *= dsc[r] * dsc[c];
   }
   for (int r = t; r < K; r += nt) g[(int64_t)r * kext + K] *= dsc[r];
+  __syncthreads();  // the diagonal must be scaled before its maximum
   if (t == 0) {
     double m = 0;
     for (int i = 0; i < K; ++i) m = fmax(m, g[(int64_t)i * kext + i]);
@@ -293,32 +308,40 @@ lsm_solve_kernel(double* __restrict__ g, int K, double rtol,
     rank = 0;
   }
   __syncthreads();
-  // right-looking Cholesky on the lower triangle
-  for (int k = 0; k < K; ++k) {
-    if (t == 0) {
-      const double d = g[(int64_t)k * kext + k];
-      if (d > rtol * maxdiag && d > 0) {
-        piv = sqrt(d);
-        ++rank;
-      } else {
-        piv = 0;  // truncated: drop this column
+  // right-looking Cholesky on the lower triangle, one barrier per column:
+  // in the phase of column k every thread reads the pivot d = G[k][k]
+  // itself, subtracts G[i][k] G[j][k] / d from the trailing lower triangle
+  // (column k is only read there) and finishes column k - 1 (divides it by
+  // its pivot's root), whose last reads were in the previous phase.
+  double prev_root = 0;
+  for (int k = 0; k <= K; ++k) {
+    if (k > 0) {
+      // finish column k - 1
+      const int c = k - 1;
+      for (int i = c + 1 + t; i < K; i += nt)
+        g[(int64_t)i * kext + c] =
+            prev_root > 0 ? g[(int64_t)i * kext + c] / prev_root : 0.0;
+      if (t == 0) g[(int64_t)c * kext + c] = prev_root;
+    }
+    if (k == K) break;
+    const double d = g[(int64_t)k * kext + k];
+    const bool keep = d > rtol * maxdiag && d > 0;
+    if (keep) {
+      const double inv = 1.0 / d;
+      const int m = K - k - 1;
+      // m <= 255, so the flat index fits 32-bit division
+      for (int e = t; e < m * m; e += nt) {
+        const int i = k + 1 + e / m, j = k + 1 + e % m;
+        if (j <= i)
+          g[(int64_t)i * kext + j] -= g[(int64_t)i * kext + k] *
+                                      g[(int64_t)j * kext + k] * inv;
       }
-      g[(int64_t)k * kext + k] = piv;
+      if (t == 0) ++rank;
     }
-    __syncthreads();
-    const double pk = piv;
-    for (int i = k + 1 + t; i < K; i += nt)
-      g[(int64_t)i * kext + k] = pk > 0 ? g[(int64_t)i * kext + k] / pk : 0.0;
-    __syncthreads();
-    const int m = K - k - 1;
-    for (int64_t e = t; e < (int64_t)m * m; e += nt) {
-      const int i = k + 1 + (int)(e / m), j = k + 1 + (int)(e % m);
-      if (j <= i)
-        g[(int64_t)i * kext + j] -= g[(int64_t)i * kext + k] *
-                                    g[(int64_t)j * kext + k];
-    }
+    prev_root = keep ? sqrt(d) : 0.0;  // truncated: the column is dropped
     __syncthreads();
   }
+  __syncthreads();
   // L z = b and L^T w = z, column-oriented so every step is parallel;
   // beta = D w.  z overwrites the rhs column, w goes to beta.
   for (int j = 0; j < K; ++j) {
@@ -422,18 +445,32 @@ lsm_final_kernel(const double* __restrict__ cash, const double* __restrict__ val
   }
 }
 
-__global__ void lsm_final_reduce_kernel(const double* __restrict__ block_sums,
-                                        int64_t n_blocks, int64_t count,
-                                        double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(256)
+lsm_final_reduce_kernel(const double* __restrict__ block_sums,
+                        int64_t n_blocks, int64_t count,
+                        double* __restrict__ out) {
+  // thread t adds blocks t, t + 256, ... in order; fixed LDS tree
+  __shared__ double s1[256], s2[256];
+  const int t = threadIdx.x;
   double a = 0, b = 0;
-  for (int64_t i = 0; i < n_blocks; ++i) {
+  for (int64_t i = t; i < n_blocks; i += 256) {
     a += block_sums[2 * i];
     b += block_sums[2 * i + 1];
   }
+  s1[t] = a;
+  s2[t] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      s1[t] += s1[t + w];
+      s2[t] += s2[t + w];
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
   const double n = (double)count;
-  const double mean = a / n;
-  double var = n > 1 ? (b - n * mean * mean) / (n - 1) : 0.0;
+  const double mean = s1[0] / n;
+  double var = n > 1 ? (s2[0] - n * mean * mean) / (n - 1) : 0.0;
   out[0] = mean;
   out[1] = sqrt((var > 0 ? var : 0.0) / n);
 }
@@ -614,7 +651,7 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
     const double ratio = df[e + 1] / df[e];
     hipLaunchKernelGGL(lsm_prepare_kernel, blocks_for(n), 256, 0, s, a, ratio,
                        cash, vals, exv, y, bsum);
-    hipLaunchKernelGGL(lsm_mean_kernel, 1, 64, 0, s, bsum, nb, a.d, n, mean,
+    hipLaunchKernelGGL(lsm_mean_kernel, 1, 256, 0, s, bsum, nb, a.d, n, mean,
                        scale);
     hipLaunchKernelGGL(lsm_gram_kernel, dim3(n_chunks, n_pairs), 256, 0, s, a,
                        exv, y, n_calib, chunk, pairs, partial);
@@ -641,7 +678,7 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
   hipLaunchKernelGGL(lsm_final_kernel, blocks_for(count), 256, 0, s, cash,
                      vals, df[1] / df[0], pd ? pd + n : nullptr, pd, first,
                      n, fsum);
-  hipLaunchKernelGGL(lsm_final_reduce_kernel, 1, 64, 0, s, fsum, fb, count,
+  hipLaunchKernelGGL(lsm_final_reduce_kernel, 1, 256, 0, s, fsum, fb, count,
                      result);
   return check_launch("tqf_lsm_price finalize");
 }
@@ -684,7 +721,7 @@ extern "C" int tqf_lsm_date_inputs(const tqf_lsm* l, int32_t t_idx,
   LsmArgs a = make_args(l, t_idx);
   hipLaunchKernelGGL(lsm_prepare_kernel, blocks_for(n), 256, 0, s, a, df_ratio,
                      cash, vals, exv, y, bsum);
-  hipLaunchKernelGGL(lsm_mean_kernel, 1, 64, 0, s, bsum, nb, a.d, n, mean,
+  hipLaunchKernelGGL(lsm_mean_kernel, 1, 256, 0, s, bsum, nb, a.d, n, mean,
                      scale);
   a.mean = mean;
   a.scale = scale;
@@ -721,7 +758,7 @@ extern "C" int tqf_lsm_final(const double* cash, const double* vals,
   hipLaunchKernelGGL(lsm_final_kernel, blocks_for(count), 256, 0, s, cash,
                      vals, df0, (const double*)nullptr, (const double*)nullptr,
                      first, n, workspace);
-  hipLaunchKernelGGL(lsm_final_reduce_kernel, 1, 64, 0, s, workspace, fb,
+  hipLaunchKernelGGL(lsm_final_reduce_kernel, 1, 256, 0, s, workspace, fb,
                      count, result);
   return check_launch("tqf_lsm_final");
 }

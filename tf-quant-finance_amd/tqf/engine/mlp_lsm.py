@@ -46,8 +46,12 @@ class MlpModel:
     self.v = torch.zeros_like(self.params)
     self.step = 0
     if n_blocks is None:
+      # one workgroup per CU: the gradient kernel runs one wave per SIMD
+      # (its registers allow no more), so a second round of workgroups
+      # only repeats the per-workgroup set-up and reduction (measured
+      # 0.079 ms per 65536-sample step against 0.120 ms with two per CU)
       n_blocks = torch.cuda.get_device_properties(
-          self.dev).multi_processor_count * 2
+          self.dev).multi_processor_count
     self.n_blocks = int(n_blocks)
     ws = _native.lib().tqf_mlp_workspace_size(self.n_blocks)
     self._ws = torch.empty(ws // 4 + 1, dtype=torch.float32, device=self.dev)
