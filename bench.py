@@ -257,7 +257,8 @@ def secondary_metrics():
                        'achieved_tflops': plan.total_flop / t / 1e12,
                        'frac_fp32_peak': plan.total_flop / t / PEAK_FP32,
                        'autotuned': cfg,
-                       'candidates_s': {'%d/%d' % (r['tile'], r['n_blocks']):
+                       'candidates_s': {'%s/%d/%d' % (r['greeks'], r['tile'],
+                                                      r['n_blocks']):
                                         round(r['seconds'], 4) for r in tune}}
   # the same contracts through the plain fused kernel, one call per contract
   # (price only, and price + central-difference Greeks with common random

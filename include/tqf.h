@@ -367,11 +367,14 @@ int tqf_mlp_train_step(float* params, float* adam_m, float* adam_v,
 #define TQF_CONTRACT_HESTON 1  /* full-truncation Heston: delta, d/dv0, rho,
                                   d/dkappa, d/dtheta, d/dxi, d/drho          */
 
+#define TQF_GREEKS_ADJOINT 0  /* Heston: reverse sweep over checkpoints     */
+#define TQF_GREEKS_TANGENT 1  /* Heston: five tangents in the forward pass  */
+
 typedef struct tqf_contract {
   int32_t kind;
   int32_t n_steps;      /* uniform grid dt = maturity / n_steps             */
   uint32_t stream_id;   /* Philox sub-stream of the contract's paths        */
-  int32_t pad0;
+  int32_t greeks_mode;  /* TQF_GREEKS_* (Heston contracts; same estimator)  */
   int64_t n_paths;
   double s0, strike, maturity, rate;
   double sigma;         /* GBM                                              */
@@ -392,8 +395,8 @@ int64_t tqf_batch_workspace_size(int32_t n_items, int32_t n_blocks,
  * persistent launch over n_items work items (device arrays; the items of a
  * contract are contiguous rows; `order` (device [n_items], or NULL for row
  * order) is the processing order, longest item first), with
- * pathwise adjoint Greeks (Heston: checkpoints every 16 steps and a reverse
- * sweep).  Philox stream: key = seed, sub-stream = the contract's
+ * pathwise Greeks (Heston: checkpoints every 16 steps and a reverse sweep,
+ * or forward tangents, per contract).  Philox stream: key = seed, sub-stream = the contract's
  * stream_id, path index = first_path + offset.  out: device [n_contracts][16] doubles:
  * 8 means (price, Greeks) then their 8 standard errors.  The launch records
  * no memset, so it can be captured in a hipGraph.  Replaces sampling with

@@ -58,10 +58,11 @@ def _oracle_heston(c, z, bump=None, h=0.0):
   return pr[-1, 0]
 
 
-def test_batch_heston_price_and_adjoint_vs_oracle_fd():
+@pytest.mark.parametrize('greeks', ['adjoint', 'tangent'])
+def test_batch_heston_price_and_adjoint_vs_oracle_fd(greeks):
   cs = [_heston(0), _heston(1, k=110., xi=0.8, rho=-0.3, steps=45),
         _heston(2, k=90., v0=0.09, kappa=3.0, steps=33)]
-  plan = batch_lib.BatchPlan(cs, tile=1024, seed=5)
+  plan = batch_lib.BatchPlan(cs, tile=1024, seed=5, greeks=greeks)
   plan.launch()
   res = plan.results()
   for c, r in zip(cs, res):
@@ -88,6 +89,23 @@ def test_batch_heston_price_and_adjoint_vs_oracle_fd():
       got, se = r[name]
       scale = max(abs(fd), 1e-2 * abs(price))
       assert abs(got - fd) <= 2e-3 * scale + 0.1 * se, (name, got, fd, se)
+
+
+def test_batch_tangent_greeks_equal_adjoint_greeks():
+  # the forward tangents and the checkpointed reverse sweep compute the
+  # same pathwise estimator on the same draws: equal up to float32
+  # rounding (prices identical)
+  cs = [_heston(0), _heston(1, k=110., xi=0.8, rho=-0.3, steps=45),
+        _heston(2, k=90., v0=0.09, kappa=3.0, steps=33),
+        _heston(3, steps=121, paths=3000), _gbm(4)]
+  adj = batch_lib.BatchPlan(cs, tile=512, seed=3).launch().cpu().numpy()
+  tan = batch_lib.BatchPlan(cs, tile=512, seed=3,
+                            greeks='tangent').launch().cpu().numpy()
+  np.testing.assert_array_equal(tan[:, 0], adj[:, 0])
+  means = slice(0, 8)
+  scale = np.maximum(np.abs(adj[:, means]), 1e-3 * np.abs(adj[:, :1]))
+  assert np.all(np.abs(tan[:, means] - adj[:, means]) <= 2e-4 * scale), (
+      np.max(np.abs(tan[:, means] - adj[:, means]) / scale))
 
 
 def test_batch_equals_loop_of_single_contracts():

@@ -10,12 +10,14 @@
 // squares) to its own row; a second kernel adds the rows of each contract
 // in a fixed order.
 //
-// Greeks are pathwise (reverse mode).  GBM's adjoint does not depend on the
-// path, so it is closed form at maturity.  Heston's does: the forward pass
-// stores the state every kSeg steps (about sqrt(steps)), and the reverse
-// sweep re-runs each segment from its checkpoint with the same Philox
-// counters, keeps the segment's states and normals in registers, and walks
-// it backwards.  Truncation kinks use the subgradient 0 at v = 0.
+// Greeks are pathwise.  GBM's adjoint does not depend on the path, so it is
+// closed form at maturity.  Heston's does: in reverse mode (the default) the
+// forward pass stores the state every kSeg steps (about sqrt(steps)), and
+// the reverse sweep re-runs each segment from its checkpoint with the same
+// Philox counters, keeps the segment's states and normals in registers, and
+// walks it backwards; in tangent mode (a contract's greeks_mode) five
+// tangents ride along the forward pass instead.  Truncation kinks use the
+// subgradient 0 at v = 0.
 //
 // Forward step (euler_sampling.py:513-537 with the truncated Heston
 // coefficients of heston_model.py:143-173, drift r):
@@ -39,7 +41,7 @@ __device__ __forceinline__ void normals4(uint32_t p_lo, uint32_t p_hi,
 }
 
 struct Contract {
-  int kind, n_steps;
+  int kind, n_steps, greeks_mode;
   uint32_t stream;
   float s0, strike, t, r;
   float sigma;                         // GBM
@@ -51,6 +53,7 @@ __device__ __forceinline__ void load_contract(const tqf_contract* __restrict__ c
                                               Contract& k) {
   k.kind = c->kind;
   k.n_steps = c->n_steps;
+  k.greeks_mode = c->greeks_mode;
   k.stream = c->stream_id;
   k.n_paths = c->n_paths;
   k.s0 = (float)c->s0;
@@ -193,6 +196,73 @@ __device__ void heston_path(const Contract& c, int64_t p, uint32_t stream,
   g[7] = rhob;
 }
 
+// One Heston path with forward-mode (tangent) Greeks: the tangents of
+// (x, v) for v0, kappa, theta, xi and rho ride along the forward pass, so
+// there is no checkpoint, no second pass and no second draw of the normals
+// (which dominate a step's cost).  Same pathwise estimator as the reverse
+// sweep above, with the same subgradient 0 at v <= 0: the price depends on
+// x_T only, so each Greek is x-bar ("xb" below) times the x-tangent, and
+// delta and rho are closed form as in the reverse sweep.
+//   v+' = v > 0 ? v' : 0,  sqrt(v+)' = v+' / (2 sqrt(v+))
+//   x'  <- x' - dt/2 v+' + sqrt(v+)' dw0
+//   v'  <- v' - dt kappa v+' + sqrt(v+)' (xi rho dw0 + xi s dw1) + dv'/dp
+__device__ void heston_path_tangent(const Contract& c, int64_t p,
+                                    uint32_t stream, uint32_t k0,
+                                    uint32_t k1, float* g) {
+  const int n = c.n_steps;
+  const float dt = c.t / (float)n;
+  const float sdt = sqrtf(dt);
+  const float s1r = sqrtf(1.f - c.rho * c.rho);
+  const float xr = c.xi * c.rho, xs = c.xi * s1r;
+  const float drho_s = -c.rho / s1r;  // d sqrt(1-rho^2) / d rho
+  const float mdt2 = -0.5f * dt, mdtk = -dt * c.kappa, dtk = dt * c.kappa;
+  const uint32_t plo = (uint32_t)p, phi = (uint32_t)(p >> 32);
+  float x = __logf(c.s0), v = c.v0;
+  // tangents of (x, v) for v0, kappa, theta, xi, rho
+  float tx[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float tv[5] = {1.f, 0.f, 0.f, 0.f, 0.f};
+  float z[4];
+  for (int i = 0; i < n; ++i) {
+    if ((i & 1) == 0)
+      normals4(plo, phi, (uint32_t)(i >> 1), stream, k0, k1, z);
+    const float z0 = (i & 1) ? z[2] : z[0];
+    const float z1 = (i & 1) ? z[3] : z[1];
+    const bool pos = v > 0.f;
+    const float vp = pos ? v : 0.f;
+    const float sq = __builtin_amdgcn_sqrtf(vp);
+    const float dw0 = z0 * sdt, dw1 = z1 * sdt;
+    const float cc = xr * dw0 + xs * dw1;
+    const float hs = sq > 0.f ? 0.5f / sq : 0.f;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const float tvp = pos ? tv[q] : 0.f;
+      const float tsq = tvp * hs;
+      tx[q] = (tx[q] + mdt2 * tvp) + tsq * dw0;
+      tv[q] = (tv[q] + mdtk * tvp) + tsq * cc;
+    }
+    tv[1] += dt * (c.theta - vp);                    // kappa
+    tv[2] += dtk;                                    // theta
+    tv[3] += sq * (c.rho * dw0 + s1r * dw1);         // xi
+    tv[4] += c.xi * sq * (dw0 + drho_s * dw1);       // rho
+    // the state update of the reverse-mode path, so both modes price the
+    // same paths bit for bit
+    heston_step(c, xr, xs, dt, sdt, z0, z1, x, v);
+  }
+  const float s = __expf(x);
+  const float disc = __expf(-c.r * c.t);
+  const float itm = s > c.strike ? 1.f : 0.f;
+  const float pay = disc * itm * (s - c.strike);
+  const float xb = disc * itm * s;
+  g[0] = pay;
+  g[1] = xb / c.s0;             // delta
+  g[2] = xb * tx[0];            // d/dv0
+  g[3] = -c.t * pay + xb * c.t; // rho (rate): x_T carries r T
+  g[4] = xb * tx[1];
+  g[5] = xb * tx[2];
+  g[6] = xb * tx[3];
+  g[7] = xb * tx[4];
+}
+
 // Persistent kernel: items from `counter`, sums per item into `item_sums`
 // [n_items][2 * kStats].
 constexpr int kStats = 8;
@@ -229,6 +299,8 @@ batch_kernel(const tqf_contract* __restrict__ contracts,
       float g[kStats];
       if (c.kind == TQF_CONTRACT_GBM)
         gbm_path(c, p, c.stream, k0, k1, g);
+      else if (c.greeks_mode == TQF_GREEKS_TANGENT)
+        heston_path_tangent(c, p, c.stream, k0, k1, g);
       else
         heston_path(c, p, c.stream, k0, k1, ck, kBlockB, g);
 #pragma unroll

@@ -71,7 +71,7 @@ class Lsm(ctypes.Structure):
 
 class Contract(ctypes.Structure):
   _fields_ = [('kind', _i32), ('n_steps', _i32), ('stream_id', _u32),
-              ('pad0', _i32), ('n_paths', _i64),
+              ('greeks_mode', _i32), ('n_paths', _i64),
               ('s0', _f64), ('strike', _f64), ('maturity', _f64),
               ('rate', _f64), ('sigma', _f64), ('v0', _f64), ('kappa', _f64),
               ('theta', _f64), ('xi', _f64), ('rho', _f64)]

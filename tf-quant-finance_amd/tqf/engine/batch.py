@@ -18,7 +18,8 @@ import torch
 from tqf import _native
 
 __all__ = ['BatchPlan', 'autotune', 'random_contracts', 'GREEK_NAMES',
-           'KIND_GBM', 'KIND_HESTON', 'FLOP_PER_PATH_STEP']
+           'KIND_GBM', 'KIND_HESTON', 'FLOP_PER_PATH_STEP',
+           'FLOP_PER_PATH_STEP_TANGENT']
 
 KIND_GBM, KIND_HESTON = 0, 1
 GREEK_NAMES = {
@@ -31,14 +32,21 @@ GREEK_NAMES = {
 # a Box-Muller normal 5 (10 per pair: 2 scale, log, mul, sqrt, sin, cos,
 # 2 mul, amortised):
 #   GBM     normal 5 + dw 1 + drift add 1 + sigma*dw FMA 2 + vega sum 1 = 10
-#   Heston  forward 30 (2 normals 10; max, sqrt, 2 dw; log-spot 6;
-#           variance 10) + the same 30 again when each checkpoint segment
-#           is recomputed + reverse sweep 39 (max, sqrt, 2 dw 4; sqrt-bar 6;
-#           v-bar 7; rate-bar 2; kappa-bar 4; theta-bar 2; xi-bar 7;
-#           rho-bar 5; v update 2) = 99
+#   Heston, reverse mode: forward 30 (2 normals 10; max, sqrt, 2 dw;
+#           log-spot 6; variance 10) + the same 30 again when each
+#           checkpoint segment is recomputed + reverse sweep 39 (max, sqrt,
+#           2 dw 4; sqrt-bar 6; v-bar 7; rate-bar 2; kappa-bar 4; theta-bar
+#           2; xi-bar 7; rho-bar 5; v update 2) = 99
+#   Heston, tangent mode: forward 26 (2 normals 10; max, sqrt, 2 dw 4;
+#           log-spot 6; variance 6 with the shared noise term) + shared
+#           noise term 3 and 1/(2 sqrt v) 2 + five tangents x 10 (select,
+#           scale, two FMA pairs) + parameter terms 14 (kappa 3, theta 1,
+#           xi 5, rho 5) = 95
 # Per-path payoff and Greek assembly (a few dozen FLOP per path) is left
 # out: at >= 50 steps it is under 1% of the total.
 FLOP_PER_PATH_STEP = {KIND_GBM: 10, KIND_HESTON: 99}
+FLOP_PER_PATH_STEP_TANGENT = {KIND_GBM: 10, KIND_HESTON: 95}
+GREEKS_ADJOINT, GREEKS_TANGENT = 0, 1
 _FIELDS = ['kind', 'n_steps', 'n_paths', 's0', 'strike', 'maturity', 'rate',
            'sigma', 'v0', 'kappa', 'theta', 'xi', 'rho']
 CACHE_PATH = os.environ.get(
@@ -71,19 +79,28 @@ def random_contracts(n=4096, seed=20240607, rate=0.03):
                rho=float(rho[i])) for i in range(n)]
 
 
-def _cost(c, n_paths):
-  # relative work of a path: Heston pays a forward, a recompute and a
-  # reverse sweep with two normals per step
-  return n_paths * c['n_steps'] * (3.2 if c['kind'] == KIND_HESTON else 1.0)
+def _cost(c, n_paths, greeks=GREEKS_ADJOINT):
+  # relative work of a path: Heston pays two normals per step and, in
+  # reverse mode, a recompute and a reverse sweep
+  if c['kind'] != KIND_HESTON:
+    return n_paths * c['n_steps']
+  return n_paths * c['n_steps'] * (3.2 if greeks == GREEKS_ADJOINT else 2.2)
 
 
 class BatchPlan:
   """Device-resident tables for one batch of contracts."""
 
   def __init__(self, contracts, tile=1024, n_blocks=None, use_counter=True,
-               seed=11, dev=None):
+               seed=11, dev=None, greeks='adjoint'):
+    """greeks: 'adjoint' (reverse sweep over sqrt(steps) checkpoints) or
+    'tangent' (forward tangents) for the Heston contracts; the same
+    pathwise estimator either way."""
     self.dev = dev or _native.device()
     self.contracts = list(contracts)
+    if greeks not in ('adjoint', 'tangent'):
+      raise ValueError("greeks must be 'adjoint' or 'tangent'")
+    self.greeks = greeks
+    mode = GREEKS_TANGENT if greeks == 'tangent' else GREEKS_ADJOINT
     self.tile = int(tile)
     self.seed = int(seed)
     self.use_counter = bool(use_counter)
@@ -97,6 +114,7 @@ class BatchPlan:
       for f in _FIELDS:
         setattr(table[i], f, c[f])
       table[i].stream_id = int(c.get('stream_id', i))
+      table[i].greeks_mode = mode
     items, first, count, costs = [], [], [], []
     for i, c in enumerate(self.contracts):
       first.append(len(items))
@@ -104,7 +122,7 @@ class BatchPlan:
       for p0 in range(0, c['n_paths'], self.tile):
         m = min(self.tile, c['n_paths'] - p0)
         items.append((i, p0, m))
-        costs.append(_cost(c, m))
+        costs.append(_cost(c, m, mode))
         k += 1
       count.append(k)
     it = (_native.WorkItem * len(items))()
@@ -128,8 +146,9 @@ class BatchPlan:
     self.out = torch.empty((n, 16), dtype=torch.float64, device=self.dev)
     self.total_path_steps = int(sum(c['n_paths'] * c['n_steps']
                                     for c in self.contracts))
-    self.total_flop = int(sum(c['n_paths'] * c['n_steps'] *
-                              FLOP_PER_PATH_STEP[c['kind']]
+    flop = (FLOP_PER_PATH_STEP_TANGENT if mode == GREEKS_TANGENT else
+            FLOP_PER_PATH_STEP)
+    self.total_flop = int(sum(c['n_paths'] * c['n_steps'] * flop[c['kind']]
                               for c in self.contracts))
 
   def launch(self):
@@ -185,8 +204,10 @@ def _time(plan, reps):
 
 
 def autotune(contracts, tiles=(512, 1024, 2048, 4096), blocks_per_cu=(4, 8),
-             reps=2, seed=11, cache_path=CACHE_PATH, key=None):
-  """Times (tile, grid) candidates on this batch and caches the best.
+             greeks=('adjoint', 'tangent'), reps=2, seed=11,
+             cache_path=CACHE_PATH, key=None):
+  """Times (tile, grid, Greek mode) candidates on this batch and caches the
+  best.
 
   Returns (best_config dict, timings list).  The cache is keyed by the
   device name and `key` (default: a signature of the contract table).
@@ -208,14 +229,17 @@ def autotune(contracts, tiles=(512, 1024, 2048, 4096), blocks_per_cu=(4, 8),
   if full_key in cache:
     return cache[full_key], None
   timings = []
-  for tile in tiles:
-    for bpc in blocks_per_cu:
-      plan = BatchPlan(contracts, tile=tile, n_blocks=cus * bpc, seed=seed,
-                       dev=dev)
-      t = _time(plan, reps)
-      timings.append({'tile': tile, 'n_blocks': cus * bpc, 'seconds': t})
+  for mode in greeks:
+    for tile in tiles:
+      for bpc in blocks_per_cu:
+        plan = BatchPlan(contracts, tile=tile, n_blocks=cus * bpc, seed=seed,
+                         dev=dev, greeks=mode)
+        t = _time(plan, reps)
+        timings.append({'tile': tile, 'n_blocks': cus * bpc, 'greeks': mode,
+                        'seconds': t})
   best = min(timings, key=lambda r: r['seconds'])
-  cfg = {'tile': best['tile'], 'n_blocks': best['n_blocks']}
+  cfg = {'tile': best['tile'], 'n_blocks': best['n_blocks'],
+         'greeks': best['greeks']}
   if cache_path:
     try:
       os.makedirs(os.path.dirname(cache_path), exist_ok=True)
