@@ -285,6 +285,37 @@ def secondary_metrics():
       'gain': res['gain'], 'gain_se': res['gain_se'],
       'accepted': res['accepted']}
   del paths
+  # Hull-White Bermudan swaptions (5nc1 and 5nc2, semiannual, a = 3%,
+  # sigma = 1%, flat 1% curve), 2^20 paths: short rate in the fused kernel,
+  # regression in the native LSM with per-path discount factors
+  from tqf.models.hull_white import bermudan_swaption_price
+  from tqf.math.random.multivariate_normal import RandomType
+  base = [1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0, 4.5, 5.0]
+  st = np.array([[[base[min(i + j, 8)] for j in range(8)] for i in range(8)],
+                 [[base[min(i + j, 8)] for j in range(8)]
+                  for i in range(2, 10)]])
+  end = np.clip(st + 0.5, 0.0, 5.0)
+  ex = np.array([base[:8], base[2:] + [5.0]])
+
+  def berm():
+    return bermudan_swaption_price(
+        exercise_times=ex, floating_leg_start_times=st,
+        floating_leg_end_times=end, fixed_leg_payment_times=end,
+        floating_leg_daycount_fractions=end - st,
+        fixed_leg_daycount_fractions=end - st,
+        fixed_leg_coupon=0.011 * np.ones_like(end),
+        reference_rate_fn=lambda t: 0.01 * torch.ones_like(
+            torch.as_tensor(t)), notional=100.0, mean_reversion=0.03,
+        volatility=0.01, num_samples=2**20, time_step=0.1,
+        random_type=RandomType.STATELESS_ANTITHETIC, seed=[0, 0],
+        dtype=torch.float64)
+  berm()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  prices = berm().cpu().numpy()
+  t = time.perf_counter() - t0
+  out['hw_bermudan_2x2e20'] = {'seconds': t, 'prices': prices.tolist(),
+                               'reference_values': [1.8892, 1.6633]}
   # 16-asset basket with adjoint Greeks
   d = 16
   rs = np.random.RandomState(3)
