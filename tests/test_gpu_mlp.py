@@ -79,6 +79,20 @@ def test_mlp_gradient_matches_oracle():
   assert np.max(np.abs(g2.cpu().numpy() - go2)) <= 2e-4 * np.max(np.abs(go2))
 
 
+def test_mlp_gradient_is_deterministic():
+  # the block reduction has a fixed order: repeated calls agree bit for bit
+  n = 70000
+  x, t, w = _data(n, 4)
+  m = mlp_lsm.MlpModel(params=_params(5))
+  dev = m.dev
+  args = [torch.as_tensor(a, device=dev) for a in (x, t, w)]
+  g1, l1 = m.grad(*args, inv_norm=1.0 / n)
+  g1, l1 = g1.clone(), l1.clone()
+  for _ in range(3):
+    g2, l2 = m.grad(*args, inv_norm=1.0 / n)
+    assert torch.equal(g1, g2) and torch.equal(l1, l2)
+
+
 def test_mlp_adam_steps_match_oracle_and_training_converges():
   n = 8192
   x, t, w = _data(n, 4)

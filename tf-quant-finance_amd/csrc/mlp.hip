@@ -151,12 +151,36 @@ __device__ __forceinline__ void load_w2t(const float* __restrict__ P,
     }
 }
 
+// Copies the parameter vector into LDS (and clears the gradient row when
+// G is given) with every global load of a thread issued before the first
+// LDS store: a load-store loop would wait out the memory latency once per
+// iteration (about 19 times per thread).
+constexpr int kStageIters = (kMlpParams + 255) / 256;
+
+__device__ __forceinline__ void stage_params(const float* __restrict__ params,
+                                             float* P, float* G) {
+  float v[kStageIters];
+#pragma unroll
+  for (int k = 0; k < kStageIters; ++k) {
+    const int e = threadIdx.x + k * 256;
+    v[k] = e < kMlpParams ? params[e] : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < kStageIters; ++k) {
+    const int e = threadIdx.x + k * 256;
+    if (e < kMlpParams) {
+      P[e] = v[k];
+      if (G) G[e] = 0.f;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256)
 mlp_forward_kernel(const float* __restrict__ params,
                    const float* __restrict__ x, int64_t n,
                    float* __restrict__ out) {
   __shared__ float P[kMlpParams];
-  for (int e = threadIdx.x; e < kMlpParams; e += 256) P[e] = params[e];
+  stage_params(params, P, nullptr);
   __syncthreads();
   bf16x4 w2b[16];
   load_w2(P, w2b);
@@ -190,13 +214,9 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
                 int64_t first, int64_t n, float inv_norm,
                 float* __restrict__ partial) {
   __shared__ float P[kMlpParams];
-  __shared__ float G[kMlpParams + 1];
+  __shared__ float Wsum[4][kMlpParams + 1];  // per-wave gradient rows
   __shared__ short tr[4][16 * 64];  // per-wave transpose buffer (bf16)
-  for (int e = threadIdx.x; e < kMlpParams; e += 256) {
-    P[e] = params[e];
-    G[e] = 0.f;
-  }
-  if (threadIdx.x == 0) G[kMlpParams] = 0.f;
+  stage_params(params, P, nullptr);
   __syncthreads();
   bf16x4 w2b[16], w2t[16];
   load_w2(P, w2b);
@@ -322,31 +342,48 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
       }
     }
   }
-  // block reduction into LDS (atomics on LDS floats), then one row out
+  // block reduction in a fixed order: each wave adds its four lane groups
+  // (q = lane / 16, which hold the same features) with shuffles and writes
+  // one row of LDS; the block adds the four rows in wave order.  (Every W2
+  // entry has exactly one owning lane per wave.)
+  float* Wr = Wsum[wave];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        atomicAdd(&G[oW2 + (mt * 16 + 4 * q + i) * kH + t * 16 + r],
-                  dw2[mt * 4 + t][i]);
+        Wr[oW2 + (mt * 16 + 4 * q + i) * kH + t * 16 + r] = dw2[mt * 4 + t][i];
+  auto qsum = [](float v) {
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+  };
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int fo = t * 16 + r;
-    atomicAdd(&G[oB1 + fo], db1[t]);
-    atomicAdd(&G[oB2 + fo], db2[t]);
-    atomicAdd(&G[oW3 + fo], dw3[t]);
+    const float sb1 = qsum(db1[t]), sb2 = qsum(db2[t]), sw3 = qsum(dw3[t]);
+    float sw1[kIn];
 #pragma unroll
-    for (int k = 0; k < kIn; ++k) atomicAdd(&G[oW1 + k * kH + fo], dw1[k][t]);
+    for (int k = 0; k < kIn; ++k) sw1[k] = qsum(dw1[k][t]);
+    if (q == 0) {
+      Wr[oB1 + fo] = sb1;
+      Wr[oB2 + fo] = sb2;
+      Wr[oW3 + fo] = sw3;
+#pragma unroll
+      for (int k = 0; k < kIn; ++k) Wr[oW1 + k * kH + fo] = sw1[k];
+    }
   }
-  if (r == 0) {
-    atomicAdd(&G[oB3], db3);
-    atomicAdd(&G[kMlpParams], loss);
+  // db3 and the loss live on the lanes with r == 0
+  const float sb3 = qsum(db3), sl = qsum(loss);
+  if (lane == 0) {
+    Wr[oB3] = sb3;
+    Wr[kMlpParams] = sl;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < kMlpParams + 1; e += 256)
-    partial[(int64_t)blockIdx.x * (kMlpParams + 1) + e] = G[e];
+    partial[(int64_t)blockIdx.x * (kMlpParams + 1) + e] =
+        ((Wsum[0][e] + Wsum[1][e]) + Wsum[2][e]) + Wsum[3][e];
 }
 
 // Level 1 of the gradient reduction: workgroup (x, c) adds the per-block
@@ -362,6 +399,9 @@ mlp_partial_reduce_kernel(const float* __restrict__ partial, int n_blocks,
   const int b0 = blockIdx.y * per;
   const int b1 = b0 + per < n_blocks ? b0 + per : n_blocks;
   float g = 0.f;
+  // unrolled so the loads of several rows are in flight at once (the sum
+  // keeps its order)
+#pragma unroll 8
   for (int b = b0; b < b1; ++b) g += partial[(int64_t)b * (kMlpParams + 1) + e];
   chunks[(int64_t)blockIdx.y * (kMlpParams + 1) + e] = g;
 }
@@ -376,6 +416,7 @@ mlp_adam_kernel(float* __restrict__ params, float* __restrict__ m,
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e > kMlpParams) return;
   float g = 0.f;
+#pragma unroll 8
   for (int b = 0; b < n_blocks; ++b) g += partial[(int64_t)b * (kMlpParams + 1) + e];
   if (e == kMlpParams) {
     if (loss_out) *loss_out = g;
