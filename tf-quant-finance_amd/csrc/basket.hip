@@ -116,6 +116,8 @@ __global__ void basket_finalize_kernel(const double* __restrict__ partial,
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= ns) return;
   double s1 = 0, s2 = 0;
+  // unrolled: several blocks' loads in flight, the sums keep their order
+#pragma unroll 8
   for (int b = 0; b < n_blocks; ++b) {
     s1 += partial[(int64_t)b * 2 * ns + q];
     s2 += partial[(int64_t)b * 2 * ns + ns + q];

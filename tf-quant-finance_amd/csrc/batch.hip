@@ -341,6 +341,7 @@ batch_finalize_kernel(const double* __restrict__ item_sums,
   double s[2 * kStats];
   for (int q = 0; q < 2 * kStats; ++q) s[q] = 0;
   const int f = contract_first_item[c], m = contract_n_items[c];
+#pragma unroll 4
   for (int i = 0; i < m; ++i)
     for (int q = 0; q < 2 * kStats; ++q) s[q] += item_sums[(int64_t)(f + i) * 2 * kStats + q];
   const double n = (double)contracts[c].n_paths;
