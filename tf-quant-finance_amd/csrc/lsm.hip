@@ -32,6 +32,8 @@ constexpr int kMaxPw = 64;      // (degree + 1) * dim power-table entries
 constexpr int kTile = 64;       // Gram output tile (kTile x kTile)
 constexpr int kSubPaths = 32;   // paths staged per LDS round
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 struct LsmArgs {
   const double* paths;      // [n_paths][n_times][d] (path-major) float64
   int64_t n_paths;
@@ -177,18 +179,20 @@ lsm_gram_kernel(LsmArgs a, const double* __restrict__ exv,
   __shared__ double wrow[kSubPaths];
   __shared__ double yrow[kSubPaths];
   __shared__ uint8_t expo[256 * kMaxAssets];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  // MFMA operands: wave w owns rows w*16 .. w*16 + 15 of the 64 x 64 tile
+  // and all four 16-column subtiles; per step of 4 paths, lane l holds
+  // A = X^T[w*16 + l%16][path l/16] and B = X[path l/16][t*16 + l%16]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
   const int2 pr = pairs[blockIdx.y];
   const int i0 = pr.x * kTile, j0 = pr.y * kTile;
   const int deg1 = a.deg + 1;
   const int kext = a.K + 1;
   for (int e = threadIdx.x; e < a.K * a.d; e += 256) expo[e] = a.expo[e];
 
-  double acc[4][4];
+  f64x4 acc[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[r][c] = 0;
+  for (int t = 0; t < 4; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
 
   const int64_t p_begin = (int64_t)blockIdx.x * chunk;
   int64_t p_end = p_begin + chunk;
@@ -233,27 +237,28 @@ lsm_gram_kernel(LsmArgs a, const double* __restrict__ exv,
       xj[s][c] = vj;
     }
     __syncthreads();
-#pragma unroll 4
-    for (int s = 0; s < kSubPaths; ++s) {
-      double ai[4], bj[4];
+    // G tile += X_i^T X_j over the 32 staged paths, 4 paths per MFMA
+    // (v_mfma_f64_16x16x4_f64, float64 products and sums)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) ai[r] = xi[s][ty * 4 + r];
+    for (int k0 = 0; k0 < kSubPaths; k0 += 4) {
+      const double av = xi[k0 + lk][wave * 16 + lr];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) bj[c] = xj[s][tx * 4 + c];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = fma(ai[r], bj[c], acc[r][c]);
+      for (int t = 0; t < 4; ++t) {
+        const double bv = xj[k0 + lk][t * 16 + lr];
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+      }
     }
   }
   // partial[(chunk, pair)][64][64]
   double* out = partial + ((int64_t)blockIdx.x * gridDim.y + blockIdx.y) *
                               (kTile * kTile);
+  // accumulator layout of the float64 16 x 16 block: lane l, entry r holds
+  // row l/16 + 4*r, column l%16
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      out[(ty * 4 + r) * kTile + tx * 4 + c] = acc[r][c];
+    for (int r = 0; r < 4; ++r)
+      out[(wave * 16 + lk + 4 * r) * kTile + t * 16 + lr] = acc[t][r];
   (void)kext;
 }
 
@@ -267,6 +272,7 @@ lsm_gram_reduce_kernel(const double* __restrict__ partial, int n_chunks,
   const int pair = blockIdx.y;
   if (e >= kTile * kTile) return;
   double s = 0;
+#pragma unroll 8
   for (int c = 0; c < n_chunks; ++c)
     s += partial[((int64_t)c * n_pairs + pair) * (kTile * kTile) + e];
   const int2 pr = pairs[pair];
@@ -369,31 +375,40 @@ lsm_solve_kernel(double* __restrict__ g, int K, double rtol,
   if (t == 0) *rank_out = rank;
 }
 
-// continuation = relu(X beta); exercise iff ex > continuation.
+// continuation = relu(X beta); exercise iff ex > continuation.  The
+// per-path power table lives in dynamic LDS, [d * (deg + 1)][blockDim.x]
+// (consecutive threads on consecutive words): as a private array indexed by
+// the exponents it would sit in scratch memory.
 __global__ void __launch_bounds__(256)
 lsm_update_kernel(LsmArgs a, double df_ratio, const double* __restrict__ exv,
                   const double* __restrict__ beta, double* __restrict__ cash,
                   double* __restrict__ vals) {
   __shared__ double sb[256];
   __shared__ uint8_t expo[256 * kMaxAssets];
-  for (int k = threadIdx.x; k < a.K; k += 256) sb[k] = beta[k];
-  for (int e = threadIdx.x; e < a.K * a.d; e += 256) expo[e] = a.expo[e];
+  extern __shared__ double pws[];
+  const int nt = blockDim.x, tid = threadIdx.x;
+  for (int k = tid; k < a.K; k += nt) sb[k] = beta[k];
+  for (int e = tid; e < a.K * a.d; e += nt) expo[e] = a.expo[e];
   __syncthreads();
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t p = (int64_t)blockIdx.x * nt + tid;
   if (p >= a.n_paths) return;
   const int deg1 = a.deg + 1;
-  double pw[kMaxPw];
   const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
   for (int j = 0; j < a.d; ++j) {
     const double c = (basis_input(a, row[j]) - a.mean[j]) * a.scale[j];
     double v = 1.0;
     for (int e = 0; e <= a.deg; ++e) {
-      pw[j * deg1 + e] = v;
+      pws[(j * deg1 + e) * nt + tid] = v;
       v *= c;
     }
   }
   double cont = 0;
-  for (int k = 0; k < a.K; ++k) cont += basis_col(pw, deg1, expo + k * a.d, a.d) * sb[k];
+  for (int k = 0; k < a.K; ++k) {
+    const uint8_t* e = expo + k * a.d;
+    double b = 1.0;
+    for (int j = 0; j < a.d; ++j) b *= pws[(j * deg1 + e[j]) * nt + tid];
+    cont += b * sb[k];
+  }
   cont = cont > 0 ? cont : 0;
   const double ex = exv[p];
   if (ex > cont) {
@@ -403,6 +418,12 @@ lsm_update_kernel(LsmArgs a, double df_ratio, const double* __restrict__ exv,
     vals[p] = disc_ratio(a, df_ratio, p) * (cash[p] + vals[p]);
     cash[p] = 0;
   }
+}
+
+// Threads per workgroup of lsm_update_kernel: 256, or 64 when the power
+// table would take more than 48 KB of LDS at 256.
+inline int update_threads(const LsmArgs& a) {
+  return a.d * (a.deg + 1) <= 24 ? 256 : 64;
 }
 
 // Terminal exercise values as the initial cashflow.
@@ -665,8 +686,12 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
                           hipMemcpyDeviceToDevice, s);
       if (he != hipSuccess) return set_error((int)he, "copy rank");
     }
-    hipLaunchKernelGGL(lsm_update_kernel, blocks_for(n), 256, 0, s, a, ratio,
-                       exv, beta, cash, vals);
+    {
+      const int ut = update_threads(a);
+      const size_t lds = (size_t)a.d * (a.deg + 1) * ut * sizeof(double);
+      hipLaunchKernelGGL(lsm_update_kernel, (unsigned)((n + ut - 1) / ut), ut,
+                         lds, s, a, ratio, exv, beta, cash, vals);
+    }
     rc = check_launch("tqf_lsm_price");
     if (rc) return rc;
   }
