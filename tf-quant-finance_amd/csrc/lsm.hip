@@ -284,95 +284,158 @@ lsm_gram_reduce_kernel(const double* __restrict__ partial, int n_chunks,
 }
 
 // Column-scaled Cholesky with pivot truncation and the two triangular
-// solves, one workgroup, in place on the device (float64).  beta[K] out.
-__global__ void __launch_bounds__(1024)
+// solves, one workgroup of 1024 threads, float64.  beta[K] out.
+//
+// The factorization runs on the extended matrix [[G, b], [b^T, .]] of order
+// n = K + 1 <= 256 (b = X^T W y is column K of the Gram buffer): its last
+// row of L is z = L^-1 b, so the forward solve comes out of the
+// factorization.  Thread (ti, tj) of a 32 x 32 grid keeps the lower-triangle
+// entries (32a + ti, 32b + tj), b <= a < 8, of the scaled matrix in
+// registers (36 slots) for the whole factorization.  Column k is published
+// through LDS before its phase (double-buffered, one barrier per column);
+// in the phase the first n threads write the finished column of L to the
+// Gram buffer, and all update the entries right of it.  The back solve
+// L^T w = z runs on one wave from the Gram buffer.
+constexpr int kSolveThreads = 1024;
+
+__device__ __forceinline__ constexpr int solve_slot(int a, int b) {
+  return a > b ? a * (a - 1) / 2 + b : 28 + a;
+}
+
+__global__ void __launch_bounds__(kSolveThreads)
 lsm_solve_kernel(double* __restrict__ g, int K, double rtol,
-                 double* __restrict__ beta, double* __restrict__ work,
-                 int* __restrict__ rank_out) {
-  const int kext = K + 1;
-  double* dsc = work;          // [K] column scales
-  double* z = work + K;        // [K] forward-substitution result
+                 double* __restrict__ beta, int* __restrict__ rank_out) {
+  const int n = K + 1, kext = K + 1;
+  __shared__ double dsc[256];      // column scales (1 for the rhs row)
+  __shared__ double colb[2][256];  // column k of the working matrix
   __shared__ double maxdiag;
-  __shared__ int rank;
-  const int t = threadIdx.x, nt = blockDim.x;
-  for (int i = t; i < K; i += nt) {
-    const double d = g[(int64_t)i * kext + i];
-    dsc[i] = d > 0 ? 1.0 / sqrt(d) : 0.0;
-  }
-  __syncthreads();
-  // scale G and the rhs column (K <= 255: 32-bit index math)
-  for (int e = t; e < K * K; e += nt) {
-    const int r = e / K, c = e % K;
-    g[(int64_t)r * kext + c] *= dsc[r] * dsc[c];
-  }
-  for (int r = t; r < K; r += nt) g[(int64_t)r * kext + K] *= dsc[r];
-  __syncthreads();  // the diagonal must be scaled before its maximum
-  if (t == 0) {
-    double m = 0;
-    for (int i = 0; i < K; ++i) m = fmax(m, g[(int64_t)i * kext + i]);
-    maxdiag = m;
-    rank = 0;
-  }
-  __syncthreads();
-  // right-looking Cholesky on the lower triangle, one barrier per column:
-  // in the phase of column k every thread reads the pivot d = G[k][k]
-  // itself, subtracts G[i][k] G[j][k] / d from the trailing lower triangle
-  // (column k is only read there) and finishes column k - 1 (divides it by
-  // its pivot's root), whose last reads were in the previous phase.
-  double prev_root = 0;
-  for (int k = 0; k <= K; ++k) {
-    if (k > 0) {
-      // finish column k - 1
-      const int c = k - 1;
-      for (int i = c + 1 + t; i < K; i += nt)
-        g[(int64_t)i * kext + c] =
-            prev_root > 0 ? g[(int64_t)i * kext + c] / prev_root : 0.0;
-      if (t == 0) g[(int64_t)c * kext + c] = prev_root;
+  const int t = threadIdx.x, ti = t >> 5, tj = t & 31;
+  if (t < 256) {
+    double s = 0.0;
+    if (t < K) {
+      const double d = g[t * kext + t];
+      s = d > 0 ? 1.0 / sqrt(d) : 0.0;
+    } else if (t == K) {
+      s = 1.0;
     }
-    if (k == K) break;
-    const double d = g[(int64_t)k * kext + k];
-    const bool keep = d > rtol * maxdiag && d > 0;
+    dsc[t] = s;
+    colb[0][t] = 0.0;
+    colb[1][t] = 0.0;
+  }
+  __syncthreads();
+  // the largest scaled diagonal entry (wave 0)
+  if (t < 64) {
+    double m = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = t + 64 * q;
+      if (i < K) m = fmax(m, g[i * kext + i] * (dsc[i] * dsc[i]));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (t == 0) maxdiag = m;
+  }
+  double v[36];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b <= a; ++b) {
+      const int i = 32 * a + ti, j = 32 * b + tj;
+      v[solve_slot(a, b)] = (i < n && j <= i)
+          ? g[i * kext + j] * (dsc[i] * dsc[j]) : 0.0;
+    }
+  if (tj == 0) {
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int i = 32 * a + ti;
+      if (i < n) colb[0][i] = v[solve_slot(a, 0)];
+    }
+  }
+  __syncthreads();
+  const double thr = rtol * maxdiag;
+  int rank = 0;
+  for (int k = 0; k < K; ++k) {
+    const double* cb = colb[k & 1];
+    const double d = cb[k];
+    const bool keep = d > thr && d > 0;
+    const double root = keep ? sqrt(d) : 0.0;  // truncated: column dropped
+    rank += keep ? 1 : 0;
+    // column k of L (rows < K) and of z (row K): the pivot's root on the
+    // diagonal, the entries below divided by it
+    if (t >= k && t < n)
+      g[t * kext + k] = t == k ? root : (root > 0 ? cb[t] / root : 0.0);
+    // trailing update of the entries right of column k
     if (keep) {
       const double inv = 1.0 / d;
-      const int m = K - k - 1;
-      // m <= 255, so the flat index fits 32-bit division
-      for (int e = t; e < m * m; e += nt) {
-        const int i = k + 1 + e / m, j = k + 1 + e % m;
-        if (j <= i)
-          g[(int64_t)i * kext + j] -= g[(int64_t)i * kext + k] *
-                                      g[(int64_t)j * kext + k] * inv;
+      double cj[8];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) cj[b] = cb[32 * b + tj];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        if (32 * a + 31 <= k) continue;
+        const double ci = cb[32 * a + ti];
+#pragma unroll
+        for (int b = 0; b <= a; ++b) {
+          if (32 * b + 31 <= k) continue;
+          if (32 * b + tj > k) v[solve_slot(a, b)] -= ci * cj[b] * inv;
+        }
       }
-      if (t == 0) ++rank;
     }
-    prev_root = keep ? sqrt(d) : 0.0;  // truncated: the column is dropped
-    __syncthreads();
-  }
-  __syncthreads();
-  // L z = b and L^T w = z, column-oriented so every step is parallel;
-  // beta = D w.  z overwrites the rhs column, w goes to beta.
-  for (int j = 0; j < K; ++j) {
-    if (t == 0) {
-      const double ljj = g[(int64_t)j * kext + j];
-      z[j] = ljj > 0 ? g[(int64_t)j * kext + K] / ljj : 0.0;
-    }
-    __syncthreads();
-    const double zj = z[j];
-    for (int i = j + 1 + t; i < K; i += nt)
-      g[(int64_t)i * kext + K] -= g[(int64_t)i * kext + j] * zj;
-    __syncthreads();
-  }
-  for (int j = K - 1; j >= 0; --j) {
-    if (t == 0) {
-      const double ljj = g[(int64_t)j * kext + j];
-      beta[j] = ljj > 0 ? z[j] / ljj : 0.0;
+    // publish column k + 1 (its last update was the one above)
+    const int k1 = k + 1;
+    if (k1 < K && tj == (k1 & 31)) {
+      const int kb1 = k1 >> 5;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        if (b != kb1) continue;
+#pragma unroll
+        for (int a = b; a < 8; ++a) {
+          const int i = 32 * a + ti;
+          if (i >= k1 && i < n) colb[k1 & 1][i] = v[solve_slot(a, b)];
+        }
+      }
     }
     __syncthreads();
-    const double bj = beta[j];
-    for (int i = t; i < j; i += nt) z[i] -= g[(int64_t)j * kext + i] * bj;
-    __syncthreads();
   }
-  for (int i = t; i < K; i += nt) beta[i] *= dsc[i];
   if (t == 0) *rank_out = rank;
+  if (t >= 64) return;
+  // L^T w = z, column-oriented on one wave: lane l keeps the remainders of
+  // entries l, l + 64, l + 128, l + 192; eight rows of L are loaded ahead of
+  // their use.  beta = D w.  (The last barrier above ordered every write of
+  // L before these reads.)
+  double r[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = t + 64 * q;
+    r[q] = j < K ? g[K * kext + j] : 0.0;
+  }
+  for (int i0 = K - 1; i0 >= 0; i0 -= 8) {
+    double lrow[8][4], ldiag[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 - u;
+      ldiag[u] = i >= 0 ? g[i * kext + i] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = t + 64 * q;
+        lrow[u][q] = (i >= 0 && j < i) ? g[i * kext + j] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 - u;
+      if (i < 0) break;
+      const int qi = i >> 6;
+      double own = r[0];
+#pragma unroll
+      for (int q = 1; q < 4; ++q) own = qi == q ? r[q] : own;
+      own = __shfl(own, i & 63, 64);
+      const double w = ldiag[u] > 0 ? own / ldiag[u] : 0.0;
+      if (t == 0) beta[i] = w * dsc[i];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) r[q] -= lrow[u][q] * w;
+    }
+  }
 }
 
 // continuation = relu(X beta); exercise iff ex > continuation.  The
@@ -590,7 +653,7 @@ extern "C" int64_t tqf_lsm_workspace_size(int64_t n_paths, int32_t n_basis) {
   bytes += nb * 2 * kMaxAssets * 8 + 2 * nb * 8;  // block sums
   bytes += chunks * pairs * kTile * kTile * 8;  // Gram partials
   bytes += kext * kext * 8;                     // G
-  bytes += 4 * kext * 8 + 64;                   // beta, work, rank
+  bytes += 4 * kext * 8 + 64;                   // beta, rank (with slack)
   bytes += pairs * 8 + 4 * kMaxAssets * 8;      // pairs, mean, scale
   return bytes + 4096;
 }
@@ -635,7 +698,6 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
   double* partial = (double*)take((int64_t)256 * n_pairs * kTile * kTile * 8);
   double* g = (double*)take((int64_t)kext * kext * 8);
   double* beta = (double*)take(kext * 8);
-  double* work = (double*)take(2 * kext * 8);
   int* rank = (int*)take(64);
   int2* pairs = (int2*)take(n_pairs * 8);
   double* mean = (double*)take(kMaxAssets * 8);
@@ -679,8 +741,8 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
     hipLaunchKernelGGL(lsm_gram_reduce_kernel,
                        dim3((kTile * kTile + 255) / 256, n_pairs), 256, 0, s,
                        partial, n_chunks, n_pairs, pairs, kext, g);
-    hipLaunchKernelGGL(lsm_solve_kernel, 1, 1024, 0, s, g, K, l->rcond, beta,
-                       work, rank);
+    hipLaunchKernelGGL(lsm_solve_kernel, 1, kSolveThreads, 0, s, g, K,
+                       l->rcond, beta, rank);
     if (ranks) {
       he = hipMemcpyAsync(ranks + (e - 1), rank, sizeof(int),
                           hipMemcpyDeviceToDevice, s);
