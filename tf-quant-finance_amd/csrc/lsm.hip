@@ -283,6 +283,270 @@ lsm_gram_reduce_kernel(const double* __restrict__ partial, int n_chunks,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Moment Gram.  For a monomial basis the Gram entry
+//   G[k][l] = sum_p w_p prod_j c_pj^(e_kj + e_lj)
+// depends only on e_k + e_l, a point of the box [0, 2 deg]^d, and the rhs
+// entry G[k][K] = sum_p w_p y_p prod_j c_pj^e_kj on a point of [0, deg]^d.
+// The box moments are one float64 MFMA product A^T B over the paths, the
+// dimensions split in a row group (j < dA) and a column group:
+//   A[p][alpha] = prod_{j < dA} c_pj^alpha_j            (alpha in [0,2deg]^dA)
+//   B[p][beta]  = w_p prod_{j >= dA} c_pj^beta_j         (moment columns)
+//               = w_p y_p prod_{j >= dA} c_pj^beta'_j    (rhs columns,
+//                                                          beta' in [0,deg])
+// so M[alpha][beta] holds every moment once; G is gathered from M.  For the
+// tensor basis of 5 assets at degree 2 this is a 32 x 160 product per path
+// in place of the 244 x 244 Gram.
+struct MomentPlan {
+  int dA;         // dimensions in the row group
+  int radix;      // 2 deg + 1
+  int ra, rap;    // rows radix^dA, padded to a multiple of 16
+  int cbm, cby;   // moment columns radix^(d - dA), rhs columns (deg+1)^(d-dA)
+  int cbp;        // cbm + cby padded to a multiple of 16
+  int nblk;       // 16 x 16 output blocks: (rap / 16) * (cbp / 16)
+  int lds;        // dynamic LDS bytes of lsm_moment_kernel
+};
+constexpr int kMomMaxBlocks = 64;  // 16 accumulator blocks per wave
+
+constexpr int kMomThreads = 512;
+
+__global__ void __launch_bounds__(kMomThreads)
+lsm_moment_kernel(LsmArgs a, MomentPlan mp, const double* __restrict__ exv,
+                  const double* __restrict__ y, int64_t n_calib,
+                  int64_t chunk, double* __restrict__ partial) {
+  extern __shared__ double mlds[];
+  const int R = mp.radix, npw = a.d * R, dB = a.d - mp.dA;
+  const int ncols = mp.cbp + mp.rap;
+  const int tid = threadIdx.x;
+  double* pw = mlds;                      // [kSubPaths][npw] powers 0..2deg
+  double* xa = pw + kSubPaths * npw;      // [kSubPaths][rap]
+  double* xb = xa + kSubPaths * mp.rap;   // [kSubPaths][cbp]
+  double* wv = xb + kSubPaths * mp.cbp;   // [2][kSubPaths]: w, w y
+  uint8_t* offs = (uint8_t*)(wv + 2 * kSubPaths);  // [ncols][kMaxAssets]
+  uint8_t* kinds = offs + ncols * kMaxAssets;       // [ncols]
+  // columns 0 .. cbp - 1 are B (kind 0 pad, 1 w, 2 w y), then the rap
+  // columns of A (kind 3 live, 0 pad); offsets index a path's power row
+  // (j * radix + power)
+  for (int c = tid; c < ncols; c += kMomThreads) {
+    int kind, rem, rad, j0, nd;
+    if (c < mp.cbp) {
+      j0 = mp.dA;
+      nd = dB;
+      rem = c;
+      rad = R;
+      kind = 0;
+      if (c < mp.cbm) {
+        kind = 1;
+      } else if (c < mp.cbm + mp.cby) {
+        kind = 2;
+        rem = c - mp.cbm;
+        rad = a.deg + 1;
+      }
+    } else {
+      j0 = 0;
+      nd = mp.dA;
+      rem = c - mp.cbp;
+      rad = R;
+      kind = rem < mp.ra ? 3 : 0;
+    }
+    for (int q = 0; q < kMaxAssets; ++q) {
+      int o = 0;
+      if (q < nd) {
+        o = (j0 + q) * R + (kind ? rem % rad : 0);
+        rem /= rad;
+      }
+      offs[c * kMaxAssets + q] = (uint8_t)o;
+    }
+    kinds[c] = (uint8_t)kind;
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int ncb = mp.cbp >> 4;
+  constexpr int kPerWave = kMomMaxBlocks / (kMomThreads / 64);
+  f64x4 acc[kPerWave];
+#pragma unroll
+  for (int q = 0; q < kPerWave; ++q) acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
+  // column pass of this thread: column c0 + 256 i, rows h * 16 .. h * 16 + 15
+  const int c0 = tid & 255, h = tid >> 8;
+
+  const int64_t p_begin = (int64_t)blockIdx.x * chunk;
+  int64_t p_end = p_begin + chunk;
+  if (p_end > n_calib) p_end = n_calib;
+  for (int64_t p0 = p_begin; p0 < p_end; p0 += kSubPaths) {
+    __syncthreads();
+    // powers 0 .. 2 deg of the centred, scaled basis variable, one thread
+    // per (path, dimension); rows of paths past the chunk or out of the
+    // money are zero
+    if (tid < kSubPaths * a.d) {
+      const int s = tid & (kSubPaths - 1), j = tid / kSubPaths;
+      const int64_t p = p0 + s;
+      double w = 0.0;
+      if (p < p_end) w = exv[p] > 0 ? 1.0 : 0.0;  // in-the-money rows only
+      double* pr = pw + s * npw + j * R;
+      if (w != 0.0) {
+        const double x = a.paths[(p * a.n_times + a.t_idx) * a.d + j];
+        const double cv = (basis_input(a, x) - a.mean[j]) * a.scale[j];
+        double v = 1.0;
+        for (int m = 0; m < R; ++m) {
+          pr[m] = v;
+          v *= cv;
+        }
+      } else {
+        for (int m = 0; m < R; ++m) pr[m] = 0.0;
+      }
+      if (j == 0) {
+        wv[s] = w;
+        wv[kSubPaths + s] = w != 0.0 ? y[p] : 0.0;
+      }
+    }
+    __syncthreads();
+    // the A and B tiles of the staged paths
+    for (int c = c0; c < ncols; c += 256) {
+      const int kind = kinds[c];
+      int off[kMaxAssets];
+#pragma unroll
+      for (int q = 0; q < kMaxAssets; ++q) off[q] = offs[c * kMaxAssets + q];
+      const int nd = c < mp.cbp ? dB : mp.dA;
+      double* dst = c < mp.cbp ? xb + c : xa + (c - mp.cbp);
+      const int ld = c < mp.cbp ? mp.cbp : mp.rap;
+#pragma unroll 4
+      for (int s = h * 16; s < h * 16 + 16; ++s) {
+        double v = kind == 0 ? 0.0
+                 : kind == 3 ? 1.0 : wv[(kind - 1) * kSubPaths + s];
+        const double* pr = pw + s * npw;
+#pragma unroll
+        for (int q = 0; q < kMaxAssets; ++q)
+          if (q < nd) v *= pr[off[q]];
+        dst[s * ld] = v;
+      }
+    }
+    __syncthreads();
+    // M += A^T B over the staged paths, 4 paths per MFMA; block
+    // wave + 8 q of the (rap / 16) x (cbp / 16) grid of 16 x 16 blocks
+#pragma unroll
+    for (int k0 = 0; k0 < kSubPaths; k0 += 4) {
+#pragma unroll
+      for (int q = 0; q < kPerWave; ++q) {
+        const int blk = wave + (kMomThreads / 64) * q;
+        if (blk < mp.nblk) {
+          const int bi = blk / ncb, bj = blk - bi * ncb;
+          const double av = xa[(k0 + lk) * mp.rap + bi * 16 + lr];
+          const double bv = xb[(k0 + lk) * mp.cbp + bj * 16 + lr];
+          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0,
+                                                        0);
+        }
+      }
+    }
+  }
+  // partial[chunk][rap][cbp]; accumulator entry r of lane l is row
+  // l/16 + 4 r, column l%16 of its block
+  double* out = partial + (int64_t)blockIdx.x * (mp.rap * mp.cbp);
+#pragma unroll
+  for (int q = 0; q < kPerWave; ++q) {
+    const int blk = wave + (kMomThreads / 64) * q;
+    if (blk < mp.nblk) {
+      const int bi = blk / ncb, bj = blk - bi * ncb;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        out[(bi * 16 + lk + 4 * r) * mp.cbp + bj * 16 + lr] = acc[q][r];
+    }
+  }
+}
+
+// Sums the chunk partials of the moment product (fixed order).
+__global__ void __launch_bounds__(256)
+lsm_moment_reduce_kernel(const double* __restrict__ partial, int n_chunks,
+                         int size, double* __restrict__ m) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= size) return;
+  double s = 0;
+#pragma unroll 8
+  for (int c = 0; c < n_chunks; ++c) s += partial[(int64_t)c * size + e];
+  m[e] = s;
+}
+
+// Gathers the extended Gram G[kext][kext] from the moments: one thread per
+// entry.  An exponent above the degree has no moment: its row becomes NaN.
+__global__ void __launch_bounds__(256)
+lsm_moment_gram_kernel(const double* __restrict__ m, MomentPlan mp,
+                       const uint8_t* __restrict__ expo, int K, int d,
+                       int deg, double* __restrict__ g) {
+  const int kext = K + 1;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= kext * kext) return;
+  const int k = e / kext, l = e - k * kext;
+  double v = 0.0;
+  if (k < K || l < K) {
+    const bool rhs = k == K || l == K;
+    const int r0 = rhs ? (k == K ? l : k) : k;
+    bool ok = true;
+    int alpha = 0, beta = 0, mul = 1;
+    for (int j = 0; j < mp.dA; ++j) {
+      const int ek = expo[r0 * d + j];
+      const int el = rhs ? 0 : expo[l * d + j];
+      ok = ok && ek <= deg && el <= deg;
+      alpha += (ek + el) * mul;
+      mul *= mp.radix;
+    }
+    const int rad = rhs ? deg + 1 : mp.radix;
+    mul = 1;
+    for (int j = mp.dA; j < d; ++j) {
+      const int ek = expo[r0 * d + j];
+      const int el = rhs ? 0 : expo[l * d + j];
+      ok = ok && ek <= deg && el <= deg;
+      beta += (ek + el) * mul;
+      mul *= rad;
+    }
+    if (rhs) beta += mp.cbm;
+    const double mv = m[ok ? alpha * mp.cbp + beta : 0];
+    v = ok ? mv : __longlong_as_double(0x7ff8000000000000LL);
+  }
+  g[e] = v;
+}
+
+// The moment plan for (d, deg): the split with the smallest padded product
+// whose tiles fit 64 KB of LDS and 64 output blocks, and only when the
+// product is cheaper than the dense Gram tiles (whose partial buffer then
+// holds the moment partials and their sum).  False when none qualifies.
+bool moment_plan(int d, int deg, int n_pairs, MomentPlan* out) {
+  const int R = 2 * deg + 1;
+  if (d * R > 256) return false;
+  bool found = false;
+  MomentPlan best{};
+  int64_t ra = 1;
+  for (int dA = 0; dA <= d; ++dA, ra *= R) {
+    if (ra > 128) break;
+    int64_t cbm = 1, cby = 1;
+    for (int j = dA; j < d; ++j) {
+      cbm *= R;
+      cby *= deg + 1;
+    }
+    if (cbm + cby > 256) continue;
+    MomentPlan p;
+    p.dA = dA;
+    p.radix = R;
+    p.ra = (int)ra;
+    p.rap = (int)((ra + 15) / 16 * 16);
+    p.cbm = (int)cbm;
+    p.cby = (int)cby;
+    p.cbp = (int)((cbm + cby + 15) / 16 * 16);
+    p.nblk = (p.rap / 16) * (p.cbp / 16);
+    if (p.nblk > kMomMaxBlocks) continue;
+    p.lds = (kSubPaths * (d * R + p.rap + p.cbp) + 2 * kSubPaths) * 8 +
+            (p.rap + p.cbp) * (kMaxAssets + 1);
+    if (p.lds > 65536) continue;
+    if (!found || (int64_t)p.rap * p.cbp < (int64_t)best.rap * best.cbp) {
+      best = p;
+      found = true;
+    }
+  }
+  if (!found) return false;
+  const int64_t size = (int64_t)best.rap * best.cbp;
+  if (size * 257 > (int64_t)256 * n_pairs * kTile * kTile) return false;
+  *out = best;
+  return true;
+}
+
 // Column-scaled Cholesky with pivot truncation and the two triangular
 // solves, one workgroup of 1024 threads, float64.  beta[K] out.
 //
@@ -720,6 +984,8 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
   if (he != hipSuccess) return set_error((int)he, "copy pairs");
 
   tqf_lsm l2 = *l;
+  MomentPlan mp{};
+  const bool use_moments = moment_plan(l->dim, l->degree, n_pairs, &mp);
   LsmArgs a = make_args(&l2, ex_idx[n_ex - 1]);
   hipLaunchKernelGGL(lsm_terminal_kernel, blocks_for(n), 256, 0, s, a, cash,
                      vals);
@@ -736,11 +1002,22 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
                        cash, vals, exv, y, bsum);
     hipLaunchKernelGGL(lsm_mean_kernel, 1, 256, 0, s, bsum, nb, a.d, n, mean,
                        scale);
-    hipLaunchKernelGGL(lsm_gram_kernel, dim3(n_chunks, n_pairs), 256, 0, s, a,
-                       exv, y, n_calib, chunk, pairs, partial);
-    hipLaunchKernelGGL(lsm_gram_reduce_kernel,
-                       dim3((kTile * kTile + 255) / 256, n_pairs), 256, 0, s,
-                       partial, n_chunks, n_pairs, pairs, kext, g);
+    if (use_moments) {
+      const int size = mp.rap * mp.cbp;
+      double* msum = partial + (int64_t)n_chunks * size;
+      hipLaunchKernelGGL(lsm_moment_kernel, n_chunks, kMomThreads, mp.lds, s,
+                         a, mp, exv, y, n_calib, chunk, partial);
+      hipLaunchKernelGGL(lsm_moment_reduce_kernel, (size + 255) / 256, 256, 0,
+                         s, partial, n_chunks, size, msum);
+      hipLaunchKernelGGL(lsm_moment_gram_kernel, (kext * kext + 255) / 256,
+                         256, 0, s, msum, mp, a.expo, K, a.d, a.deg, g);
+    } else {
+      hipLaunchKernelGGL(lsm_gram_kernel, dim3(n_chunks, n_pairs), 256, 0, s,
+                         a, exv, y, n_calib, chunk, pairs, partial);
+      hipLaunchKernelGGL(lsm_gram_reduce_kernel,
+                         dim3((kTile * kTile + 255) / 256, n_pairs), 256, 0, s,
+                         partial, n_chunks, n_pairs, pairs, kext, g);
+    }
     hipLaunchKernelGGL(lsm_solve_kernel, 1, kSolveThreads, 0, s, g, K,
                        l->rcond, beta, rank);
     if (ranks) {
