@@ -309,142 +309,202 @@ struct MomentPlan {
 constexpr int kMomMaxBlocks = 64;  // 16 accumulator blocks per wave
 
 constexpr int kMomThreads = 512;
+constexpr int kMomMaxLds = 160 * 1024;  // gfx950 LDS per workgroup
+constexpr int kMomMaxHi = 128;          // high-digit combinations per group
 
+// Columns of one dimension group for the staged paths: column
+// sum_q digit_q rad^q (digit q of dimension j0 + q) is
+// wgt_s prod_q c_s,j0+q ^ digit_q.  One item per (path, high digits): the
+// high digits' product times the rad powers of dimension j0.
+__device__ __forceinline__ void moment_group(
+    const double* __restrict__ pwb, int npw, int R,
+    const double* __restrict__ wgt, int j0, int nd, int rad,
+    const uint8_t* __restrict__ hioff, double* __restrict__ dst, int ld,
+    int tid) {
+  if (nd == 0) {
+    if (tid < kSubPaths) dst[tid * ld] = wgt ? wgt[tid] : 1.0;
+    return;
+  }
+  int n_hi = 1;
+  for (int q = 1; q < nd; ++q) n_hi *= rad;
+  for (int it = tid; it < kSubPaths * n_hi; it += kMomThreads) {
+    const int s = it & (kSubPaths - 1), hi = it / kSubPaths;
+    const double* pr = pwb + s * npw;
+    double base = wgt ? wgt[s] : 1.0;
+    for (int q = 0; q < nd - 1; ++q)
+      base *= pr[hioff[hi * kMaxAssets + q]];
+    const double* p0 = pr + j0 * R;
+    double* o = dst + s * ld + hi * rad;
+    for (int e = 0; e < rad; ++e) o[e] = base * p0[e];
+  }
+}
+
+// The product runs as a pipeline over 32-path sub-batches with one barrier
+// per phase: in phase i the waves multiply the tiles of sub-batch i, build
+// the tiles of sub-batch i + 1 from its powers, write the powers of
+// sub-batch i + 2 from registers and load the inputs of sub-batch i + 3.
+// Tiles, powers and weights are double-buffered in LDS; the tiles' leading
+// dimensions are odd, so the 32 paths of a column fall in distinct banks.
 __global__ void __launch_bounds__(kMomThreads)
 lsm_moment_kernel(LsmArgs a, MomentPlan mp, const double* __restrict__ exv,
                   const double* __restrict__ y, int64_t n_calib,
                   int64_t chunk, double* __restrict__ partial) {
   extern __shared__ double mlds[];
   const int R = mp.radix, npw = a.d * R, dB = a.d - mp.dA;
-  const int ncols = mp.cbp + mp.rap;
+  const int lda = mp.rap + 1, ldb = mp.cbp + 1;
   const int tid = threadIdx.x;
-  double* pw = mlds;                      // [kSubPaths][npw] powers 0..2deg
-  double* xa = pw + kSubPaths * npw;      // [kSubPaths][rap]
-  double* xb = xa + kSubPaths * mp.rap;   // [kSubPaths][cbp]
-  double* wv = xb + kSubPaths * mp.cbp;   // [2][kSubPaths]: w, w y
-  uint8_t* offs = (uint8_t*)(wv + 2 * kSubPaths);  // [ncols][kMaxAssets]
-  uint8_t* kinds = offs + ncols * kMaxAssets;       // [ncols]
-  // columns 0 .. cbp - 1 are B (kind 0 pad, 1 w, 2 w y), then the rap
-  // columns of A (kind 3 live, 0 pad); offsets index a path's power row
-  // (j * radix + power)
-  for (int c = tid; c < ncols; c += kMomThreads) {
-    int kind, rem, rad, j0, nd;
-    if (c < mp.cbp) {
-      j0 = mp.dA;
-      nd = dB;
-      rem = c;
-      rad = R;
-      kind = 0;
-      if (c < mp.cbm) {
-        kind = 1;
-      } else if (c < mp.cbm + mp.cby) {
-        kind = 2;
-        rem = c - mp.cbm;
-        rad = a.deg + 1;
+  double* const pw0 = mlds;                        // [2][kSubPaths][npw]
+  double* const xa0 = pw0 + 2 * kSubPaths * npw;   // [2][kSubPaths][lda]
+  double* const xb0 = xa0 + 2 * kSubPaths * lda;   // [2][kSubPaths][ldb]
+  double* const wv0 = xb0 + 2 * kSubPaths * ldb;   // [2][2][kSubPaths]
+  // high-digit offsets into a path's power row (j * radix + power) of the
+  // groups A, B moments, B rhs: [3][kMomMaxHi][kMaxAssets]
+  uint8_t* hioff = (uint8_t*)(wv0 + 4 * kSubPaths);
+  {
+    const int j0s[3] = {0, mp.dA, mp.dA};
+    const int nds[3] = {mp.dA, dB, dB};
+    const int rads[3] = {R, R, a.deg + 1};
+    for (int gq = 0; gq < 3; ++gq) {
+      int n_hi = 1;
+      for (int q = 1; q < nds[gq]; ++q) n_hi *= rads[gq];
+      for (int hi = tid; hi < n_hi; hi += kMomThreads) {
+        int rem = hi;
+        for (int q = 1; q < nds[gq]; ++q) {
+          hioff[(gq * kMomMaxHi + hi) * kMaxAssets + q - 1] =
+              (uint8_t)((j0s[gq] + q) * R + rem % rads[gq]);
+          rem /= rads[gq];
+        }
       }
-    } else {
-      j0 = 0;
-      nd = mp.dA;
-      rem = c - mp.cbp;
-      rad = R;
-      kind = rem < mp.ra ? 3 : 0;
     }
-    for (int q = 0; q < kMaxAssets; ++q) {
-      int o = 0;
-      if (q < nd) {
-        o = (j0 + q) * R + (kind ? rem % rad : 0);
-        rem /= rad;
-      }
-      offs[c * kMaxAssets + q] = (uint8_t)o;
-    }
-    kinds[c] = (uint8_t)kind;
   }
-  const int lane = tid & 63, wave = tid >> 6;
+  // padding columns stay zero (both buffers)
+  for (int e = tid; e < 2 * kSubPaths * lda; e += kMomThreads)
+    if (e % lda >= mp.ra) xa0[e] = 0.0;
+  for (int e = tid; e < 2 * kSubPaths * ldb; e += kMomThreads)
+    if (e % ldb >= mp.cbm + mp.cby) xb0[e] = 0.0;
+
+  // the wave index is uniform: an SGPR keeps the block guards scalar
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lk = lane >> 4;
   const int ncb = mp.cbp >> 4;
   constexpr int kPerWave = kMomMaxBlocks / (kMomThreads / 64);
+  const int nmine = (mp.nblk - wave + (kMomThreads / 64) - 1) /
+                    (kMomThreads / 64);  // blocks of this wave
+  // LDS offsets of this lane's A and B operands in block wave + 8 q
+  int aoff[kPerWave], boff[kPerWave];
+#pragma unroll
+  for (int q = 0; q < kPerWave; ++q) {
+    const int blk = wave + (kMomThreads / 64) * q;
+    const int bi = blk / ncb, bj = blk - bi * ncb;
+    aoff[q] = lk * lda + bi * 16 + lr;
+    boff[q] = lk * ldb + bj * 16 + lr;
+  }
   f64x4 acc[kPerWave];
 #pragma unroll
   for (int q = 0; q < kPerWave; ++q) acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
-  // column pass of this thread: column c0 + 256 i, rows h * 16 .. h * 16 + 15
-  const int c0 = tid & 255, h = tid >> 8;
 
   const int64_t p_begin = (int64_t)blockIdx.x * chunk;
   int64_t p_end = p_begin + chunk;
   if (p_end > n_calib) p_end = n_calib;
-  for (int64_t p0 = p_begin; p0 < p_end; p0 += kSubPaths) {
-    __syncthreads();
-    // powers 0 .. 2 deg of the centred, scaled basis variable, one thread
-    // per (path, dimension); rows of paths past the chunk or out of the
-    // money are zero
-    if (tid < kSubPaths * a.d) {
-      const int s = tid & (kSubPaths - 1), j = tid / kSubPaths;
-      const int64_t p = p0 + s;
-      double w = 0.0;
-      if (p < p_end) w = exv[p] > 0 ? 1.0 : 0.0;  // in-the-money rows only
-      double* pr = pw + s * npw + j * R;
-      if (w != 0.0) {
-        const double x = a.paths[(p * a.n_times + a.t_idx) * a.d + j];
-        const double cv = (basis_input(a, x) - a.mean[j]) * a.scale[j];
-        double v = 1.0;
-        for (int m = 0; m < R; ++m) {
-          pr[m] = v;
-          v *= cv;
-        }
-      } else {
-        for (int m = 0; m < R; ++m) pr[m] = 0.0;
-      }
-      if (j == 0) {
-        wv[s] = w;
-        wv[kSubPaths + s] = w != 0.0 ? y[p] : 0.0;
-      }
+  const int nsub = p_end > p_begin
+      ? (int)((p_end - p_begin + kSubPaths - 1) / kSubPaths) : 0;
+
+  // power threads: one per (path s, dimension j) of a sub-batch
+  const bool pthread = tid < kSubPaths * a.d;
+  const int ps = tid & (kSubPaths - 1), pj = tid / kSubPaths;
+  double in_e = 0.0, in_x = 0.0, in_y = 0.0;  // inputs of the next powers
+  auto load = [&](int i) {
+    const int64_t p = p_begin + (int64_t)i * kSubPaths + ps;
+    in_e = 0.0;
+    if (pthread && i < nsub && p < p_end) {
+      in_e = exv[p];
+      in_x = a.paths[(p * a.n_times + a.t_idx) * a.d + pj];
+      if (pj == 0) in_y = y[p];
     }
-    __syncthreads();
-    // the A and B tiles of the staged paths
-    for (int c = c0; c < ncols; c += 256) {
-      const int kind = kinds[c];
-      int off[kMaxAssets];
-#pragma unroll
-      for (int q = 0; q < kMaxAssets; ++q) off[q] = offs[c * kMaxAssets + q];
-      const int nd = c < mp.cbp ? dB : mp.dA;
-      double* dst = c < mp.cbp ? xb + c : xa + (c - mp.cbp);
-      const int ld = c < mp.cbp ? mp.cbp : mp.rap;
-#pragma unroll 4
-      for (int s = h * 16; s < h * 16 + 16; ++s) {
-        double v = kind == 0 ? 0.0
-                 : kind == 3 ? 1.0 : wv[(kind - 1) * kSubPaths + s];
-        const double* pr = pw + s * npw;
-#pragma unroll
-        for (int q = 0; q < kMaxAssets; ++q)
-          if (q < nd) v *= pr[off[q]];
-        dst[s * ld] = v;
+  };
+  // powers 0 .. 2 deg of the centred, scaled basis variable; rows of paths
+  // past the chunk or out of the money are zero
+  auto powers = [&](int buf) {
+    if (!pthread) return;
+    const double w = in_e > 0 ? 1.0 : 0.0;  // in-the-money rows only
+    double* pr = pw0 + (buf * kSubPaths + ps) * npw + pj * R;
+    if (w != 0.0) {
+      const double cv = (basis_input(a, in_x) - a.mean[pj]) * a.scale[pj];
+      double v = 1.0;
+      for (int m = 0; m < R; ++m) {
+        pr[m] = v;
+        v *= cv;
       }
+    } else {
+      for (int m = 0; m < R; ++m) pr[m] = 0.0;
     }
-    __syncthreads();
-    // M += A^T B over the staged paths, 4 paths per MFMA; block
-    // wave + 8 q of the (rap / 16) x (cbp / 16) grid of 16 x 16 blocks
+    if (pj == 0) {
+      double* wv = wv0 + buf * 2 * kSubPaths;
+      wv[ps] = w;
+      wv[kSubPaths + ps] = w != 0.0 ? in_y : 0.0;
+    }
+  };
+  // the A and B tiles of one sub-batch
+  auto tiles = [&](int buf) {
+    const double* pwb = pw0 + buf * kSubPaths * npw;
+    const double* wv = wv0 + buf * 2 * kSubPaths;
+    double* xa = xa0 + buf * kSubPaths * lda;
+    double* xb = xb0 + buf * kSubPaths * ldb;
+    moment_group(pwb, npw, R, nullptr, 0, mp.dA, R, hioff, xa, lda, tid);
+    moment_group(pwb, npw, R, wv, mp.dA, dB, R, hioff + kMomMaxHi * kMaxAssets,
+                 xb, ldb, tid);
+    moment_group(pwb, npw, R, wv + kSubPaths, mp.dA, dB, a.deg + 1,
+                 hioff + 2 * kMomMaxHi * kMaxAssets, xb + mp.cbm, ldb, tid);
+  };
+  // M += A^T B over one sub-batch, 4 paths per MFMA; block wave + 8 q of
+  // the (rap / 16) x (cbp / 16) grid of 16 x 16 blocks
+  auto multiply = [&](int buf) {
+    const double* xa = xa0 + buf * kSubPaths * lda;
+    const double* xb = xb0 + buf * kSubPaths * ldb;
 #pragma unroll
     for (int k0 = 0; k0 < kSubPaths; k0 += 4) {
+      double av[kPerWave], bv[kPerWave];
 #pragma unroll
       for (int q = 0; q < kPerWave; ++q) {
-        const int blk = wave + (kMomThreads / 64) * q;
-        if (blk < mp.nblk) {
-          const int bi = blk / ncb, bj = blk - bi * ncb;
-          const double av = xa[(k0 + lk) * mp.rap + bi * 16 + lr];
-          const double bv = xb[(k0 + lk) * mp.cbp + bj * 16 + lr];
-          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0,
-                                                        0);
+        if (q < nmine) {
+          av[q] = xa[k0 * lda + aoff[q]];
+          bv[q] = xb[k0 * ldb + boff[q]];
         }
       }
+#pragma unroll
+      for (int q = 0; q < kPerWave; ++q)
+        if (q < nmine)
+          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc[q],
+                                                        0, 0, 0);
     }
+  };
+
+  load(0);
+  __syncthreads();  // offset tables and padding
+  powers(0);
+  load(1);
+  __syncthreads();
+  if (nsub > 0) tiles(0);
+  powers(1);
+  load(2);
+  __syncthreads();
+  for (int i = 0; i < nsub; ++i) {
+    multiply(i & 1);
+    if (i + 1 < nsub) tiles((i + 1) & 1);
+    if (i + 2 < nsub) {
+      powers(i & 1);
+      load(i + 3);
+    }
+    __syncthreads();
   }
   // partial[chunk][rap][cbp]; accumulator entry r of lane l is row
   // l/16 + 4 r, column l%16 of its block
   double* out = partial + (int64_t)blockIdx.x * (mp.rap * mp.cbp);
 #pragma unroll
   for (int q = 0; q < kPerWave; ++q) {
-    const int blk = wave + (kMomThreads / 64) * q;
-    if (blk < mp.nblk) {
+    if (q < nmine) {
+      const int blk = wave + (kMomThreads / 64) * q;
       const int bi = blk / ncb, bj = blk - bi * ncb;
 #pragma unroll
       for (int r = 0; r < 4; ++r)
@@ -505,9 +565,10 @@ lsm_moment_gram_kernel(const double* __restrict__ m, MomentPlan mp,
 }
 
 // The moment plan for (d, deg): the split with the smallest padded product
-// whose tiles fit 64 KB of LDS and 64 output blocks, and only when the
-// product is cheaper than the dense Gram tiles (whose partial buffer then
-// holds the moment partials and their sum).  False when none qualifies.
+// whose buffers fit the LDS of a workgroup and 64 output blocks, and only
+// when the product is cheaper than the dense Gram tiles (whose partial
+// buffer then holds the moment partials and their sum).  False when none
+// qualifies.
 bool moment_plan(int d, int deg, int n_pairs, MomentPlan* out) {
   const int R = 2 * deg + 1;
   if (d * R > 256) return false;
@@ -532,9 +593,14 @@ bool moment_plan(int d, int deg, int n_pairs, MomentPlan* out) {
     p.cbp = (int)((cbm + cby + 15) / 16 * 16);
     p.nblk = (p.rap / 16) * (p.cbp / 16);
     if (p.nblk > kMomMaxBlocks) continue;
-    p.lds = (kSubPaths * (d * R + p.rap + p.cbp) + 2 * kSubPaths) * 8 +
-            (p.rap + p.cbp) * (kMaxAssets + 1);
-    if (p.lds > 65536) continue;
+    p.lds = (2 * kSubPaths * (d * R + p.rap + 1 + p.cbp + 1) +
+             4 * kSubPaths) * 8 + 3 * kMomMaxHi * kMaxAssets;
+    if (p.lds > kMomMaxLds) continue;
+    // high-digit combinations of each group
+    int64_t hi_a = 1, hi_b = 1;
+    for (int j = 1; j < dA; ++j) hi_a *= R;
+    for (int j = dA + 1; j < d; ++j) hi_b *= R;
+    if (hi_a > kMomMaxHi || hi_b > kMomMaxHi) continue;
     if (!found || (int64_t)p.rap * p.cbp < (int64_t)best.rap * best.cbp) {
       best = p;
       found = true;
