@@ -813,6 +813,106 @@ lsm_update_kernel(LsmArgs a, double df_ratio, const double* __restrict__ exv,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Dense update.  The continuation value of a monomial basis is the
+// polynomial sum_k beta_k prod_j c_j^e_kj; with its coefficients scattered
+// into the dense tensor T[m_0 + (deg+1) m_1 + ...] it is evaluated by nested
+// Horner steps, about (deg+1)^d FMAs per path, the coefficients read through
+// uniform (scalar) loads.
+
+constexpr int kMaxDense = 1024;  // dense coefficients
+
+// T from beta: entry i is the sum, in basis order, of the beta_k whose
+// exponent row indexes i (rows with an exponent above the degree have no
+// entry).  One workgroup.
+__global__ void __launch_bounds__(kMaxDense)
+lsm_dense_beta_kernel(const double* __restrict__ beta,
+                      const uint8_t* __restrict__ expo, int K, int d, int deg,
+                      int ndense, double* __restrict__ dense) {
+  __shared__ int idx[256];
+  const int t = threadIdx.x;
+  if (t < K) {
+    int i = 0, mul = 1;
+    bool ok = true;
+    for (int j = 0; j < d; ++j) {
+      const int e = expo[t * d + j];
+      ok = ok && e <= deg;
+      i += e * mul;
+      mul *= deg + 1;
+    }
+    idx[t] = ok ? i : -1;
+  }
+  __syncthreads();
+  if (t >= ndense) return;
+  double v = 0.0;
+  for (int k = 0; k < K; ++k)
+    if (idx[k] == t) v += beta[k];
+  dense[t] = v;
+}
+
+// sum_m c_J^m (polynomial of the lower dimensions at t + m st_J), by Horner
+template <int J>
+struct DenseHorner {
+  __device__ __forceinline__ static double eval(const double* __restrict__ t,
+                                                const double* c, const int* st,
+                                                int deg) {
+    double h = DenseHorner<J - 1>::eval(t + deg * st[J], c, st, deg);
+    for (int m = deg - 1; m >= 0; --m)
+      h = fma(h, c[J], DenseHorner<J - 1>::eval(t + m * st[J], c, st, deg));
+    return h;
+  }
+};
+template <>
+struct DenseHorner<0> {
+  __device__ __forceinline__ static double eval(const double* __restrict__ t,
+                                                const double* c, const int*,
+                                                int deg) {
+    double h = t[deg];
+    for (int m = deg - 1; m >= 0; --m) h = fma(h, c[0], t[m]);
+    return h;
+  }
+};
+
+// lsm_update_kernel with the continuation value from the dense tensor.
+template <int D>
+__global__ void __launch_bounds__(256)
+lsm_update_dense_kernel(LsmArgs a, double df_ratio,
+                        const double* __restrict__ exv,
+                        const double* __restrict__ dense,
+                        double* __restrict__ cash, double* __restrict__ vals) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.n_paths) return;
+  const double* row = a.paths + (p * a.n_times + a.t_idx) * D;
+  double c[D];
+  int st[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    c[j] = (basis_input(a, row[j]) - a.mean[j]) * a.scale[j];
+    st[j] = j == 0 ? 1 : st[j - 1] * (a.deg + 1);
+  }
+  double cont = DenseHorner<D - 1>::eval(dense, c, st, a.deg);
+  cont = cont > 0 ? cont : 0;
+  const double ex = exv[p];
+  if (ex > cont) {
+    vals[p] = 0;
+    cash[p] = ex;
+  } else {
+    vals[p] = disc_ratio(a, df_ratio, p) * (cash[p] + vals[p]);
+    cash[p] = 0;
+  }
+}
+
+// Dense coefficients of (d, deg), or 0 when the dense update does not apply:
+// more than kMaxDense of them, or more work than the basis-order sum.
+int dense_size(int d, int deg, int K) {
+  int64_t n = 1;
+  for (int j = 0; j < d; ++j) {
+    n *= deg + 1;
+    if (n > kMaxDense) return 0;
+  }
+  return n <= (int64_t)K * (d + 1) ? (int)n : 0;
+}
+
 // Threads per workgroup of lsm_update_kernel: 256, or 64 when the power
 // table would take more than 48 KB of LDS at 256.
 inline int update_threads(const LsmArgs& a) {
@@ -1052,6 +1152,7 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
   tqf_lsm l2 = *l;
   MomentPlan mp{};
   const bool use_moments = moment_plan(l->dim, l->degree, n_pairs, &mp);
+  const int ndense = dense_size(l->dim, l->degree, K);
   LsmArgs a = make_args(&l2, ex_idx[n_ex - 1]);
   hipLaunchKernelGGL(lsm_terminal_kernel, blocks_for(n), 256, 0, s, a, cash,
                      vals);
@@ -1091,7 +1192,28 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
                           hipMemcpyDeviceToDevice, s);
       if (he != hipSuccess) return set_error((int)he, "copy rank");
     }
-    {
+    if (ndense > 0) {
+      // the Gram partials are spent: their buffer holds the dense tensor
+      double* dense = partial;
+      hipLaunchKernelGGL(lsm_dense_beta_kernel, 1, kMaxDense, 0, s, beta,
+                         a.expo, K, a.d, a.deg, ndense, dense);
+      switch (a.d) {
+#define TQF_DENSE_UPDATE(D)                                                  \
+  case D:                                                                    \
+    hipLaunchKernelGGL(lsm_update_dense_kernel<D>, blocks_for(n), 256, 0, s, \
+                       a, ratio, exv, dense, cash, vals);                    \
+    break;
+        TQF_DENSE_UPDATE(1)
+        TQF_DENSE_UPDATE(2)
+        TQF_DENSE_UPDATE(3)
+        TQF_DENSE_UPDATE(4)
+        TQF_DENSE_UPDATE(5)
+        TQF_DENSE_UPDATE(6)
+        TQF_DENSE_UPDATE(7)
+        TQF_DENSE_UPDATE(8)
+#undef TQF_DENSE_UPDATE
+      }
+    } else {
       const int ut = update_threads(a);
       const size_t lds = (size_t)a.d * (a.deg + 1) * ut * sizeof(double);
       hipLaunchKernelGGL(lsm_update_kernel, (unsigned)((n + ut - 1) / ut), ut,
