@@ -36,12 +36,18 @@ __device__ __forceinline__ short to_bf16(float f) {
   return (short)(u >> 16);
 }
 
-__device__ __forceinline__ float silu(float z) {
-  return z / (1.0f + __expf(-z));
+// logistic sigmoid through the hardware reciprocal (1 ulp; the network runs
+// its products in bf16)
+__device__ __forceinline__ float sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
+// SiLU'(z) from z and its sigmoid s
+__device__ __forceinline__ float silu_grad_s(float z, float s) {
+  return s * (1.0f + z * (1.0f - s));
 }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
-  return s * (1.0f + z * (1.0f - s));
+  return silu_grad_s(z, sigmoid(z));
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -58,12 +64,20 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Sum over the 16 lanes sharing l / 16 (lanes differing in bits 0..3).
+// Sum over the 16 lanes sharing l / 16 (one DPP row), by row-local DPP
+// moves: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror.  Every lane adds the same two partial sums at each step, so
+// all 16 lanes end with the same value.
+template <int kCtrl>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(
+      0, __float_as_int(v), kCtrl, 0xf, 0xf, false));
+}
 __device__ __forceinline__ float sum16(float v) {
-  v += __shfl_xor(v, 1, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 4, 64);
-  v += __shfl_xor(v, 8, 64);
+  v += dpp<0xb1>(v);
+  v += dpp<0x4e>(v);
+  v += dpp<0x141>(v);
+  v += dpp<0x140>(v);
   return v;
 }
 
@@ -73,6 +87,7 @@ struct Fwd {
   float z1a[4][4];    // [chunk c][j]: sample l%16, feature c*16+4*(l/16)+j
   bf16x4 h1a[4];      // A operands of layer 2
   f32x4 z2[4];        // [n-tile t]: rows 4*(l/16)+i, column t*16 + l%16
+  f32x4 s2[4];        // sigmoid(z2), same layout
   float y[4];         // outputs of rows 4*(l/16)+i
 };
 
@@ -112,7 +127,11 @@ __device__ __forceinline__ void forward_group(const float* __restrict__ P,
   for (int i = 0; i < 4; ++i) {
     float s = 0.f;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) s += silu(f.z2[t][i]) * P[oW3 + t * 16 + r];
+    for (int t = 0; t < 4; ++t) {
+      const float sg = sigmoid(f.z2[t][i]);
+      f.s2[t][i] = sg;
+      s += f.z2[t][i] * sg * P[oW3 + t * 16 + r];
+    }
     f.y[i] = sum16(s) + P[oB3];
   }
 }
@@ -187,13 +206,20 @@ mlp_forward_kernel(const float* __restrict__ params,
   const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
   const int64_t waves = (int64_t)gridDim.x * 4;
   const int64_t wid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  // the next group's rows are loaded while this one is processed
+  float nx[kIn];
+  auto fetch = [&](int64_t g) {
+    const int64_t sa = g * 16 + r;
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) nx[i] = sa < n ? x[sa * kMlpStride + i] : 0.f;
+  };
+  fetch(wid);
   for (int64_t g = wid; g * 16 < n; g += waves) {
     const int64_t base = g * 16;
     float xa[kIn];
-    const int64_t sa = base + r;
 #pragma unroll
-    for (int i = 0; i < kIn; ++i)
-      xa[i] = sa < n ? x[sa * kMlpStride + i] : 0.f;
+    for (int i = 0; i < kIn; ++i) xa[i] = nx[i];
+    fetch(g + waves);
     Fwd f;
     forward_group(P, w2b, xa, f);
     if (r == 0) {
@@ -236,31 +262,38 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
   }
   const int64_t waves = (int64_t)gridDim.x * 4;
   const int64_t wid = (int64_t)blockIdx.x * 4 + wave;
+  const int64_t end = first + n;
+  // inputs of a group: lane l holds sample base + l % 16 (its row, weight
+  // and target; zero past the end).  The next group's are loaded while
+  // this one is processed, so the memory latency is not waited out per
+  // group; the rows 4q + i the backward pass needs come by shuffles.
+  float nx[kIn], nw = 0.f, nt = 0.f;
+  auto fetch = [&](int64_t g) {
+    const int64_t sa = first + g * 16 + r;
+    const bool ok = g * 16 < n && sa < end;
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) nx[i] = ok ? x[sa * kMlpStride + i] : 0.f;
+    nw = ok ? wts[sa] : 0.f;
+    nt = ok ? tgt[sa] : 0.f;
+  };
+  fetch(wid);
   for (int64_t g = wid; g * 16 < n; g += waves) {
     const int64_t base = first + g * 16;
-    const int64_t end = first + n;
     float xa[kIn];
-    const int64_t sa = base + r;
 #pragma unroll
-    for (int i = 0; i < kIn; ++i)
-      xa[i] = sa < end ? x[sa * kMlpStride + i] : 0.f;
+    for (int i = 0; i < kIn; ++i) xa[i] = nx[i];
+    const float wa = nw, ta = nt;
+    fetch(g + waves);
     Fwd f;
     forward_group(P, w2b, xa, f);
     // dy for the lane's rows
     float dy[4], xr[4][kIn];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int64_t s = base + 4 * q + i;
-      float w = 0.f, t = 0.f;
-      if (s < end) {
-        w = wts[s];
-        t = tgt[s];
+      const int src = 4 * q + i;
 #pragma unroll
-        for (int k = 0; k < kIn; ++k) xr[i][k] = x[s * kMlpStride + k];
-      } else {
-#pragma unroll
-        for (int k = 0; k < kIn; ++k) xr[i][k] = 0.f;
-      }
+      for (int k = 0; k < kIn; ++k) xr[i][k] = __shfl(xa[k], src, 64);
+      const float w = __shfl(wa, src, 64), t = __shfl(ta, src, 64);
       const float e = f.y[i] - t;
       dy[i] = 2.f * w * e * inv_norm;
       if (r == 0) loss += w * e * e * inv_norm;
@@ -274,8 +307,9 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float z = f.z2[t][i];
-        dw3[t] += silu(z) * dy[i];
-        const float d = dy[i] * w3 * silu_grad(z);
+        const float sg = f.s2[t][i];
+        dw3[t] += z * sg * dy[i];
+        const float d = dy[i] * w3 * silu_grad_s(z, sg);
         dz2[t][i] = d;
         db2[t] += d;
       }
