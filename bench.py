@@ -210,9 +210,17 @@ def contract_loop(contracts, seed=11, h=1e-3):
   return run
 
 
-def secondary_metrics():
-  """One timed run each of the other configurations (not the headline)."""
+SECONDARY = ('gbm_euler_f64', 'batch_4096', 'learned_lsm_5asset',
+             'hw_bermudan_2x2e20', 'basket16_greeks')
+
+
+def secondary_metrics(only=None):
+  """One timed run each of the other configurations (not the headline), or
+  of the one named by `only`."""
   import torch
+
+  def want(name):
+    return only is None or only == name
   from tqf.engine import batch as batch_lib
   from tqf.engine import greeks
   from tqf.engine import euler as engine
@@ -232,6 +240,21 @@ def secondary_metrics():
     return a.elapsed_time(b) / 1e3 / reps, r
 
   # plain method: GBM Euler float64, 1e5 paths x 100 steps, 8 calls
+  if want('gbm_euler_f64'):
+    _gbm_euler(out, timed, pricing, engine)
+  if want('batch_4096'):
+    _batch(out, timed, batch_lib)
+  if want('learned_lsm_5asset'):
+    _learned_lsm(out)
+  if want('hw_bermudan_2x2e20'):
+    _hw_bermudan(out)
+  if want('basket16_greeks'):
+    _basket16(out, timed, greeks)
+  return out
+
+
+def _gbm_euler(out, timed, pricing, engine):
+  import torch
   all_times, keep = pricing.heston_grid_for([1.0], 100, 'reference',
                                             torch.float64)
   c1 = pricing.european_calls(('gbm', 0.05, 0.2), [100.0],
@@ -242,6 +265,9 @@ def secondary_metrics():
   t, _ = timed(c1.launch, 10)
   out['gbm_euler_f64'] = {'value': 1e7 / t, 'unit': 'path-steps/s',
                           'seconds': t}
+
+
+def _batch(out, timed, batch_lib):
   # batched heterogeneous contracts with Greeks
   cs = batch_lib.random_contracts(4096)
   cfg, tune = batch_lib.autotune(cs, cache_path=None)
@@ -270,6 +296,10 @@ def secondary_metrics():
   out['batch_4096']['loop_fd_greeks_seconds'] = t_greeks
   out['batch_4096']['speedup_vs_loop_fd_greeks'] = t_greeks / t
   out['batch_4096']['speedup_vs_loop_price_only'] = t_price / t
+
+
+def _learned_lsm(out):
+  import torch
   # learned LSM at full size: 5-asset Bermudan max-call, 2^20 training +
   # 2^20 pricing paths, 9 dates x 20 steps, MLP vs degree-2 polynomial
   from tqf.engine import mlp_lsm
@@ -285,6 +315,10 @@ def secondary_metrics():
       'gain': res['gain'], 'gain_se': res['gain_se'],
       'accepted': res['accepted']}
   del paths
+
+
+def _hw_bermudan(out):
+  import torch
   # Hull-White Bermudan swaptions (5nc1 and 5nc2, semiannual, a = 3%,
   # sigma = 1%, flat 1% curve), 2^20 paths: short rate in the fused kernel,
   # regression in the native LSM with per-path discount factors
@@ -316,6 +350,9 @@ def secondary_metrics():
   t = time.perf_counter() - t0
   out['hw_bermudan_2x2e20'] = {'seconds': t, 'prices': prices.tolist(),
                                'reference_values': [1.8892, 1.6633]}
+
+
+def _basket16(out, timed, greeks):
   # 16-asset basket with adjoint Greeks
   d = 16
   rs = np.random.RandomState(3)
@@ -327,7 +364,6 @@ def secondary_metrics():
       np.full(d, 100.0), sig, corr, 0.02, 1.0, 100.0, 64, 262144), 3)
   out['basket16_greeks'] = {'value': 262144 * 64 / t, 'unit': 'path-steps/s',
                             'seconds': t, 'greeks': 33}
-  return out
 
 
 def main():
@@ -341,6 +377,9 @@ def main():
   ap.add_argument('--no-check', action='store_true')
   ap.add_argument('--quick', action='store_true',
                   help='headline workload only (no secondary metrics)')
+  ap.add_argument('--only', choices=SECONDARY, default=None,
+                  help='run only this secondary workload and print its '
+                       'result (for profiling)')
   ap.add_argument('--calib-copy', type=int, default=0, metavar='MIB',
                   help='only run 5 device-to-device copies of MIB MiB (the '
                        'known-size kernel that scales HBM counters)')
@@ -366,6 +405,9 @@ def main():
 
   dev = torch.device('cuda', 0)
   torch.cuda.set_device(dev)
+  if args.only:
+    print(json.dumps({args.only: secondary_metrics(args.only)[args.only]}))
+    return
   launch = pricing.heston_ft_calls(num_paths=args.paths,
                                    num_time_steps=NUM_STEPS,
                                    maturities=MATURITIES, grid='reference',

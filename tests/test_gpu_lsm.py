@@ -250,3 +250,43 @@ def test_table_payoff_and_path_discounts_native_matches_torch_and_oracle():
   np.testing.assert_allclose(_np(native), want, rtol=2e-6)
   np.testing.assert_allclose(_np(generic), want, rtol=2e-6)
   assert native.shape == (2,)
+
+
+@pytest.mark.gpu
+def test_total_degree_table_takes_the_dense_gram_and_basis_order_update():
+  """A total-degree exponent table (5 assets, degree 2: 21 monomials) is
+  cheaper as dense Gram tiles than as moments of the 5^5 box, and cheaper as
+  a basis-order sum than as the 3^5 dense coefficient tensor, so it runs the
+  other Gram and update kernels than the tensor grids of the other tests:
+  its price matches the oracle with the same monomials."""
+  import itertools
+  rs = np.random.RandomState(21)
+  n, d = 20000, 5
+  times = np.linspace(0.25, 1.0, 4)
+  z = rs.standard_normal((n, 4, d))
+  paths = np.concatenate(
+      [ogbm.sample_paths(times, 0.02, 0.25 + 0.02 * j, 100.0, z[..., j:j + 1])
+       for j in range(d)], -1)
+  paths = np.concatenate([np.full((n, 1, d), 100.0), paths], 1)
+  df = np.exp(-0.02 * times)
+  table = np.array([e for e in itertools.product(range(3), repeat=d)
+                    if sum(e) <= 2], np.uint8)
+  assert table.shape == (21, d)
+
+  def basis(sample_paths, time_index):
+    sl = np.asarray(sample_paths)[None][:, :, time_index, :]
+    c = sl - sl.mean(axis=1, keepdims=True)
+    out = np.prod(c[:, :, None, :]**table[None, None].astype(float), axis=-1)
+    return np.transpose(out, (0, 2, 1))
+
+  want = olsm.least_square_mc(paths, [1, 2, 3, 4],
+                              olsm.make_basket_put_payoff([100.0]), basis,
+                              discount_factors=df)
+  from tqf import _native
+  dev = _native.device()
+  x = torch.as_tensor(paths, dtype=torch.float64, device=dev)
+  pspec = lsm.make_basket_put_payoff([100.0])._tqf_payoff
+  got = lsm._native_lsm(x, [1, 2, 3, 4], pspec, 2,
+                        lsm._padded_discounts(df, 4, torch.float64), None,
+                        torch.float64, dev, exponents=table)
+  np.testing.assert_allclose(_np(got), want, rtol=2e-6)

@@ -149,7 +149,9 @@ def least_square_mc(sample_paths, exercise_times, payoff_fn, basis_fn,
 
 
 def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
-                return_details=False, basis_log=False):
+                return_details=False, basis_log=False, exponents=None):
+  """exponents: an optional [K, d] table of monomial exponents (each at
+  most `degree`) in place of the tensor grid of make_polynomial_basis."""
   kind, strikes = pspec
   paths_b = x.unsqueeze(0) if x.dim() == 3 else x  # [Bp, N, T, d]
   bp = paths_b.shape[0]
@@ -160,7 +162,13 @@ def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
   n_batch = max(bp, strikes.shape[0], df.shape[-1])
   per_path = df.ndim == 3 and df.shape[1] > 1
   d = paths_b.shape[-1]
-  expo = exponent_grid(degree, d).astype(np.uint8)
+  if exponents is None:
+    expo = exponent_grid(degree, d).astype(np.uint8)
+  else:
+    expo = np.ascontiguousarray(exponents, np.uint8)
+    if expo.ndim != 2 or expo.shape[1] != d or expo.max() > degree:
+      raise ValueError('exponent table must be [K, %d] with entries <= %d'
+                       % (d, degree))
   k = expo.shape[0]
   if k > 255 or d > 8:
     raise NotImplementedError('the GPU LSM supports at most 8 assets and 255 '
