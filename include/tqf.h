@@ -370,6 +370,15 @@ int tqf_mlp_train_step(float* params, float* adam_m, float* adam_v,
 #define TQF_GREEKS_ADJOINT 0  /* Heston: reverse sweep over checkpoints     */
 #define TQF_GREEKS_TANGENT 1  /* Heston: five tangents in the forward pass  */
 
+/* tqf_batch_price flags */
+#define TQF_BATCH_COUNTER 1       /* items from an atomic counter (else a
+                                     static round robin)                    */
+#define TQF_BATCH_TANGENT_ONLY 2  /* no reverse sweep in the launch: Heston
+                                     contracts get forward-tangent Greeks
+                                     whatever their greeks_mode (the same
+                                     pathwise estimator); fewer registers,
+                                     more waves per SIMD                    */
+
 typedef struct tqf_contract {
   int32_t kind;
   int32_t n_steps;      /* uniform grid dt = maturity / n_steps             */
@@ -408,7 +417,7 @@ int tqf_batch_price(const tqf_contract* contracts, int32_t n_contracts,
                     int32_t n_items,
                     const int32_t* contract_first_item,
                     const int32_t* contract_n_items, uint64_t seed,
-                    int32_t n_blocks, int32_t max_steps, int32_t use_counter,
+                    int32_t n_blocks, int32_t max_steps, int32_t flags,
                     void* workspace, double* out, void* stream);
 
 /* ---- Basket call with adjoint Greeks ------------------------------------ */

@@ -267,6 +267,10 @@ __device__ void heston_path_tangent(const Contract& c, int64_t p,
 // [n_items][2 * kStats].
 constexpr int kStats = 8;
 
+// kAdjoint false: the kernel has no reverse sweep (a Heston contract that
+// asks for adjoint Greeks gets the forward tangents, the same pathwise
+// estimator), so it needs far fewer registers and runs more waves per SIMD.
+template <bool kAdjoint>
 __global__ void __launch_bounds__(kBlockB)
 batch_kernel(const tqf_contract* __restrict__ contracts,
              const tqf_work_item* __restrict__ items,
@@ -299,7 +303,7 @@ batch_kernel(const tqf_contract* __restrict__ contracts,
       float g[kStats];
       if (c.kind == TQF_CONTRACT_GBM)
         gbm_path(c, p, c.stream, k0, k1, g);
-      else if (c.greeks_mode == TQF_GREEKS_TANGENT)
+      else if (!kAdjoint || c.greeks_mode == TQF_GREEKS_TANGENT)
         heston_path_tangent(c, p, c.stream, k0, k1, g);
       else
         heston_path(c, p, c.stream, k0, k1, ck, kBlockB, g);
@@ -369,7 +373,7 @@ extern "C" int tqf_batch_price(const tqf_contract* contracts,
                                const int32_t* contract_first_item,
                                const int32_t* contract_n_items, uint64_t seed,
                                int32_t n_blocks, int32_t max_steps,
-                               int32_t use_counter, void* workspace,
+                               int32_t flags, void* workspace,
                                double* out, void* stream) {
   clear_error();
   TQF_REQUIRE(contracts && items && contract_first_item && contract_n_items &&
@@ -388,9 +392,17 @@ extern "C" int tqf_batch_price(const tqf_contract* contracts,
   const int64_t nseg = (max_steps + kSeg - 1) / kSeg;
   const int64_t per_block = (int64_t)kBlockB * nseg * 2;
   hipLaunchKernelGGL(zero_counter_kernel, 1, 1, 0, s, counter);
-  hipLaunchKernelGGL(batch_kernel, n_blocks, kBlockB, 0, s, contracts, items,
-                     order, n_items, (uint32_t)seed, (uint32_t)(seed >> 32), counter,
-                     ckpt, per_block, item_sums, use_counter);
+  const int use_counter = flags & TQF_BATCH_COUNTER;
+  if (flags & TQF_BATCH_TANGENT_ONLY)
+    hipLaunchKernelGGL(batch_kernel<false>, n_blocks, kBlockB, 0, s, contracts,
+                       items, order, n_items, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), counter, ckpt, per_block,
+                       item_sums, use_counter);
+  else
+    hipLaunchKernelGGL(batch_kernel<true>, n_blocks, kBlockB, 0, s, contracts,
+                       items, order, n_items, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), counter, ckpt, per_block,
+                       item_sums, use_counter);
   hipLaunchKernelGGL(batch_finalize_kernel, (n_contracts + 255) / 256, 256, 0,
                      s, item_sums, contract_first_item, contract_n_items,
                      contracts, n_contracts, out);

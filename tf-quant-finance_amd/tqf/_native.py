@@ -16,6 +16,8 @@ TQF_F32, TQF_F64 = 0, 1
 MODEL_GBM, MODEL_CONST, MODEL_HESTON_FT, MODEL_LOCAL_VOL, MODEL_HESTON_QE = (
     0, 1, 2, 3, 4)
 MODEL_HULL_WHITE = 5
+BATCH_COUNTER = 1
+BATCH_TANGENT_ONLY = 2
 RNG_DRAWS, RNG_PHILOX, RNG_PHILOX_ANTITHETIC, RNG_SOBOL, RNG_SOBOL_SHIFTED = (
     0, 1, 2, 3, 4)
 OUT_PATH_MAJOR, OUT_STEP_MAJOR, OUT_PAYOFF, OUT_PAYOFF_PARTIALS = 0, 1, 2, 3

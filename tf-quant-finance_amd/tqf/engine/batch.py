@@ -144,6 +144,10 @@ class BatchPlan:
                                                 self.max_steps)
     self._ws = torch.empty(ws // 8 + 1, dtype=torch.float64, device=self.dev)
     self.out = torch.empty((n, 16), dtype=torch.float64, device=self.dev)
+    # a tangent plan launches the kernel without the reverse sweep
+    self.flags = ((_native.BATCH_COUNTER if self.use_counter else 0) |
+                  (_native.BATCH_TANGENT_ONLY if mode == GREEKS_TANGENT
+                   else 0))
     self.total_path_steps = int(sum(c['n_paths'] * c['n_steps']
                                     for c in self.contracts))
     flop = (FLOP_PER_PATH_STEP_TANGENT if mode == GREEKS_TANGENT else
@@ -161,7 +165,7 @@ class BatchPlan:
                  ctypes.c_void_p(self._count.data_ptr()),
                  ctypes.c_uint64(self.seed), ctypes.c_int32(self.n_blocks),
                  ctypes.c_int32(self.max_steps),
-                 ctypes.c_int32(int(self.use_counter)),
+                 ctypes.c_int32(self.flags),
                  ctypes.c_void_p(self._ws.data_ptr()),
                  ctypes.c_void_p(self.out.data_ptr()),
                  _native.stream_ptr(self.dev))
