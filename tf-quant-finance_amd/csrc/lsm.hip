@@ -298,6 +298,7 @@ lsm_gram_reduce_kernel(const double* __restrict__ partial, int n_chunks,
 // tensor basis of 5 assets at degree 2 this is a 32 x 160 product per path
 // in place of the 244 x 244 Gram.
 struct MomentPlan {
+  int small;      // 1: every moment in registers (lsm_moment_small_kernel)
   int dA;         // dimensions in the row group
   int radix;      // 2 deg + 1
   int ra, rap;    // rows radix^dA, padded to a multiple of 16
@@ -513,6 +514,58 @@ lsm_moment_kernel(LsmArgs a, MomentPlan mp, const double* __restrict__ exv,
   }
 }
 
+// One asset: the moments are c^0 .. c^(2 deg) and y c^0 .. y c^deg
+// (5 + 3 at degree 2).  Each thread accumulates them for its paths in
+// registers by running products; a fixed-order workgroup sum gives one row
+// of partials [chunk][cbp].
+constexpr int kSmallMoments = 40;
+
+__global__ void __launch_bounds__(256)
+lsm_moment_small_kernel(LsmArgs a, MomentPlan mp,
+                        const double* __restrict__ exv,
+                        const double* __restrict__ y, int64_t n_calib,
+                        int64_t chunk, double* __restrict__ partial) {
+  __shared__ double red[4][kSmallMoments];
+  const int nm = mp.cbp, nx = mp.cbm;  // nx = 2 deg + 1 moments, then rhs
+  double acc[kSmallMoments];
+#pragma unroll
+  for (int m = 0; m < kSmallMoments; ++m) acc[m] = 0.0;
+  const int64_t p_begin = (int64_t)blockIdx.x * chunk;
+  int64_t p_end = p_begin + chunk;
+  if (p_end > n_calib) p_end = n_calib;
+  for (int64_t p = p_begin + threadIdx.x; p < p_end; p += 256) {
+    if (!(exv[p] > 0)) continue;  // in-the-money rows only
+    const double x = a.paths[p * a.n_times + a.t_idx];
+    const double c = (basis_input(a, x) - a.mean[0]) * a.scale[0];
+    double v = 1.0, u = y[p];
+#pragma unroll
+    for (int m = 0; m < kSmallMoments; ++m) {
+      if (m < nx) {
+        acc[m] += v;
+        v *= c;
+      } else if (m < nm) {
+        acc[m] += u;
+        u *= c;
+      }
+    }
+  }
+  // fixed-order sums: wave butterflies, then the four waves in order
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < kSmallMoments; ++m) {
+    if (m < nm) {
+      double v = acc[m];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[wave][m] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < nm)
+    partial[(int64_t)blockIdx.x * nm + threadIdx.x] =
+        ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
+        red[3][threadIdx.x];
+}
+
 // Sums the chunk partials of the moment product (fixed order).
 __global__ void __launch_bounds__(256)
 lsm_moment_reduce_kernel(const double* __restrict__ partial, int n_chunks,
@@ -572,6 +625,19 @@ lsm_moment_gram_kernel(const double* __restrict__ m, MomentPlan mp,
 bool moment_plan(int d, int deg, int n_pairs, MomentPlan* out) {
   const int R = 2 * deg + 1;
   if (d * R > 256) return false;
+  if (d == 1 && R + deg + 1 <= kSmallMoments) {
+    // one asset: every moment in registers
+    MomentPlan p{};
+    p.small = 1;
+    p.dA = 0;
+    p.radix = R;
+    p.ra = p.rap = 1;
+    p.cbm = R;
+    p.cby = deg + 1;
+    p.cbp = R + deg + 1;
+    *out = p;
+    return true;
+  }
   bool found = false;
   MomentPlan best{};
   int64_t ra = 1;
@@ -583,7 +649,7 @@ bool moment_plan(int d, int deg, int n_pairs, MomentPlan* out) {
       cby *= deg + 1;
     }
     if (cbm + cby > 256) continue;
-    MomentPlan p;
+    MomentPlan p{};
     p.dA = dA;
     p.radix = R;
     p.ra = (int)ra;
@@ -1172,8 +1238,12 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
     if (use_moments) {
       const int size = mp.rap * mp.cbp;
       double* msum = partial + (int64_t)n_chunks * size;
-      hipLaunchKernelGGL(lsm_moment_kernel, n_chunks, kMomThreads, mp.lds, s,
-                         a, mp, exv, y, n_calib, chunk, partial);
+      if (mp.small)
+        hipLaunchKernelGGL(lsm_moment_small_kernel, n_chunks, 256, 0, s, a,
+                           mp, exv, y, n_calib, chunk, partial);
+      else
+        hipLaunchKernelGGL(lsm_moment_kernel, n_chunks, kMomThreads, mp.lds,
+                           s, a, mp, exv, y, n_calib, chunk, partial);
       hipLaunchKernelGGL(lsm_moment_reduce_kernel, (size + 255) / 256, 256, 0,
                          s, partial, n_chunks, size, msum);
       hipLaunchKernelGGL(lsm_moment_gram_kernel, (kext * kext + 255) / 256,

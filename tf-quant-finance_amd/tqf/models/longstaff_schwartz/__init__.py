@@ -102,7 +102,19 @@ def make_table_payoff(values):
 
 
 def _padded_discounts(discount_factors, n_ex, dtype):
-  """Reference padding/transposition (lsm.py:239-256) -> [n_ex + 1, *, B]."""
+  """Reference padding/transposition (lsm.py:239-256) -> [n_ex + 1, *, B].
+  A torch tensor stays a tensor on its device (per-path factors are as
+  large as the paths); anything else becomes a numpy array."""
+  if torch.is_tensor(discount_factors):
+    df = discount_factors.to(torch.float64)
+    if df.ndim == 0:
+      df = df.reshape(1, 1, 1)
+    elif df.ndim == 1:
+      df = df.reshape(1, 1, -1)
+    elif df.ndim == 2:
+      df = df.reshape(1, -1)
+    df = torch.nn.functional.pad(df, (1, 0), value=1.0)
+    return df.permute((df.ndim - 1,) + tuple(range(df.ndim - 1)))
   if discount_factors is None:
     df = np.ones((1, 1, n_ex))
   else:
@@ -202,16 +214,20 @@ def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
       keep.append(tb)
       l.payoff_table = tb.data_ptr()
     if per_path:
-      pdb = torch.as_tensor(
-          np.ascontiguousarray(df[:, :, b if df.shape[2] > 1 else 0]),
-          dtype=torch.float64, device=dev).contiguous()  # [n_ex + 1, N]
+      col = df[:, :, b if df.shape[2] > 1 else 0]  # [n_ex + 1, N]
+      if torch.is_tensor(col):
+        pdb = col.to(device=dev, dtype=torch.float64).contiguous()
+      else:
+        pdb = torch.as_tensor(np.ascontiguousarray(col), dtype=torch.float64,
+                              device=dev).contiguous()
       keep.append(pdb)
       l.path_discounts = pdb.data_ptr()
       dfb = np.ones(df.shape[0])
     else:
+      dfn = df.cpu().numpy() if torch.is_tensor(df) else df
       dfb = np.ascontiguousarray(
-          np.broadcast_to(df, df.shape[:-1] + (n_batch,))[..., b].reshape(
-              df.shape[0], -1)[:, 0], np.float64)
+          np.broadcast_to(dfn, dfn.shape[:-1] + (n_batch,))[..., b].reshape(
+              dfn.shape[0], -1)[:, 0], np.float64)
     res = torch.empty(2, dtype=torch.float64, device=dev)
     ranks = torch.zeros(max(len(ex) - 1, 1), dtype=torch.int32, device=dev)
     _native.call('tqf_lsm_price', ctypes.byref(l), exi,
