@@ -26,6 +26,7 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = kMlpIn, kH = 64;
+constexpr int kG1Stride = 68;
 constexpr int oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH,
               oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH;
 static_assert(oB3 + 1 == kMlpParams, "parameter layout");
@@ -41,13 +42,9 @@ __device__ __forceinline__ short to_bf16(float f) {
 __device__ __forceinline__ float sigmoid(float z) {
   return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
 }
-__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 // SiLU'(z) from z and its sigmoid s
 __device__ __forceinline__ float silu_grad_s(float z, float s) {
   return s * (1.0f + z * (1.0f - s));
-}
-__device__ __forceinline__ float silu_grad(float z) {
-  return silu_grad_s(z, sigmoid(z));
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -85,6 +82,7 @@ __device__ __forceinline__ float sum16(float v) {
 // z2 (C layout, 16), y (4 samples of the lane's row block).
 struct Fwd {
   float z1a[4][4];    // [chunk c][j]: sample l%16, feature c*16+4*(l/16)+j
+  float s1a[4][4];    // sigmoid(z1a), same layout
   bf16x4 h1a[4];      // A operands of layer 2
   f32x4 z2[4];        // [n-tile t]: rows 4*(l/16)+i, column t*16 + l%16
   f32x4 s2[4];        // sigmoid(z2), same layout
@@ -107,7 +105,9 @@ __device__ __forceinline__ void forward_group(const float* __restrict__ P,
 #pragma unroll
       for (int i = 0; i < kIn; ++i) z += xa[i] * P[oW1 + i * kH + fo];
       f.z1a[c][j] = z;
-      h[j] = to_bf16(silu(z));
+      const float sg = sigmoid(z);
+      f.s1a[c][j] = sg;
+      h[j] = to_bf16(z * sg);
     }
     f.h1a[c] = bf16x4{h[0], h[1], h[2], h[3]};
   }
@@ -242,6 +242,9 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
   __shared__ float P[kMlpParams];
   __shared__ float Wsum[4][kMlpParams + 1];  // per-wave gradient rows
   __shared__ short tr[4][16 * 64];  // per-wave transpose buffer (bf16)
+  // per-wave SiLU'(z1) of the group, [sample][feature] with a row stride of
+  // 68 floats (16-byte rows, the four row groups of a read in distinct banks)
+  __shared__ float g1s[4][16 * kG1Stride];
   stage_params(params, P, nullptr);
   __syncthreads();
   bf16x4 w2b[16], w2t[16];
@@ -298,6 +301,17 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
       dy[i] = 2.f * w * e * inv_norm;
       if (r == 0) loss += w * e * e * inv_norm;
       db3 += r == 0 ? dy[i] : 0.f;
+    }
+    // SiLU'(z1) from the forward pass, to the C layout of the layer-1
+    // backward through LDS (in place of recomputing z1 and its sigmoid)
+    float* G1 = g1s[wave];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      f32x4 gv;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        gv[j] = silu_grad_s(f.z1a[c][j], f.s1a[c][j]);
+      *(f32x4*)(G1 + r * kG1Stride + c * 16 + 4 * q) = gv;
     }
     // layer 3 and SiLU' of layer 2: dZ2 in C layout (rows 4q+i, col t*16+r)
     f32x4 dz2[4];
@@ -365,11 +379,7 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
       const int fo = t * 16 + r;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        // recompute z1 for (sample 4q+i, feature fo)
-        float z = P[oB1 + fo];
-#pragma unroll
-        for (int k = 0; k < kIn; ++k) z += xr[i][k] * P[oW1 + k * kH + fo];
-        const float d = dh[i] * silu_grad(z);
+        const float d = dh[i] * G1[(4 * q + i) * kG1Stride + fo];
         db1[t] += d;
 #pragma unroll
         for (int k = 0; k < kIn; ++k) dw1[k][t] += xr[i][k] * d;
