@@ -230,3 +230,19 @@ def test_sobol_scrambling_host_matches_oracle_and_keeps_the_net():
     for m in (1, 4, 10):
       blk = (pts[:2**m, d] >> (nd - m)).astype(np.int64)
       assert sorted(blk) == list(range(2**m))
+
+
+@pytest.mark.parametrize('shape', [(), (4,), (3, 4), (5, 1, 4), (5, 7, 4)])
+def test_padded_discounts_tensor_and_array_agree(shape):
+  """Discount factors given as a tensor are padded and transposed on their
+  device exactly as the array form is (the reference's layout,
+  lsm.py:239-256): [n_ex + 1, *, B] with a leading row of ones."""
+  from tqf.models import longstaff_schwartz as lsm
+  rs = np.random.RandomState(4)
+  df = rs.uniform(0.8, 1.0, shape)
+  want = lsm._padded_discounts(df, 4, torch.float64)
+  got = lsm._padded_discounts(torch.as_tensor(df), 4, torch.float64)
+  assert torch.is_tensor(got)
+  assert tuple(got.shape) == want.shape
+  np.testing.assert_array_equal(got.numpy(), want)
+  np.testing.assert_array_equal(want[0], np.ones_like(want[0]))
