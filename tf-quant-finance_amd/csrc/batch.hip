@@ -77,15 +77,29 @@ __device__ __forceinline__ void gbm_path(const Contract& c, int64_t p,
   const float drift = (c.r - 0.5f * c.sigma * c.sigma) * dt;
   float x = 0.f, w = 0.f;
   float z[4];
-  for (int i = 0; i < c.n_steps; ++i) {
-    if ((i & 3) == 0)
-      normals4((uint32_t)p, (uint32_t)(p >> 32), (uint32_t)(i >> 2), stream, k0,
-               k1, z);
-    const float zi = (i & 3) == 0 ? z[0] : (i & 3) == 1 ? z[1]
-                                       : (i & 3) == 2 ? z[2] : z[3];
-    const float dw = zi * sdt;
-    x = (x + drift) + c.sigma * dw;
-    w += dw;
+  // four steps per Philox block (block i / 4 serves steps i .. i + 3)
+  int i = 0;
+  for (; i + 4 <= c.n_steps; i += 4) {
+    normals4((uint32_t)p, (uint32_t)(p >> 32), (uint32_t)(i >> 2), stream, k0,
+             k1, z);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float dw = z[u] * sdt;
+      x = (x + drift) + c.sigma * dw;
+      w += dw;
+    }
+  }
+  if (i < c.n_steps) {
+    normals4((uint32_t)p, (uint32_t)(p >> 32), (uint32_t)(i >> 2), stream, k0,
+             k1, z);
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      if (i + u < c.n_steps) {
+        const float dw = z[u] * sdt;
+        x = (x + drift) + c.sigma * dw;
+        w += dw;
+      }
+    }
   }
   const float s = c.s0 * __expf(x);
   const float disc = __expf(-c.r * c.t);
