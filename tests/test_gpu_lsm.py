@@ -65,8 +65,15 @@ def test_lsm_duplicate_assets_equal_one_asset():
   assert a.shape == (3,)
 
 
+# (assets, degree) pairs cover the one-asset register moments and the MFMA
+# moment product with row groups of 0, 1 and 2 dimensions and padded tiles:
+# (1, 3) registers; (2, 2) 5 x 8 of 16 x 16; (3, 2) 25 x 8 of 32 x 16;
+# (2, 5) 11 x 17 of 16 x 32; (3, 1) 3 x 13 of 16 x 16; (4, 2) 25 x 34 of
+# 32 x 48
 @pytest.mark.parametrize('dim,degree,kind', [(1, 3, 'put'), (2, 2, 'put'),
-                                             (3, 2, 'max_call')])
+                                             (3, 2, 'max_call'),
+                                             (2, 5, 'put'), (3, 1, 'max_call'),
+                                             (4, 2, 'max_call')])
 def test_lsm_matches_oracle_on_simulated_paths(dim, degree, kind):
   rs = np.random.RandomState(dim)
   n, times = 20000, np.linspace(0.25, 1.0, 4)
