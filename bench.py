@@ -211,7 +211,8 @@ def contract_loop(contracts, seed=11, h=1e-3):
 
 
 SECONDARY = ('gbm_euler_f64', 'batch_4096', 'learned_lsm_5asset',
-             'hw_bermudan_2x2e20', 'basket16_greeks')
+             'hw_bermudan_2x2e20', 'basket16_greeks', 'paths_and_points')
+HBM_PEAK = 8e12  # bytes/s
 
 
 def secondary_metrics(only=None):
@@ -250,6 +251,8 @@ def secondary_metrics(only=None):
     _hw_bermudan(out)
   if want('basket16_greeks'):
     _basket16(out, timed, greeks)
+  if want('paths_and_points'):
+    _paths_and_points(out, timed, engine)
   return out
 
 
@@ -350,6 +353,90 @@ def _hw_bermudan(out):
   t = time.perf_counter() - t0
   out['hw_bermudan_2x2e20'] = {'seconds': t, 'prices': prices.tolist(),
                                'reference_values': [1.8892, 1.6633]}
+
+
+def _paths_and_points(out, timed, engine):
+  """The other call surfaces at full size, one warm call then timed
+  calls: the C2 scheme in path-store mode (every step of (log S, v) written
+  coalesced [step][d][N], HBM-bound), sobol.sample of the C2 draw set,
+  HestonModel (QE) and GeometricBrownianMotion (exact) sample_paths,
+  Halton points and the pricing platform's American options."""
+  import torch
+  import tqf
+  from tqf.engine import pricing
+  from tqf.experimental import american_option
+  from tqf.math.random import halton
+  from tqf.math.random import sobol as tsobol
+  res = {}
+  n = 2**20
+  # path store: 2^20 paths x 365 steps, all 366 records of (log S, v) FP32
+  all_times = np.arange(366) / 365.0
+  keep = np.ones(366, bool)
+  rng = engine.RngSpec('sobol_shifted',
+                       shift=pricing.sobol_digital_shift(730, 7),
+                       first_index=0)
+  lp = engine.prepare(('heston', 0.03, 2.0, 0.04, 0.3, -0.7, 0.0), all_times,
+                      keep, 366, np.array([np.log(100.0), 0.04]), n, rng,
+                      torch.float32, mode='store')
+  lp.launch()
+  t, _ = timed(lp.launch, 3)
+  nbytes = 366 * 2 * n * 4
+  res['pathstore_heston_ft'] = {
+      'value': nbytes / t / 1e9, 'unit': 'GB/s', 'seconds': t,
+      'bytes_written': nbytes, 'frac_hbm_peak': nbytes / t / HBM_PEAK,
+      'path_steps_per_s': n * 365 / t}
+  del lp
+  torch.cuda.empty_cache()
+  # sobol.sample(730, 2^20) float32 (the C2 draw set, 3.06 GB)
+  tsobol.sample(730, n)
+  t, _ = timed(lambda: tsobol.sample(730, n), 3)
+  nbytes = 730 * n * 4
+  res['sobol_sample_730x2e20'] = {
+      'value': nbytes / t / 1e9, 'unit': 'GB/s (output)', 'seconds': t,
+      'frac_hbm_peak': nbytes / t / HBM_PEAK}
+  torch.cuda.empty_cache()
+  # HestonModel.sample_paths (QE), 2^20 paths, 365 steps to T = 1
+  hm = tqf.models.HestonModel(2.0, 0.04, 0.3, -0.7)
+  qe = lambda: hm.sample_paths([1.0], [np.log(100.0), 0.04], num_samples=n,
+                               num_time_steps=365, seed=5)
+  qe()
+  t, _ = timed(qe, 3)
+  res['heston_qe_sample_paths'] = {'value': n * 365 / t,
+                                   'unit': 'path-steps/s', 'seconds': t}
+  # GeometricBrownianMotion.sample_paths (exact), 2^20 paths x 100 dates
+  gm = tqf.models.GeometricBrownianMotion(0.05, 0.2)
+  gt = np.linspace(0.01, 1.0, 100)
+  ge = lambda: gm.sample_paths(gt, initial_state=100.0, num_samples=n,
+                               seed=42)
+  ge()
+  t, _ = timed(ge, 3)
+  res['gbm_exact_sample_paths'] = {'value': n * 100 / t,
+                                   'unit': 'path-steps/s', 'seconds': t,
+                                   'output_GB_per_s': n * 100 * 4 / t / 1e9}
+  torch.cuda.empty_cache()
+  # Halton, 100 dimensions x 2^20 points (plain)
+  ha = lambda: halton.sample(100, num_results=n, randomized=False)
+  ha()
+  t, _ = timed(ha, 3)
+  res['halton_100x2e20'] = {'value': 100 * n / t, 'unit': 'point-dims/s',
+                            'seconds': t}
+  # pricing platform American options: 4 options, 2^20 paths, 100 dates
+  spots = [90.0, 100.0, 110.0, 100.0]
+  exp_t = [1.0, 1.0, 1.0, 2.0]
+
+  def am():
+    return american_option.bs_lsm_price(
+        spots, exp_t, [100.0] * 4, [0.2, 0.25, 0.3, 0.2],
+        [float(np.exp(-0.05 * e)) for e in exp_t], num_samples=n,
+        num_exercise_times=100, is_call_option=False, seed=(1, 2))
+  am()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  pr = am().cpu().numpy()
+  t = time.perf_counter() - t0
+  res['american_bs_lsm_4x2e20'] = {'value': 4 / t, 'unit': 'options/s',
+                                   'seconds': t, 'prices': pr.tolist()}
+  out['paths_and_points'] = res
 
 
 def _basket16(out, timed, greeks):

@@ -256,9 +256,12 @@ __device__ __forceinline__ void record(const EulerArgs<R>& a, R* lds_pay,
       for (int c = 0; c < D; ++c)
         a.paths[(path * a.n_slots + slot) * D + c] = y[c];
     } else {
+      // coalesced rows, written once and not read back here: streaming
+      // (nontemporal) stores
 #pragma unroll
       for (int c = 0; c < D; ++c)
-        a.paths[((int64_t)slot * D + c) * a.n_paths + path] = y[c];
+        __builtin_nontemporal_store(
+            y[c], &a.paths[((int64_t)slot * D + c) * a.n_paths + path]);
     }
   }
 }
