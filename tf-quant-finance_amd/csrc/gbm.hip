@@ -32,7 +32,17 @@ struct GbmArgs {
   int mode;
   R* out;
   int64_t n_paths;
+  int stage_off;   // LDS byte offset of the path-major staging buffer
+  int stage_recs;  // records per flush (stage_recs * d <= kStageWords)
 };
+
+// Path-major output ([N][k][d]) is staged: a block keeps up to kStageWords
+// values per path in LDS ([path][kStageStride], odd stride: the per-path
+// writes and the flush's reads fall in distinct banks) and flushes them as
+// one contiguous segment per path, in place of one scattered store per
+// path and record.
+constexpr int kStageWords = 32;
+constexpr int kStageStride = kStageWords + 1;
 
 template <typename R, int RNG, int MAXD>
 __global__ void __launch_bounds__(kPathBlock) gbm_exact_kernel(GbmArgs<R> a) {
@@ -51,18 +61,48 @@ __global__ void __launch_bounds__(kPathBlock) gbm_exact_kernel(GbmArgs<R> a) {
   Normals<R, RNG> nrm;
   nrm.init(a.rng, pc, d, lds, a.n_steps);
 
+  const bool stage = a.mode == TQF_OUT_PATH_MAJOR;
+  R* buf = (R*)(lds + a.stage_off);
+  int nrec = 0, first_slot = 0;  // block-uniform
+  // all threads of the block: write the staged records of every path as
+  // one segment of slots [first_slot, first_slot + nrec)
+  auto flush = [&]() {
+    __syncthreads();
+    const int seg = nrec * d;
+    const int64_t p0 = (int64_t)blockIdx.x * kPathBlock;
+    for (int e = threadIdx.x; e < kPathBlock * seg; e += kPathBlock) {
+      const int p = e / seg, rem = e - p * seg;
+      const int64_t pg = p0 + p;
+      if (pg < a.n_paths)
+        a.out[(pg * a.n_slots + first_slot) * d + rem] =
+            buf[p * kStageStride + rem];
+    }
+    __syncthreads();
+    nrec = 0;
+  };
   auto record = [&](int slot, bool initial) {
-    if (!active) return;
+    if (!stage) {
+      if (!active) return;
 #pragma unroll
-    for (int c = 0; c < MAXD; ++c) {
-      if (c < d) {
-        const R v = initial ? s0[c] : s0[c] * fast_exp(cum[c]);
-        if (a.mode == TQF_OUT_PATH_MAJOR)
-          a.out[(path * a.n_slots + slot) * d + c] = v;
-        else
+      for (int c = 0; c < MAXD; ++c) {
+        if (c < d) {
+          const R v = initial ? s0[c] : s0[c] * fast_exp(cum[c]);
           a.out[((int64_t)slot * d + c) * a.n_paths + path] = v;
+        }
+      }
+      return;
+    }
+    if (nrec > 0 && slot != first_slot + nrec) flush();
+    if (nrec == 0) first_slot = slot;
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < MAXD; ++c) {
+        if (c < d)
+          buf[threadIdx.x * kStageStride + nrec * d + c] =
+              initial ? s0[c] : s0[c] * fast_exp(cum[c]);
       }
     }
+    if (++nrec == a.stage_recs) flush();
   };
   if (a.init_slot >= 0) record(a.init_slot, true);
 
@@ -91,6 +131,7 @@ __global__ void __launch_bounds__(kPathBlock) gbm_exact_kernel(GbmArgs<R> a) {
     const int s = ld_uniform(a.slot, i);
     if (s >= 0) record(s, false);
   }
+  if (stage && nrec > 0) flush();
 }
 
 template <typename R, int MAXD>
@@ -140,7 +181,11 @@ int launch(const tqf_gbm* g, const tqf_grid* grid, const tqf_rng* r,
   a.mode = mode;
   a.out = (R*)out;
   a.n_paths = n_paths;
-  size_t lds = (size_t)rng_lds_bytes(r->kind, g->dim);
+  size_t lds = ((size_t)rng_lds_bytes(r->kind, g->dim) + 15) & ~(size_t)15;
+  a.stage_off = (int)lds;
+  a.stage_recs = kStageWords / g->dim;  // dim <= 16: at least 2
+  if (mode == TQF_OUT_PATH_MAJOR)
+    lds += (size_t)kPathBlock * kStageStride * sizeof(R);
   if (lds == 0) lds = 16;
   const unsigned blocks = (unsigned)((n_paths + kPathBlock - 1) / kPathBlock);
   if (g->dim <= 1) return launch_d<R, 1>(a, r->kind, blocks, lds, s);
