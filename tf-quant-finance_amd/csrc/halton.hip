@@ -31,29 +31,78 @@ struct HaltonArgs {
   int64_t stride;
 };
 
+// One base-b digit of k: c = k mod b, k /= b.  A 32-bit division when k
+// fits (the common case; the 64-bit one is a long software sequence).
+__device__ __forceinline__ int next_digit(uint64_t& k, uint32_t b) {
+  if (k <= 0xffffffffull) {
+    const uint32_t k32 = (uint32_t)k, q = k32 / b;
+    k = q;
+    return (int)(k32 - q * b);
+  }
+  const uint64_t q = k / b;
+  const int c = (int)(k - q * (uint64_t)b);
+  k = q;
+  return c;
+}
+
 __global__ void __launch_bounds__(256) halton_kernel(HaltonArgs a) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.n * (int64_t)a.dim) return;
-  const int64_t i = g / a.dim;
-  const int d = (int)(g - i * a.dim);
+  // (point, axis) of this thread: 32-bit arithmetic when the element
+  // count fits
+  int64_t i;
+  int d;
+  if (a.n * (int64_t)a.dim <= 0xffffffffll - 256) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int64_t)g >= a.n * (int64_t)a.dim) return;
+    const uint32_t i32 = g / (uint32_t)a.dim;
+    i = i32;
+    d = (int)(g - i32 * (uint32_t)a.dim);
+  } else {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.n * (int64_t)a.dim) return;
+    i = g / a.dim;
+    d = (int)(g - i * a.dim);
+  }
   uint64_t k = (uint64_t)((a.seq ? a.seq[i] : a.first + i) + 1);
-  const uint64_t b = (uint64_t)a.bases[d];
+  const uint32_t b = (uint32_t)a.bases[d];
   const int nd = a.max_digits[d];
-  double acc = 0.0, w = 1.0;
+  // sum_j c_j b^-(j+1) as R / B with the digit-reversed integer
+  // R = sum_j c_j b^(m-1-j) and B = b^m over the m digits taken (one
+  // rounding instead of one per digit); exact in 64 bits while
+  // m * bits(b) <= 63: always for plain points of an index below 2^32
+  // (m = the index's digit count), not for every randomised axis, which
+  // then adds digit by digit
+  const int bits = 32 - __builtin_clz(b);
+  double acc = 0.0;
   if (a.perms) {
     const int32_t* pd = a.perms + a.offsets[d];
-    for (int j = 0; j < nd; ++j) {
-      const uint64_t q = k / b;
-      const int c = (int)(k - q * b);
-      k = q;
-      w *= (double)b;
-      acc += (double)pd[(int64_t)j * a.radix_sum + c] / w;
+    if (nd * bits <= 63) {
+      uint64_t R = 0, B = 1;
+      for (int j = 0; j < nd; ++j) {
+        const int c = next_digit(k, b);
+        R = R * b + (uint64_t)pd[(int64_t)j * a.radix_sum + c];
+        B *= b;
+      }
+      acc = (double)R / (double)B;
+    } else {
+      double w = 1.0;
+      for (int j = 0; j < nd; ++j) {
+        const int c = next_digit(k, b);
+        w *= (double)b;
+        acc += (double)pd[(int64_t)j * a.radix_sum + c] / w;
+      }
     }
-  } else {
+  } else if (k <= 0xffffffffull) {
+    uint64_t R = 0, B = 1;
     for (int j = 0; j < nd && k; ++j) {
-      const uint64_t q = k / b;
-      const int c = (int)(k - q * b);
-      k = q;
+      const int c = next_digit(k, b);
+      R = R * b + (uint64_t)c;
+      B *= b;
+    }
+    acc = (double)R / (double)B;
+  } else {
+    double w = 1.0;
+    for (int j = 0; j < nd && k; ++j) {
+      const int c = next_digit(k, b);
       w *= (double)b;
       acc += (double)c / w;
     }
