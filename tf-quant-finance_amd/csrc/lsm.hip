@@ -520,16 +520,19 @@ lsm_moment_kernel(LsmArgs a, MomentPlan mp, const double* __restrict__ exv,
 // of partials [chunk][cbp].
 constexpr int kSmallMoments = 40;
 
+// NM: the accumulators a thread keeps (at least the plan's moment count),
+// so degree 2 holds 8 float64 sums in registers rather than 40.
+template <int NM>
 __global__ void __launch_bounds__(256)
 lsm_moment_small_kernel(LsmArgs a, MomentPlan mp,
                         const double* __restrict__ exv,
                         const double* __restrict__ y, int64_t n_calib,
                         int64_t chunk, double* __restrict__ partial) {
-  __shared__ double red[4][kSmallMoments];
+  __shared__ double red[4][NM];
   const int nm = mp.cbp, nx = mp.cbm;  // nx = 2 deg + 1 moments, then rhs
-  double acc[kSmallMoments];
+  double acc[NM];
 #pragma unroll
-  for (int m = 0; m < kSmallMoments; ++m) acc[m] = 0.0;
+  for (int m = 0; m < NM; ++m) acc[m] = 0.0;
   const int64_t p_begin = (int64_t)blockIdx.x * chunk;
   int64_t p_end = p_begin + chunk;
   if (p_end > n_calib) p_end = n_calib;
@@ -539,7 +542,7 @@ lsm_moment_small_kernel(LsmArgs a, MomentPlan mp,
     const double c = (basis_input(a, x) - a.mean[0]) * a.scale[0];
     double v = 1.0, u = y[p];
 #pragma unroll
-    for (int m = 0; m < kSmallMoments; ++m) {
+    for (int m = 0; m < NM; ++m) {
       if (m < nx) {
         acc[m] += v;
         v *= c;
@@ -552,7 +555,7 @@ lsm_moment_small_kernel(LsmArgs a, MomentPlan mp,
   // fixed-order sums: wave butterflies, then the four waves in order
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int m = 0; m < kSmallMoments; ++m) {
+  for (int m = 0; m < NM; ++m) {
     if (m < nm) {
       double v = acc[m];
       for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1238,9 +1241,15 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
     if (use_moments) {
       const int size = mp.rap * mp.cbp;
       double* msum = partial + (int64_t)n_chunks * size;
-      if (mp.small)
-        hipLaunchKernelGGL(lsm_moment_small_kernel, n_chunks, 256, 0, s, a,
-                           mp, exv, y, n_calib, chunk, partial);
+      if (mp.small && mp.cbp <= 8)
+        hipLaunchKernelGGL(lsm_moment_small_kernel<8>, n_chunks, 256, 0, s,
+                           a, mp, exv, y, n_calib, chunk, partial);
+      else if (mp.small && mp.cbp <= 16)
+        hipLaunchKernelGGL(lsm_moment_small_kernel<16>, n_chunks, 256, 0, s,
+                           a, mp, exv, y, n_calib, chunk, partial);
+      else if (mp.small)
+        hipLaunchKernelGGL(lsm_moment_small_kernel<kSmallMoments>, n_chunks,
+                           256, 0, s, a, mp, exv, y, n_calib, chunk, partial);
       else
         hipLaunchKernelGGL(lsm_moment_kernel, n_chunks, kMomThreads, mp.lds,
                            s, a, mp, exv, y, n_calib, chunk, partial);
