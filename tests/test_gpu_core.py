@@ -40,9 +40,14 @@ def _close(got, want, rtol, scale=None):
 
 # ---------------------------------------------------------------- Sobol ----
 
+# the dimension picks the kernel: whole-row tiles written with 4-, 8- or
+# 16-byte stores (dim odd, even, a multiple of 4; dim <= 768), else one
+# thread per (dimension, 256-index run)
 @pytest.mark.parametrize('dim,n,skip', [
     (1, 1, 0), (2, 31, 0), (5, 1000, 17), (1, 3, 2**31 - 5), (2, 2**20, 0),
-    (730, 2**13, 0), (730, 4096, 12345), (37, 777, 2**20 - 3)])
+    (730, 2**13, 0), (730, 4096, 12345), (37, 777, 2**20 - 3),
+    (4, 1000, 3), (8, 333, 100), (768, 100, 15), (769, 64, 0),
+    (800, 500, 7)])
 def test_sobol_integers_bit_exact(dim, n, skip):
   print('sobol ints dim=%d n=%d skip=%d' % (dim, n, skip), flush=True)
   got, nd = tsobol.sample_int(dim, n, skip)
@@ -71,6 +76,15 @@ def test_sobol_sample_goldens_and_dtypes():
   # float32 rounding of cast(int) / 2^nd matches the oracle at nd = 31
   got = _np(tqf.math.random.sobol.sample(3, 1000, skip=2**31 - 1002))
   np.testing.assert_array_equal(got, osobol.sample(3, 1000, 2**31 - 1002))
+
+
+@pytest.mark.parametrize('dim,n,skip', [(730, 300, 5), (800, 100, 0),
+                                         (3, 1000, 17)])
+def test_sobol_float64_matches_oracle(dim, n, skip):
+  """float64 points (8-byte outputs take the per-(dimension, run) kernel)
+  are cast(int) / 2^nd exactly."""
+  got = _np(tqf.math.random.sobol.sample(dim, n, skip, dtype=np.float64))
+  np.testing.assert_array_equal(got, osobol.sample(dim, n, skip, np.float64))
 
 
 def test_sobol_validate_args():
@@ -506,6 +520,19 @@ def test_halton_matches_oracle(dtype, rtol):
                         zero_correction=_np(params.zero_correction),
                         dtype=npd)
   np.testing.assert_allclose(_np(got), want, rtol=rtol, atol=1e-30)
+  # 1000 axes (bases up to 7919): in float64 the randomised axes whose
+  # b^max_digits passes 2^63 add digit by digit, the others in one division
+  g2, p2 = halton.sample(1000, num_results=16, seed=3, dtype=dtype)
+  w2 = ohalton.sample(1000, num_results=16, perms=_np(p2.perms),
+                      zero_correction=_np(p2.zero_correction), dtype=npd)
+  np.testing.assert_allclose(_np(g2), w2, rtol=rtol, atol=1e-30)
+  if dtype == torch.float64:
+    # indices past 2^32 take the 64-bit digit division
+    idx = np.array([2**32 - 2, 2**32 - 1, 2**33 + 5, 2**40 + 12345])
+    g3, _ = halton.sample(40, sequence_indices=idx, randomized=False,
+                          dtype=dtype)
+    w3 = ohalton.sample(40, sequence_indices=idx, dtype=npd)
+    np.testing.assert_allclose(_np(g3), w3, rtol=rtol, atol=1e-30)
   # the same seed gives the same points; params reproduce them by index
   again, _ = halton.sample(7, num_results=2000, seed=11, dtype=dtype)
   np.testing.assert_array_equal(_np(again), _np(got))
