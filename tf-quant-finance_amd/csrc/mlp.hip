@@ -241,10 +241,12 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
                 float* __restrict__ partial) {
   __shared__ float P[kMlpParams];
   __shared__ float Wsum[4][kMlpParams + 1];  // per-wave gradient rows
-  __shared__ short tr[4][16 * 64];  // per-wave transpose buffer (bf16)
-  // per-wave SiLU'(z1) of the group, [sample][feature] with a row stride of
-  // 68 floats (16-byte rows, the four row groups of a read in distinct banks)
-  __shared__ float g1s[4][16 * kG1Stride];
+  // per-wave transpose buffers (bf16) of the two groups in flight
+  __shared__ short tr[4][2][16 * 64];
+  // per-wave SiLU'(z1) of the two groups, [sample][feature] with a row
+  // stride of 68 floats (16-byte rows, the four row groups of a read in
+  // distinct banks)
+  __shared__ float g1s[4][2][16 * kG1Stride];
   stage_params(params, P, nullptr);
   __syncthreads();
   bf16x4 w2b[16], w2t[16];
@@ -252,7 +254,6 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
   load_w2t(P, w2t);
   const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
   const int wave = threadIdx.x >> 6;
-  short* T = tr[wave];
   f32x4 dw2[16];
 #pragma unroll
   for (int e = 0; e < 16; ++e) dw2[e] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -266,123 +267,166 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
   const int64_t waves = (int64_t)gridDim.x * 4;
   const int64_t wid = (int64_t)blockIdx.x * 4 + wave;
   const int64_t end = first + n;
+  // Two groups in flight per wave, g and g + waves: every stage below runs
+  // for both, so the two independent dependency chains hide each other's
+  // latencies (one wave per SIMD leaves nothing else to switch to).  Each
+  // accumulator takes group g's terms before group g + waves', the order
+  // of one group at a time; a partner past the end has zero inputs and
+  // weight and adds exact zeros.
   // inputs of a group: lane l holds sample base + l % 16 (its row, weight
-  // and target; zero past the end).  The next group's are loaded while
-  // this one is processed, so the memory latency is not waited out per
-  // group; the rows 4q + i the backward pass needs come by shuffles.
-  float nx[kIn], nw = 0.f, nt = 0.f;
-  auto fetch = [&](int64_t g) {
+  // and target; zero past the end), loaded one pair ahead; the rows 4q + i
+  // the backward pass needs come by shuffles.
+  float nx[2][kIn], nw[2], nt[2];
+  auto fetch = [&](int u, int64_t g) {
     const int64_t sa = first + g * 16 + r;
     const bool ok = g * 16 < n && sa < end;
 #pragma unroll
-    for (int i = 0; i < kIn; ++i) nx[i] = ok ? x[sa * kMlpStride + i] : 0.f;
-    nw = ok ? wts[sa] : 0.f;
-    nt = ok ? tgt[sa] : 0.f;
+    for (int i = 0; i < kIn; ++i) nx[u][i] = ok ? x[sa * kMlpStride + i] : 0.f;
+    nw[u] = ok ? wts[sa] : 0.f;
+    nt[u] = ok ? tgt[sa] : 0.f;
   };
-  fetch(wid);
-  for (int64_t g = wid; g * 16 < n; g += waves) {
-    const int64_t base = first + g * 16;
-    float xa[kIn];
+  fetch(0, wid);
+  fetch(1, wid + waves);
+  for (int64_t g = wid; g * 16 < n; g += 2 * waves) {
+    float xa[2][kIn], wa[2], ta[2];
 #pragma unroll
-    for (int i = 0; i < kIn; ++i) xa[i] = nx[i];
-    const float wa = nw, ta = nt;
-    fetch(g + waves);
-    Fwd f;
-    forward_group(P, w2b, xa, f);
+    for (int u = 0; u < 2; ++u) {
+#pragma unroll
+      for (int i = 0; i < kIn; ++i) xa[u][i] = nx[u][i];
+      wa[u] = nw[u];
+      ta[u] = nt[u];
+    }
+    fetch(0, g + 2 * waves);
+    fetch(1, g + 3 * waves);
+    Fwd f[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) forward_group(P, w2b, xa[u], f[u]);
     // dy for the lane's rows
-    float dy[4], xr[4][kIn];
+    float dy[2][4], xr[2][4][kIn];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int src = 4 * q + i;
-#pragma unroll
-      for (int k = 0; k < kIn; ++k) xr[i][k] = __shfl(xa[k], src, 64);
-      const float w = __shfl(wa, src, 64), t = __shfl(ta, src, 64);
-      const float e = f.y[i] - t;
-      dy[i] = 2.f * w * e * inv_norm;
-      if (r == 0) loss += w * e * e * inv_norm;
-      db3 += r == 0 ? dy[i] : 0.f;
-    }
-    // SiLU'(z1) from the forward pass, to the C layout of the layer-1
-    // backward through LDS (in place of recomputing z1 and its sigmoid)
-    float* G1 = g1s[wave];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      f32x4 gv;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        gv[j] = silu_grad_s(f.z1a[c][j], f.s1a[c][j]);
-      *(f32x4*)(G1 + r * kG1Stride + c * 16 + 4 * q) = gv;
-    }
-    // layer 3 and SiLU' of layer 2: dZ2 in C layout (rows 4q+i, col t*16+r)
-    f32x4 dz2[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const float w3 = P[oW3 + t * 16 + r];
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const float z = f.z2[t][i];
-        const float sg = f.s2[t][i];
-        dw3[t] += z * sg * dy[i];
-        const float d = dy[i] * w3 * silu_grad_s(z, sg);
-        dz2[t][i] = d;
-        db2[t] += d;
+        const int src = 4 * q + i;
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) xr[u][i][k] = __shfl(xa[u][k], src, 64);
+        const float w = __shfl(wa[u], src, 64), t = __shfl(ta[u], src, 64);
+        const float e = f[u].y[i] - t;
+        dy[u][i] = 2.f * w * e * inv_norm;
+        if (r == 0) loss += w * e * e * inv_norm;
+        db3 += r == 0 ? dy[u][i] : 0.f;
+      }
+    // SiLU'(z1) from the forward pass, to the C layout of the layer-1
+    // backward through LDS (in place of recomputing z1 and its sigmoid)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      float* G1 = g1s[wave][u];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        f32x4 gv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          gv[j] = silu_grad_s(f[u].z1a[c][j], f[u].s1a[c][j]);
+        *(f32x4*)(G1 + r * kG1Stride + c * 16 + 4 * q) = gv;
       }
     }
+    // layer 3 and SiLU' of layer 2: dZ2 in C layout (rows 4q+i, col t*16+r)
+    f32x4 dz2[2][4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float w3 = P[oW3 + t * 16 + r];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float z = f[u].z2[t][i];
+          const float sg = f[u].s2[t][i];
+          dw3[t] += z * sg * dy[u][i];
+          const float d = dy[u][i] * w3 * silu_grad_s(z, sg);
+          dz2[u][t][i] = d;
+          db2[t] += d;
+        }
+      }
     // dW2 += H1^T dZ2: A = H1^T tile (features mt*16 + r, samples 4q + j)
     // from the transpose of the A-layout h1 via LDS.
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        T[r * 64 + c * 16 + 4 * q + j] = f.h1a[c][j];  // [sample][feature]
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)  // [sample][feature]
+          tr[wave][u][r * 64 + c * 16 + 4 * q + j] = f[u].h1a[c][j];
     wave_lds_sync();
-    bf16x4 h1t[4];
+    bf16x4 h1t[2][4], dzb[2][4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      short v[4];
+    for (int u = 0; u < 2; ++u) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = T[(4 * q + j) * 64 + mt * 16 + r];
-      h1t[mt] = bf16x4{v[0], v[1], v[2], v[3]};
+      for (int mt = 0; mt < 4; ++mt) {
+        short v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          v[j] = tr[wave][u][(4 * q + j) * 64 + mt * 16 + r];
+        h1t[u][mt] = bf16x4{v[0], v[1], v[2], v[3]};
+      }
+      // dZ2 as B operand: col t*16+r, k (samples) 4q + i
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        dzb[u][t] = bf16x4{to_bf16(dz2[u][t][0]), to_bf16(dz2[u][t][1]),
+                           to_bf16(dz2[u][t][2]), to_bf16(dz2[u][t][3])};
     }
-    bf16x4 dzb[4];  // dZ2 as B operand: col t*16+r, k (samples) 4q + i
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-      dzb[t] = bf16x4{to_bf16(dz2[t][0]), to_bf16(dz2[t][1]),
-                      to_bf16(dz2[t][2]), to_bf16(dz2[t][3])};
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+      for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) dw2[mt * 4 + t] = mfma(h1t[mt], dzb[t], dw2[mt * 4 + t]);
+        for (int t = 0; t < 4; ++t)
+          dw2[mt * 4 + t] = mfma(h1t[u][mt], dzb[u][t], dw2[mt * 4 + t]);
     // dH1 = dZ2 W2^T: A = dZ2 (rows = samples r, k = n c*16 + 4q + j) via LDS
     wave_lds_sync();
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        T[(4 * q + i) * 64 + t * 16 + r] = dzb[t][i];  // [sample][n]
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)  // [sample][n]
+          tr[wave][u][(4 * q + i) * 64 + t * 16 + r] = dzb[u][t][i];
     wave_lds_sync();
-    bf16x4 dza[4];
+    bf16x4 dza[2][4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      short v[4];
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = T[r * 64 + c * 16 + 4 * q + j];
-      dza[c] = bf16x4{v[0], v[1], v[2], v[3]};
-    }
+      for (int c = 0; c < 4; ++c) {
+        short v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          v[j] = tr[wave][u][r * 64 + c * 16 + 4 * q + j];
+        dza[u][c] = bf16x4{v[0], v[1], v[2], v[3]};
+      }
     wave_lds_sync();
     // dH1 tiles (C layout): rows (samples) 4q + i, features t*16 + r
+    f32x4 dh[2][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      f32x4 dh = {0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) dh = mfma(dza[c], w2t[t * 4 + c], dh);
-      const int fo = t * 16 + r;
+      for (int t = 0; t < 4; ++t) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float d = dh[i] * G1[(4 * q + i) * kG1Stride + fo];
-        db1[t] += d;
+        for (int c = 0; c < 4; ++c) acc = mfma(dza[u][c], w2t[t * 4 + c], acc);
+        dh[u][t] = acc;
+      }
 #pragma unroll
-        for (int k = 0; k < kIn; ++k) dw1[k][t] += xr[i][k] * d;
+    for (int u = 0; u < 2; ++u) {
+      const float* G1 = g1s[wave][u];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int fo = t * 16 + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float d = dh[u][t][i] * G1[(4 * q + i) * kG1Stride + fo];
+          db1[t] += d;
+#pragma unroll
+          for (int k = 0; k < kIn; ++k) dw1[k][t] += xr[u][i][k] * d;
+        }
       }
     }
   }
