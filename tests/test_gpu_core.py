@@ -2,8 +2,8 @@
 kernels against the CPU oracle through the C ABI.
 
 Tolerances: integers bit-exact; float64 paths 1e-12 relative (scale-aware),
-float32 paths 2e-5 relative, with identical injected draws (the north-star
-bar).  Streams generated in the kernels (Philox/Box-Muller, Sobol + inverse
+float32 paths 2e-5 relative, with identical injected draws (the parity
+bar of the engine against euler_sampling).  Streams generated in the kernels (Philox/Box-Muller, Sobol + inverse
 CDF) are compared with the oracle's streams at the precision of the device
 transcendentals, stated per test.
 """
