@@ -387,6 +387,19 @@ def _paths_and_points(out, timed, engine):
       'path_steps_per_s': n * 365 / t}
   del lp
   torch.cuda.empty_cache()
+  # the same in the reference layout [N, k, d] (what euler_sampling.sample
+  # returns): records staged per path in LDS, one segment per path
+  lp = engine.prepare(('heston', 0.03, 2.0, 0.04, 0.3, -0.7, 0.0), all_times,
+                      keep, 366, np.array([np.log(100.0), 0.04]), n, rng,
+                      torch.float32, mode='paths')
+  lp.launch()
+  t, _ = timed(lp.launch, 3)
+  res['pathmajor_heston_ft'] = {
+      'value': nbytes / t / 1e9, 'unit': 'GB/s', 'seconds': t,
+      'bytes_written': nbytes, 'frac_hbm_peak': nbytes / t / HBM_PEAK,
+      'path_steps_per_s': n * 365 / t}
+  del lp
+  torch.cuda.empty_cache()
   # sobol.sample(730, 2^20) float32 (the C2 draw set, 3.06 GB)
   tsobol.sample(730, n)
   t, _ = timed(lambda: tsobol.sample(730, n), 3)

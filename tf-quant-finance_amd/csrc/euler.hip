@@ -31,6 +31,13 @@ namespace {
 
 constexpr int kBlock = kPathBlock;
 constexpr int kWaves = kBlock / kWave;
+// Path-major output ([N][k][D]) is staged in the payoff area of LDS, which
+// that mode does not use: up to kStageWords values per path ([path]
+// [kStageStride], odd stride) flushed as one contiguous segment per path,
+// in place of one scattered store per path and record.  It is an
+// instantiation of its own (kStage), so the other modes keep their code.
+constexpr int kStageWords = 32;
+constexpr int kStageStride = kStageWords + 1;
 constexpr int kMaxLvLdsBytes = 32768;  // local-vol tables up to this size go to LDS
 
 template <typename R>
@@ -60,6 +67,7 @@ struct EulerArgs {
   const R* x0;
   int64_t x0_stride;
   int lds_pay_off;   // byte offset of the payoff area in dynamic LDS
+  int stage_recs;    // path-major: records staged per flush (D * it <= 32)
   int lds_spot_off;  // byte offset of the per-wave spot rows from it
 };
 
@@ -266,7 +274,7 @@ __device__ __forceinline__ void record(const EulerArgs<R>& a, R* lds_pay,
   }
 }
 
-template <typename R, int MODEL, int RNG>
+template <typename R, int MODEL, int RNG, bool kStage>
 __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   constexpr int D = ModelDim<MODEL>::D;
   extern __shared__ __align__(16) unsigned char lds[];
@@ -292,10 +300,41 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   Normals<R, RNG> nrm;
   nrm.init(a.rng, pc, D, lds, a.n_steps);
 
-  if (a.init_slot >= 0)
-    record<R, MODEL>(a, lds_pay, a.init_slot, x,
-                     MODEL == TQF_MODEL_HULL_WHITE ? a.p[0] : (R)0, path,
-                     active);
+  // path-major staging (block-uniform bookkeeping; flushes synchronise
+  // the block, so they only happen at the uniform record steps)
+  int nrec = 0, first_slot = 0;
+  auto flush = [&]() {
+    __syncthreads();
+    const int seg = nrec * D;
+    const int64_t p0 = (int64_t)blockIdx.x * kBlock;
+    for (int e = threadIdx.x; e < kBlock * seg; e += kBlock) {
+      const int pl = e / seg, rem = e - pl * seg;
+      const int64_t pg = p0 + pl;
+      if (pg < a.n_paths)
+        a.paths[(pg * a.n_slots + first_slot) * D + rem] =
+            lds_pay[pl * kStageStride + rem];
+    }
+    __syncthreads();
+    nrec = 0;
+  };
+  auto put = [&](int slot, R off) {
+    if (nrec > 0 && slot != first_slot + nrec) flush();
+    if (nrec == 0) first_slot = slot;
+    if (active) {
+      R* b = lds_pay + threadIdx.x * kStageStride + nrec * D;
+#pragma unroll
+      for (int c = 0; c < D; ++c) b[c] = c == 0 ? x[c] + off : x[c];
+    }
+    if (++nrec == a.stage_recs) flush();
+  };
+
+  if (a.init_slot >= 0) {
+    const R off = MODEL == TQF_MODEL_HULL_WHITE ? a.p[0] : (R)0;
+    if constexpr (kStage)
+      put(a.init_slot, off);
+    else
+      record<R, MODEL>(a, lds_pay, a.init_slot, x, off, path, active);
+  }
 
   int lv_row = 0;
   const int n_dims = a.n_steps * D;
@@ -346,9 +385,13 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
         }
       }
       Stepper<R, MODEL>::step(a, x, zc, dt, sdt, tab, lv_row, lv_w, i, row);
-      if (s >= 0)
-        record<R, MODEL>(a, lds_pay, s, x,
-                         kHasStepRow<MODEL> ? row.c[3] : (R)0, path, active);
+      if (s >= 0) {
+        const R off = kHasStepRow<MODEL> ? row.c[3] : (R)0;
+        if constexpr (kStage)
+          put(s, off);
+        else
+          record<R, MODEL>(a, lds_pay, s, x, off, path, active);
+      }
     };
     // two steps per iteration with the normals alternating between zA and
     // zB, so the pipelining needs no register copies
@@ -360,6 +403,9 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
       one_step(i + 1, zB, zA);
     }
     if (i < c1) one_step(i, zA, zB);
+  }
+  if constexpr (kStage) {
+    if (nrec > 0) flush();
   }
 
   if (a.mode == TQF_OUT_PAYOFF) {
@@ -502,6 +548,9 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   // dynamic LDS: Sobol high-bit words | local-vol table | payoff sums and
   // spot rows; the float32 inverse-CDF table is static LDS (kIcdfLdsBytes)
   size_t pay = 0;
+  const bool stage = a.mode == TQF_OUT_PATH_MAJOR;
+  a.stage_recs = kStageWords / D;
+  if (stage) pay = (size_t)kBlock * kStageStride * sizeof(R);
   if (a.mode == TQF_OUT_PAYOFF) {
     const size_t sums = (size_t)kWaves * g->n_slots * out->n_strikes * 2 * sizeof(R);
     a.lds_spot_off = (int)((sums + 15) & ~(size_t)15);
@@ -526,7 +575,9 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
                        ? kModelHestonOwn
                        : m->kind;
 #define TQF_PICK(MODEL, RNG) \
-  if (kind == MODEL && r->kind == RNG) kern = euler_kernel<R, MODEL, RNG>;
+  if (kind == MODEL && r->kind == RNG)                                   \
+    kern = stage ? euler_kernel<R, MODEL, RNG, true>                     \
+                 : euler_kernel<R, MODEL, RNG, false>;
 #define TQF_PICK_RNG(MODEL)                       \
   TQF_PICK(MODEL, TQF_RNG_DRAWS)                  \
   TQF_PICK(MODEL, TQF_RNG_PHILOX)                 \
