@@ -474,38 +474,37 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
         ((Wsum[0][e] + Wsum[1][e]) + Wsum[2][e]) + Wsum[3][e];
 }
 
-// Level 1 of the gradient reduction: workgroup (x, c) adds the per-block
-// partials of blocks [c * per, (c + 1) * per) for entries x*256 + t, so the
-// reduction runs on many CUs instead of one row of workgroups.
-constexpr int kGradChunks = 32;
+// The block partials summed and the Adam update in one kernel: workgroup
+// x owns parameters [64 x, 64 x + 64); its wave w (of kRedWaves) adds the
+// rows of blocks [w per, (w + 1) per) in block order (consecutive lanes
+// read consecutive parameters), then the wave sums are added in wave order.
+constexpr int kRedWaves = 16;
 
-__global__ void __launch_bounds__(256)
-mlp_partial_reduce_kernel(const float* __restrict__ partial, int n_blocks,
-                          int per, float* __restrict__ chunks) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e > kMlpParams) return;
-  const int b0 = blockIdx.y * per;
-  const int b1 = b0 + per < n_blocks ? b0 + per : n_blocks;
+__global__ void __launch_bounds__(kRedWaves * 64)
+mlp_reduce_adam_kernel(float* __restrict__ params, float* __restrict__ m,
+                       float* __restrict__ v,
+                       const float* __restrict__ partial, int n_blocks,
+                       float lr, float b1, float b2, float eps, int step,
+                       float* __restrict__ grad_out,
+                       float* __restrict__ loss_out) {
+  __shared__ float red[kRedWaves][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + lane;
+  const int per = (n_blocks + kRedWaves - 1) / kRedWaves;
+  const int b0 = w * per;
+  const int b1e = b0 + per < n_blocks ? b0 + per : n_blocks;
   float g = 0.f;
-  // unrolled so the loads of several rows are in flight at once (the sum
-  // keeps its order)
+  if (e <= kMlpParams) {
 #pragma unroll 8
-  for (int b = b0; b < b1; ++b) g += partial[(int64_t)b * (kMlpParams + 1) + e];
-  chunks[(int64_t)blockIdx.y * (kMlpParams + 1) + e] = g;
-}
-
-// Level 2 (the chunk sums, fixed order) and the Adam update.
-__global__ void __launch_bounds__(256)
-mlp_adam_kernel(float* __restrict__ params, float* __restrict__ m,
-                float* __restrict__ v, const float* __restrict__ partial,
-                int n_blocks, float lr, float b1, float b2, float eps,
-                int step, float* __restrict__ grad_out,
-                float* __restrict__ loss_out) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e > kMlpParams) return;
-  float g = 0.f;
-#pragma unroll 8
-  for (int b = 0; b < n_blocks; ++b) g += partial[(int64_t)b * (kMlpParams + 1) + e];
+    for (int bb = b0; bb < b1e; ++bb)
+      g += partial[(int64_t)bb * (kMlpParams + 1) + e];
+  }
+  red[w][lane] = g;
+  __syncthreads();
+  if (w != 0 || e > kMlpParams) return;
+  g = red[0][lane];
+#pragma unroll
+  for (int q = 1; q < kRedWaves; ++q) g += red[q][lane];
   if (e == kMlpParams) {
     if (loss_out) *loss_out = g;
     return;
@@ -540,15 +539,9 @@ int mlp_grad_step(float* params, float* m, float* v, const float* x,
                   hipStream_t s) {
   hipLaunchKernelGGL(mlp_grad_kernel, n_blocks, 256, 0, s, params, x, tgt, w,
                      first, n, inv_norm, partial);
-  const int n_chunks = n_blocks < kGradChunks ? n_blocks : kGradChunks;
-  const int per = (n_blocks + n_chunks - 1) / n_chunks;
-  float* chunks = partial + (int64_t)n_blocks * (kMlpParams + 1);
-  const unsigned tiles = (kMlpParams + 1 + 255) / 256;
-  hipLaunchKernelGGL(mlp_partial_reduce_kernel, dim3(tiles, n_chunks), 256, 0,
-                     s, partial, n_blocks, per, chunks);
-  hipLaunchKernelGGL(mlp_adam_kernel, tiles, 256, 0, s, params, m, v, chunks,
-                     n_chunks, lr, 0.9f, 0.999f, 1e-8f, step, grad_out,
-                     loss_out);
+  hipLaunchKernelGGL(mlp_reduce_adam_kernel, (kMlpParams + 1 + 63) / 64,
+                     kRedWaves * 64, 0, s, params, m, v, partial, n_blocks, lr,
+                     0.9f, 0.999f, 1e-8f, step, grad_out, loss_out);
   return check_launch("mlp_grad_step");
 }
 
@@ -566,8 +559,8 @@ extern "C" int tqf_mlp_forward(const float* params, const float* x, int64_t n,
 }
 
 extern "C" int64_t tqf_mlp_workspace_size(int32_t n_blocks) {
-  const int64_t chunks = n_blocks < kGradChunks ? n_blocks : kGradChunks;
-  return ((int64_t)n_blocks + chunks) * (kMlpParams + 1) * 4;
+  // one row of gradient partial sums (and the loss) per workgroup
+  return (int64_t)n_blocks * (kMlpParams + 1) * 4;
 }
 
 extern "C" int tqf_mlp_train_step(float* params, float* adam_m, float* adam_v,
