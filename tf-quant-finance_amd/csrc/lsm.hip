@@ -125,22 +125,52 @@ lsm_prepare_kernel(LsmArgs a, double df_ratio, const double* __restrict__ cash,
 }
 
 // mean[d] and scale[d] = 1 / std (1 when the spread is zero).  One
-// workgroup: thread t adds blocks t, t + 256, ... in order, then a fixed
-// LDS tree (deterministic).
-__global__ void __launch_bounds__(256)
+// workgroup of kMeanThreads: thread t adds blocks t, t + kMeanThreads, ...
+// in order (only the 2 d columns in use), then wave butterflies and the
+// wave sums in wave order (deterministic).
+constexpr int kMeanThreads = 1024;
+
+__global__ void __launch_bounds__(kMeanThreads)
 lsm_mean_kernel(const double* __restrict__ block_sums, int64_t n_blocks,
                 int d, int64_t n_paths, double* __restrict__ mean,
                 double* __restrict__ scale) {
-  __shared__ double red[2 * kMaxAssets][256];
-  const int t = threadIdx.x;
+  __shared__ double red[kMeanThreads / 64][2 * kMaxAssets];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   double acc[2 * kMaxAssets];
 #pragma unroll
   for (int j = 0; j < 2 * kMaxAssets; ++j) acc[j] = 0;
-  for (int64_t b = t; b < n_blocks; b += 256) {
+#pragma unroll 4
+  for (int64_t b = t; b < n_blocks; b += kMeanThreads) {
     const double* row = block_sums + b * 2 * kMaxAssets;
 #pragma unroll
-    for (int j = 0; j < 2 * kMaxAssets; ++j) acc[j] += row[j];
+    for (int j = 0; j < kMaxAssets; ++j) {
+      if (j < d) {
+        acc[j] += row[j];
+        acc[kMaxAssets + j] += row[kMaxAssets + j];
+      }
+    }
   }
+#pragma unroll
+  for (int j = 0; j < 2 * kMaxAssets; ++j) {
+    if ((j < kMaxAssets ? j : j - kMaxAssets) < d) {
+      double v = acc[j];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[wave][j] = v;
+    }
+  }
+  __syncthreads();
+  if (t >= d) return;
+  double s1 = 0, s2 = 0;
+  for (int w = 0; w < kMeanThreads / 64; ++w) {
+    s1 += red[w][t];
+    s2 += red[w][kMaxAssets + t];
+  }
+  const double n = (double)n_paths;
+  const double m = s1 / n;
+  const double var = s2 / n - m * m;
+  mean[t] = m;
+  scale[t] = var > 1e-300 && sqrt(var) > 1e-12 * fabs(m) ? 1.0 / sqrt(var) : 1.0;
+}
 #pragma unroll
   for (int j = 0; j < 2 * kMaxAssets; ++j) red[j][t] = acc[j];
   __syncthreads();
@@ -1236,8 +1266,8 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
     const double ratio = df[e + 1] / df[e];
     hipLaunchKernelGGL(lsm_prepare_kernel, blocks_for(n), 256, 0, s, a, ratio,
                        cash, vals, exv, y, bsum);
-    hipLaunchKernelGGL(lsm_mean_kernel, 1, 256, 0, s, bsum, nb, a.d, n, mean,
-                       scale);
+    hipLaunchKernelGGL(lsm_mean_kernel, 1, kMeanThreads, 0, s, bsum, nb, a.d,
+                       n, mean, scale);
     if (use_moments) {
       const int size = mp.rap * mp.cbp;
       double* msum = partial + (int64_t)n_chunks * size;
@@ -1352,8 +1382,8 @@ extern "C" int tqf_lsm_date_inputs(const tqf_lsm* l, int32_t t_idx,
   LsmArgs a = make_args(l, t_idx);
   hipLaunchKernelGGL(lsm_prepare_kernel, blocks_for(n), 256, 0, s, a, df_ratio,
                      cash, vals, exv, y, bsum);
-  hipLaunchKernelGGL(lsm_mean_kernel, 1, 256, 0, s, bsum, nb, a.d, n, mean,
-                     scale);
+  hipLaunchKernelGGL(lsm_mean_kernel, 1, kMeanThreads, 0, s, bsum, nb, a.d, n,
+                     mean, scale);
   a.mean = mean;
   a.scale = scale;
   hipLaunchKernelGGL(lsm_features_kernel, blocks_for(n), 256, 0, s, a, exv, y,
