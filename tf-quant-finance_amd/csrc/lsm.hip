@@ -171,22 +171,6 @@ lsm_mean_kernel(const double* __restrict__ block_sums, int64_t n_blocks,
   mean[t] = m;
   scale[t] = var > 1e-300 && sqrt(var) > 1e-12 * fabs(m) ? 1.0 / sqrt(var) : 1.0;
 }
-#pragma unroll
-  for (int j = 0; j < 2 * kMaxAssets; ++j) red[j][t] = acc[j];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w)
-#pragma unroll
-      for (int j = 0; j < 2 * kMaxAssets; ++j) red[j][t] += red[j][t + w];
-    __syncthreads();
-  }
-  if (t >= d) return;
-  const double n = (double)n_paths;
-  const double m = red[t][0] / n;
-  const double var = red[kMaxAssets + t][0] / n - m * m;
-  mean[t] = m;
-  scale[t] = var > 1e-300 && sqrt(var) > 1e-12 * fabs(m) ? 1.0 / sqrt(var) : 1.0;
-}
 
 // Basis column k of one path from its per-asset power table.
 __device__ __forceinline__ double basis_col(const double* pw, int deg1,
