@@ -322,7 +322,8 @@ int tqf_lsm_price(const tqf_lsm* lsm, const int32_t* ex_idx, int32_t n_ex,
  *   tqf_lsm_terminal     cash = payoff at t_idx, vals = 0
  *   tqf_lsm_date_inputs  exv = payoff, y = df_ratio (cash + vals), centring
  *                        moments (tqf_lsm_moments_size bytes), and MLP rows
- *                        x[n][8] = (standardised S_1..S_dim, exv / K, 0...),
+ *                        x[n][8] = (standardised S_1..S_dim, exv / K, 0...)
+ *                        (x 16-byte aligned),
  *                        tgt = y / K, w = 1{exv > 0}   (dim <= 5)
  *   tqf_lsm_update_cont  exercise iff exv > K max(cont, 0) (lsm.py:381-400)
  *   tqf_lsm_final        (mean, stderr) of df0 (cash + vals) over [first, n)

@@ -1081,14 +1081,22 @@ lsm_features_kernel(LsmArgs a, const double* __restrict__ exv,
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= a.n_paths) return;
   const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
-  float* xr = x + p * kMlpStride;
+  // the row of 8 floats in registers, stored as two 16-byte halves (a
+  // wave's stores then cover whole lines)
+  float v[kMlpStride];
+#pragma unroll
   for (int j = 0; j < kMlpIn - 1; ++j)
-    xr[j] = j < a.d ? (float)((basis_input(a, row[j]) - a.mean[j]) * a.scale[j])
-                    : 0.f;
-  xr[kMlpIn - 1] = (float)(exv[p] / a.strike);
-  xr[6] = xr[7] = 0.f;
+    v[j] = j < a.d ? (float)((basis_input(a, row[j]) - a.mean[j]) * a.scale[j])
+                   : 0.f;
+  const double e = exv[p];
+  v[kMlpIn - 1] = (float)(e / a.strike);
+  v[6] = v[7] = 0.f;
+  typedef float f32x4v __attribute__((ext_vector_type(4)));
+  f32x4v* xr = (f32x4v*)(x + p * kMlpStride);
+  xr[0] = f32x4v{v[0], v[1], v[2], v[3]};
+  xr[1] = f32x4v{v[4], v[5], v[6], v[7]};
   tgt[p] = (float)(y[p] / a.strike);
-  w[p] = exv[p] > 0 ? 1.f : 0.f;
+  w[p] = e > 0 ? 1.f : 0.f;
 }
 
 // Exercise update from a given continuation estimate (in units of K).
@@ -1356,6 +1364,7 @@ extern "C" int tqf_lsm_date_inputs(const tqf_lsm* l, int32_t t_idx,
               kMlpIn - 1);
   TQF_REQUIRE(cash && vals && exv && y && moments && x && tgt && w,
               "null argument");
+  TQF_REQUIRE(((uintptr_t)x & 15) == 0, "x must be 16-byte aligned");
   const hipStream_t s = (hipStream_t)stream;
   const int64_t n = l->n_paths;
   const int64_t nb = (n + 255) / 256;
