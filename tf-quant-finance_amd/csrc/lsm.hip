@@ -125,10 +125,16 @@ lsm_prepare_kernel(LsmArgs a, double df_ratio, const double* __restrict__ cash,
 }
 
 // mean[d] and scale[d] = 1 / std (1 when the spread is zero).  One
-// workgroup of kMeanThreads: thread t adds blocks t, t + kMeanThreads, ...
-// in order (only the 2 d columns in use), then wave butterflies and the
-// wave sums in wave order (deterministic).
+// workgroup of kMeanThreads reads the block sums [n_blocks][2 kMaxAssets] as
+// one flat array (consecutive threads, consecutive doubles); since
+// kMeanThreads and 64 are multiples of 2 kMaxAssets, thread t only ever adds
+// column t % (2 kMaxAssets), in block order.  Lanes l, l^16, l^32, l^48 hold
+// the same column and are combined by two butterflies, then the wave sums
+// are added in wave order (deterministic).  Columns past d are never mixed
+// into the ones in use.
 constexpr int kMeanThreads = 1024;
+static_assert(64 % (2 * kMaxAssets) == 0 && 2 * kMaxAssets == 16,
+              "the butterflies assume 16 columns");
 
 __global__ void __launch_bounds__(kMeanThreads)
 lsm_mean_kernel(const double* __restrict__ block_sums, int64_t n_blocks,
@@ -136,28 +142,13 @@ lsm_mean_kernel(const double* __restrict__ block_sums, int64_t n_blocks,
                 double* __restrict__ scale) {
   __shared__ double red[kMeanThreads / 64][2 * kMaxAssets];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  double acc[2 * kMaxAssets];
-#pragma unroll
-  for (int j = 0; j < 2 * kMaxAssets; ++j) acc[j] = 0;
-#pragma unroll 4
-  for (int64_t b = t; b < n_blocks; b += kMeanThreads) {
-    const double* row = block_sums + b * 2 * kMaxAssets;
-#pragma unroll
-    for (int j = 0; j < kMaxAssets; ++j) {
-      if (j < d) {
-        acc[j] += row[j];
-        acc[kMaxAssets + j] += row[kMaxAssets + j];
-      }
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < 2 * kMaxAssets; ++j) {
-    if ((j < kMaxAssets ? j : j - kMaxAssets) < d) {
-      double v = acc[j];
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      if (lane == 0) red[wave][j] = v;
-    }
-  }
+  const int64_t total = n_blocks * 2 * kMaxAssets;
+  double acc = 0;
+#pragma unroll 8
+  for (int64_t i = t; i < total; i += kMeanThreads) acc += block_sums[i];
+  acc += __shfl_xor(acc, 16, 64);
+  acc += __shfl_xor(acc, 32, 64);
+  if (lane < 2 * kMaxAssets) red[wave][lane] = acc;
   __syncthreads();
   if (t >= d) return;
   double s1 = 0, s2 = 0;
