@@ -475,10 +475,13 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
 }
 
 // The block partials summed and the Adam update in one kernel: workgroup
-// x owns parameters [64 x, 64 x + 64); its wave w (of kRedWaves) adds the
-// rows of blocks [w per, (w + 1) per) in block order (consecutive lanes
-// read consecutive parameters), then the wave sums are added in wave order.
+// x owns parameters [32 x, 32 x + 32) (147 workgroups, so most CUs share
+// the read); half-wave q = 2 w + (lane >> 5) of its kRedSlices adds the rows
+// of blocks [q per, (q + 1) per) in block order (consecutive lanes read
+// consecutive parameters), then the slice sums are added in slice order.
 constexpr int kRedWaves = 16;
+constexpr int kRedParams = 32;
+constexpr int kRedSlices = kRedWaves * 64 / kRedParams;
 
 __global__ void __launch_bounds__(kRedWaves * 64)
 mlp_reduce_adam_kernel(float* __restrict__ params, float* __restrict__ m,
@@ -487,11 +490,11 @@ mlp_reduce_adam_kernel(float* __restrict__ params, float* __restrict__ m,
                        float lr, float b1, float b2, float eps, int step,
                        float* __restrict__ grad_out,
                        float* __restrict__ loss_out) {
-  __shared__ float red[kRedWaves][64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int e = blockIdx.x * 64 + lane;
-  const int per = (n_blocks + kRedWaves - 1) / kRedWaves;
-  const int b0 = w * per;
+  __shared__ float red[kRedSlices][kRedParams];
+  const int c = threadIdx.x % kRedParams, q = threadIdx.x / kRedParams;
+  const int e = blockIdx.x * kRedParams + c;
+  const int per = (n_blocks + kRedSlices - 1) / kRedSlices;
+  const int b0 = q * per;
   const int b1e = b0 + per < n_blocks ? b0 + per : n_blocks;
   float g = 0.f;
   if (e <= kMlpParams) {
@@ -499,12 +502,12 @@ mlp_reduce_adam_kernel(float* __restrict__ params, float* __restrict__ m,
     for (int bb = b0; bb < b1e; ++bb)
       g += partial[(int64_t)bb * (kMlpParams + 1) + e];
   }
-  red[w][lane] = g;
+  red[q][c] = g;
   __syncthreads();
-  if (w != 0 || e > kMlpParams) return;
-  g = red[0][lane];
+  if (q != 0 || e > kMlpParams) return;
+  g = red[0][c];
 #pragma unroll
-  for (int q = 1; q < kRedWaves; ++q) g += red[q][lane];
+  for (int k = 1; k < kRedSlices; ++k) g += red[k][c];
   if (e == kMlpParams) {
     if (loss_out) *loss_out = g;
     return;
@@ -539,7 +542,8 @@ int mlp_grad_step(float* params, float* m, float* v, const float* x,
                   hipStream_t s) {
   hipLaunchKernelGGL(mlp_grad_kernel, n_blocks, 256, 0, s, params, x, tgt, w,
                      first, n, inv_norm, partial);
-  hipLaunchKernelGGL(mlp_reduce_adam_kernel, (kMlpParams + 1 + 63) / 64,
+  hipLaunchKernelGGL(mlp_reduce_adam_kernel,
+                     (kMlpParams + 1 + kRedParams - 1) / kRedParams,
                      kRedWaves * 64, 0, s, params, m, v, partial, n_blocks, lr,
                      0.9f, 0.999f, 1e-8f, step, grad_out, loss_out);
   return check_launch("mlp_grad_step");
