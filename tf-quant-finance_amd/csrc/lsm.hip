@@ -92,8 +92,11 @@ __global__ void __launch_bounds__(256)
 lsm_prepare_kernel(LsmArgs a, double df_ratio, const double* __restrict__ cash,
                    const double* __restrict__ vals, double* __restrict__ exv,
                    double* __restrict__ y, double* __restrict__ block_sums) {
-  __shared__ double red[2 * kMaxAssets][256];
+  // per-asset sums of the basis input and its square: wave butterflies,
+  // then the four wave sums in wave order (deterministic)
+  __shared__ double red[4][2 * kMaxAssets];
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double x[kMaxAssets];
   for (int j = 0; j < kMaxAssets; ++j) x[j] = 0;
   if (p < a.n_paths) {
@@ -104,23 +107,24 @@ lsm_prepare_kernel(LsmArgs a, double df_ratio, const double* __restrict__ cash,
   }
   for (int j = 0; j < a.d; ++j) {
     const double b = p < a.n_paths ? basis_input(a, x[j]) : 0.0;
-    red[j][threadIdx.x] = b;
-    red[kMaxAssets + j][threadIdx.x] = b * b;
+    double s1 = b, s2 = b * b;
+    for (int o = 32; o > 0; o >>= 1) {
+      s1 += __shfl_xor(s1, o, 64);
+      s2 += __shfl_xor(s2, o, 64);
+    }
+    if (lane == 0) {
+      red[wave][j] = s1;
+      red[wave][kMaxAssets + j] = s2;
+    }
   }
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s)
-      for (int j = 0; j < a.d; ++j) {
-        red[j][threadIdx.x] += red[j][threadIdx.x + s];
-        red[kMaxAssets + j][threadIdx.x] += red[kMaxAssets + j][threadIdx.x + s];
-      }
-    __syncthreads();
-  }
   if (threadIdx.x < a.d) {
-    block_sums[(int64_t)blockIdx.x * 2 * kMaxAssets + threadIdx.x] =
-        red[threadIdx.x][0];
-    block_sums[(int64_t)blockIdx.x * 2 * kMaxAssets + kMaxAssets + threadIdx.x] =
-        red[kMaxAssets + threadIdx.x][0];
+    const int j = threadIdx.x;
+    const double s1 = ((red[0][j] + red[1][j]) + red[2][j]) + red[3][j];
+    const double s2 = ((red[0][kMaxAssets + j] + red[1][kMaxAssets + j]) +
+                       red[2][kMaxAssets + j]) + red[3][kMaxAssets + j];
+    block_sums[(int64_t)blockIdx.x * 2 * kMaxAssets + j] = s1;
+    block_sums[(int64_t)blockIdx.x * 2 * kMaxAssets + kMaxAssets + j] = s2;
   }
 }
 
