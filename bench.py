@@ -41,30 +41,213 @@ MATURITIES = (0.25, 0.5, 0.75, 1.0)
 NUM_STEPS = 365
 
 
-def cpu_baseline(n_paths, chunk=2**14):
-  """Times the oracle's numpy restatement (FP32 state) on host cores."""
+# ------------------------------------------------------------ CPU legs ----
+# The reference's TF CPU path cannot run here, so each CPU baseline times the
+# oracle's numpy restatement of the same algorithm on this machine's host
+# cores: one process per core (numpy's elementwise work is single-threaded;
+# the workers are started with `spawn` and import numpy, not torch), path
+# chunks spread over them, one warm-up run, then the median of three.
+
+
+def cpu_cores():
+  """Host cores this process may use: its CPU affinity, capped by
+  OMP_NUM_THREADS when that is set (the machine's CPU share)."""
+  n = len(os.sched_getaffinity(0))
+  cap = os.environ.get('OMP_NUM_THREADS', '')
+  if cap.isdigit() and int(cap) > 0:
+    n = min(n, int(cap))
+  return max(n, 1)
+
+
+def _worker_init():
+  for p in (ROOT, os.path.join(ROOT, 'tf-quant-finance_amd')):
+    if p not in sys.path:
+      sys.path.insert(0, p)
+  try:
+    import threadpoolctl
+    threadpoolctl.threadpool_limits(1)
+  except ImportError:
+    pass
+
+
+class CpuPool:
+  """`cores` spawned worker processes (context manager)."""
+
+  def __init__(self, cores):
+    import multiprocessing as mp
+    self.cores = cores
+    self.pool = mp.get_context('spawn').Pool(cores, initializer=_worker_init)
+
+  def __enter__(self):
+    return self
+
+  def __exit__(self, *exc):
+    self.pool.terminate()
+    self.pool.join()
+
+  def map(self, fn, items):
+    return self.pool.map(fn, items, chunksize=1)
+
+
+def median_time(run, runs=3):
+  """One warm-up call, then the median wall time of `runs` calls."""
+  run()
+  ts = []
+  for _ in range(runs):
+    t0 = time.perf_counter()
+    run()
+    ts.append(time.perf_counter() - t0)
+  return float(np.median(ts)), ts
+
+
+def _w_heston_sobol(a):
+  """Headline scheme on one chunk of paths: the oracle's 32-digit shifted
+  Sobol points, the engine's normal transform, the FT step loop and the
+  payoffs (float32 state)."""
   from oracle import heston as oheston
   from oracle import normal as onormal
-  from tqf.engine import pricing
-  import torch
-  all_times, keep = pricing.heston_grid_for(MATURITIES, NUM_STEPS,
-                                            'reference', torch.float32)
+  first, m, shift, all_times, keep = a
   steps = all_times.shape[0] - 1
-  shift = pricing.sobol_digital_shift(2 * steps, 7)
-  strikes = np.linspace(60, 140, 64)
-  t0 = time.perf_counter()
-  done = 0
-  while done < n_paths:
-    m = min(chunk, n_paths - done)
-    ints = oheston.shifted_sobol_ints(2 * steps, m, shift, first=done)
-    z = onormal.shifted_sobol_normals(ints).reshape(m, steps, 2)
-    oheston.ft_euler_calls(np.log(PARAMS['s0']), PARAMS['v0'],
-                           PARAMS['kappa'], PARAMS['theta'], PARAMS['xi'],
-                           PARAMS['rho'], PARAMS['r'], strikes, all_times,
-                           keep, z.astype(np.float32), dtype=np.float32)
-    done += m
-  dt = time.perf_counter() - t0
-  return n_paths * steps / dt, dt
+  ints = oheston.shifted_sobol_ints(2 * steps, m, shift, first=first)
+  z = onormal.shifted_sobol_normals(ints).reshape(m, steps, 2)
+  oheston.ft_euler_calls(np.log(PARAMS['s0']), PARAMS['v0'], PARAMS['kappa'],
+                         PARAMS['theta'], PARAMS['xi'], PARAMS['rho'],
+                         PARAMS['r'], np.linspace(60, 140, 64), all_times,
+                         keep, z.astype(np.float32), dtype=np.float32)
+  return m
+
+
+def cpu_baseline_headline(pool, all_times, keep, shift, n_paths,
+                          chunk=2**14):
+  steps = all_times.shape[0] - 1
+  work = [(f, min(chunk, n_paths - f), shift, all_times, keep)
+          for f in range(0, n_paths, chunk)]
+  t, ts = median_time(lambda: pool.map(_w_heston_sobol, work))
+  return {'value': n_paths * steps / t, 'unit': 'path-steps/s',
+          'cores': pool.cores, 'kind': 'port',
+          'sample': '%d paths x %d steps%s, numpy FP32 restatement of the '
+                    'same scheme incl. the Sobol points and normals, %d '
+                    'processes, median of 3 after a warm-up (%.2f s; runs %s '
+                    's)' % (n_paths, steps,
+                            ' (the full workload)' if n_paths == 2**20 else '',
+                            pool.cores, t, [round(x, 2) for x in ts])}
+
+
+def _w_gbm_euler(a):
+  """Plain GBM Euler on S, float64 (euler_sampling's step loop with the
+  GeometricBrownianMotion coefficients), 8 discounted calls."""
+  from oracle import euler as oeuler
+  seed, m, all_times, strikes = a
+  steps = all_times.shape[0] - 1
+  z = np.random.default_rng(seed).standard_normal((steps, m, 1))
+  drift, vol = oeuler.gbm_fns(0.05, 0.2)
+  keep = np.zeros(steps + 1, bool)
+  keep[-1] = True
+  x = oeuler.run(drift, vol, all_times, keep, 1, np.full((m, 1), 100.0), z,
+                 np.float64)
+  return np.exp(-0.05) * np.maximum(x[:, 0, :] - strikes, 0).sum(0)
+
+
+def cpu_baseline_gbm_euler(pool, all_times, n_paths=100000, chunk=6250):
+  work = [(i, min(chunk, n_paths - f), all_times, np.linspace(80, 120, 8))
+          for i, f in enumerate(range(0, n_paths, chunk))]
+  t, _ = median_time(lambda: pool.map(_w_gbm_euler, work))
+  steps = all_times.shape[0] - 1
+  return {'value': n_paths * steps / t, 'unit': 'path-steps/s',
+          'cores': pool.cores, 'kind': 'port',
+          'sample': 'full size (%d paths x %d steps, float64, numpy normals), '
+                    '%d processes, median of 3 after a warm-up (%.3f s)' % (
+                        n_paths, steps, pool.cores, t)}
+
+
+def _w_contract(a):
+  """One contract of the heterogeneous batch: price and pathwise Greeks
+  (GBM log-Euler: delta, vega, rho; Heston full truncation: forward
+  tangents of the step loop for delta, v0, r, kappa, theta, xi, rho)."""
+  from oracle import heston as oheston
+  c, m, seed = a
+  n, T, r = c['n_steps'], c['maturity'], c['rate']
+  rng = np.random.default_rng(seed)
+  if c['kind'] == 0:
+    dt = T / n
+    sig = c['sigma']
+    x = np.full(m, np.log(c['s0']))
+    w = np.zeros(m)
+    for _ in range(n):
+      dw = rng.standard_normal(m) * np.sqrt(dt)
+      x = x + (r - 0.5 * sig * sig) * dt + sig * dw
+      w = w + dw
+    s = np.exp(x)
+    disc = np.exp(-r * T)
+    itm = s > c['strike']
+    pay = disc * np.where(itm, s - c['strike'], 0)
+    sb = disc * np.where(itm, s, 0)
+    g = np.stack([pay, sb / c['s0'], sb * (w - sig * T), -T * pay + sb * T])
+    return g.mean(1)
+  z = rng.standard_normal((m, n, 2))
+  mean, _ = oheston.ft_euler_call_greeks(
+      np.log(c['s0']), c['v0'], c['kappa'], c['theta'], c['xi'], c['rho'], r,
+      c['strike'], np.linspace(0.0, T, n + 1), z)
+  return mean
+
+
+def cpu_baseline_batch(pool, contracts, total_path_steps, n_sample=64,
+                       max_paths=2048):
+  """64 contracts, every 64th of the table (stratified by position), at
+  most `max_paths` paths each; the time is extrapolated linearly in
+  path-steps to the whole table."""
+  idx = list(range(0, len(contracts), max(len(contracts) // n_sample, 1)))
+  idx = idx[:n_sample]
+  work = [(contracts[i], min(contracts[i]['n_paths'], max_paths), i)
+          for i in idx]
+  sample_ps = sum(m * c['n_steps'] for c, m, _ in work)
+  t, _ = median_time(lambda: pool.map(_w_contract, work))
+  t_full = t * total_path_steps / sample_ps
+  return {'value': len(contracts) / t_full, 'unit': 'options+Greeks/s',
+          'cores': pool.cores, 'kind': 'port',
+          'sample': '%d contracts (every %d-th), at most %d paths each '
+                    '(%.3g path-steps) in %.2f s (median of 3 after a '
+                    'warm-up, %d processes); extrapolated linearly in '
+                    'path-steps to the %.3g of the whole table' % (
+                        len(work), len(contracts) // n_sample, max_paths,
+                        sample_ps, t, pool.cores, total_path_steps)}
+
+
+def _w_basket(a):
+  from oracle import gbm as ogbm
+  s0, sig, corr, m, steps, seed = a
+  z = np.random.default_rng(seed).standard_normal((m, steps, len(s0)))
+  p = ogbm.basket_call_paths(s0, sig, corr, 0.02, 1.0, 100.0, z)
+  return p['pay'].sum()
+
+
+def cpu_baseline_basket(pool, s0, sig, corr, n_paths=262144, steps=64,
+                        chunk=8192):
+  work = [(s0, sig, corr, min(chunk, n_paths - f), steps, i)
+          for i, f in enumerate(range(0, n_paths, chunk))]
+  t, _ = median_time(lambda: pool.map(_w_basket, work))
+  return {'value': n_paths * steps / t, 'unit': 'path-steps/s',
+          'cores': pool.cores, 'kind': 'port',
+          'sample': 'full size (%d paths x %d steps x %d assets, float64, '
+                    'price + 33 pathwise Greeks), %d processes, median of 3 '
+                    'after a warm-up (%.2f s)' % (n_paths, steps, len(s0),
+                                                   pool.cores, t)}
+
+
+def _w_lsm(a):
+  """Polynomial LSM of the oracle on a subsample: the standardised basis
+  (the engine's judge) and the reference's raw basis with pinv."""
+  from oracle import lsm as olsm
+  paths, ex, df, n_cal, cores, standardise = a
+  try:
+    import threadpoolctl
+    threadpoolctl.threadpool_limits(cores)
+  except ImportError:
+    pass
+  return float(olsm.least_square_mc(
+      paths, ex, olsm.make_basket_call_max_payoff([100.0]),
+      olsm.make_polynomial_basis(2, standardise), discount_factors=df,
+      num_calibration_samples=n_cal)[0])
 
 
 def accuracy_check(prices, ses):
@@ -215,9 +398,10 @@ SECONDARY = ('gbm_euler_f64', 'batch_4096', 'learned_lsm_5asset',
 HBM_PEAK = 8e12  # bytes/s
 
 
-def secondary_metrics(only=None):
+def secondary_metrics(only=None, pool=None):
   """One timed run each of the other configurations (not the headline), or
-  of the one named by `only`."""
+  of the one named by `only`; with a CpuPool each also times its CPU
+  baseline."""
   import torch
 
   def want(name):
@@ -242,21 +426,26 @@ def secondary_metrics(only=None):
 
   # plain method: GBM Euler float64, 1e5 paths x 100 steps, 8 calls
   if want('gbm_euler_f64'):
-    _gbm_euler(out, timed, pricing, engine)
+    _gbm_euler(out, timed, pricing, engine, pool)
   if want('batch_4096'):
-    _batch(out, timed, batch_lib)
+    _batch(out, timed, batch_lib, pool)
   if want('learned_lsm_5asset'):
-    _learned_lsm(out)
+    _learned_lsm(out, pool)
   if want('hw_bermudan_2x2e20'):
     _hw_bermudan(out)
   if want('basket16_greeks'):
-    _basket16(out, timed, greeks)
+    _basket16(out, timed, greeks, pool)
   if want('paths_and_points'):
     _paths_and_points(out, timed, engine)
   return out
 
 
-def _gbm_euler(out, timed, pricing, engine):
+FLOP_GBM_EULER = 8.5  # FP64 per path-step (update 3, half a Box-Muller
+                      # pair 5, payoff 0.4 amortised)
+PEAK_FP64 = 78.6e12
+
+
+def _gbm_euler(out, timed, pricing, engine, pool):
   import torch
   all_times, keep = pricing.heston_grid_for([1.0], 100, 'reference',
                                             torch.float64)
@@ -266,11 +455,21 @@ def _gbm_euler(out, timed, pricing, engine):
                               discount_rate=0.05, dtype=torch.float64,
                               prepare=True)
   t, _ = timed(c1.launch, 10)
-  out['gbm_euler_f64'] = {'value': 1e7 / t, 'unit': 'path-steps/s',
-                          'seconds': t}
+  ach = FLOP_GBM_EULER * 1e7 / t
+  out['gbm_euler_f64'] = {
+      'value': 1e7 / t, 'unit': 'path-steps/s', 'seconds': t,
+      'roofline': {'bound': 'valu', 'achieved': ach / 1e12,
+                   'peak': PEAK_FP64 / 1e12, 'unit': 'TFLOP/s',
+                   'frac': ach / PEAK_FP64,
+                   'flop_per_path_step': FLOP_GBM_EULER,
+                   'note': 'one launch of 1e5 paths fills 391 workgroups: '
+                           'latency-bound, no roofline target'}}
+  if pool is not None:
+    out['gbm_euler_f64']['cpu_baseline'] = cpu_baseline_gbm_euler(
+        pool, np.asarray(all_times, np.float64))
 
 
-def _batch(out, timed, batch_lib):
+def _batch(out, timed, batch_lib, pool):
   # batched heterogeneous contracts with Greeks
   cs = batch_lib.random_contracts(4096)
   cfg, tune = batch_lib.autotune(cs, cache_path=None)
@@ -278,13 +477,36 @@ def _batch(out, timed, batch_lib):
   g = plan.capture()
   t, _ = timed(lambda: g.replay(), 2)
   acc = batch_accuracy(cs, plan.out[:, 0].cpu().numpy())
+  # the other Greek mode on the whole table, timed: the checkpointed
+  # reverse sweep and the forward tangents are two implementations of the
+  # same pathwise estimator on the same draws, so they must agree for every
+  # contract and Greek up to float32 rounding
+  other = 'adjoint' if cfg['greeks'] == 'tangent' else 'tangent'
+  plan2 = batch_lib.BatchPlan(cs, tile=cfg['tile'], n_blocks=cfg['n_blocks'],
+                              greeks=other)
+  t2, _ = timed(plan2.launch, 2)
+  a = plan.out.cpu().numpy()
+  b = plan2.out.cpu().numpy()
+  means = a[:, :8]
+  scale = np.maximum(np.abs(means), 1e-3 * np.abs(a[:, :1]))
+  acc['greeks_%s_vs_%s' % (other, cfg['greeks'])] = {
+      'contracts': len(cs),
+      'max_rel_diff': float(np.max(np.abs(b[:, :8] - means) / scale)),
+      'prices_identical': bool(np.array_equal(a[:, 0], b[:, 0]))}
   out['batch_4096'] = {'value': 4096 / t, 'unit': 'options+Greeks/s',
+                       ('%s_seconds' % other): t2,
                        'seconds': t, 'check': acc,
                        'path_steps': plan.total_path_steps,
                        'path_steps_per_s': plan.total_path_steps / t,
                        'flop': plan.total_flop,
-                       'achieved_tflops': plan.total_flop / t / 1e12,
-                       'frac_fp32_peak': plan.total_flop / t / PEAK_FP32,
+                       'roofline': {
+                           'bound': 'valu',
+                           'achieved': plan.total_flop / t / 1e12,
+                           'peak': PEAK_FP32 / 1e12, 'unit': 'TFLOP/s',
+                           'frac': plan.total_flop / t / PEAK_FP32,
+                           'kernel': 'batch_kernel',
+                           'flop_per_path_step': 'GBM 10, Heston 95 '
+                                                 '(tangent) / 99 (adjoint)'},
                        'autotuned': cfg,
                        'candidates_s': {'%s/%d/%d' % (r['greeks'], r['tile'],
                                                       r['n_blocks']):
@@ -299,24 +521,99 @@ def _batch(out, timed, batch_lib):
   out['batch_4096']['loop_fd_greeks_seconds'] = t_greeks
   out['batch_4096']['speedup_vs_loop_fd_greeks'] = t_greeks / t
   out['batch_4096']['speedup_vs_loop_price_only'] = t_price / t
+  if pool is not None:
+    out['batch_4096']['cpu_baseline'] = cpu_baseline_batch(
+        pool, cs, plan.total_path_steps)
 
 
-def _learned_lsm(out):
+FLOP_MLP_SAMPLE_STEP = 27264  # forward 4544 MAC + backward 2x, per sample
+FLOP_LSM_DATE = 6.4e10  # one K = 243 regression date at 2^20 calibration
+                        # paths (Gram, X^T y, continuation)
+PEAK_BF16 = 2.5e15
+
+
+def _learned_lsm(out, pool):
   import torch
   # learned LSM at full size: 5-asset Bermudan max-call, 2^20 training +
   # 2^20 pricing paths, 9 dates x 20 steps, MLP vs degree-2 polynomial
   from tqf.engine import mlp_lsm
+  from tqf.models import longstaff_schwartz as lsm_lib
+  ex = list(range(1, 10))
+  n_cal = 2**20
   torch.cuda.synchronize()
   t0 = time.perf_counter()
-  paths, df = mlp_lsm.bermudan_max_call_paths(2**21)
-  res = mlp_lsm.mlp_lsm_price(paths, list(range(1, 10)), 100.0, df, 2**20)
+  paths, df = mlp_lsm.bermudan_max_call_paths(2 * n_cal)
+  res = mlp_lsm.mlp_lsm_price(paths, ex, 100.0, df, n_cal)
   torch.cuda.synchronize()
-  out['learned_lsm_5asset'] = {
-      'seconds': time.perf_counter() - t0,
+  total = time.perf_counter() - t0
+  dfp = lsm_lib._padded_discounts(df, len(ex), torch.float64)
+  payoff = (lsm_lib.PAYOFF_MAX_CALL, np.array([100.0]))
+
+  def wall(fn):
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t1
+  # the two parts on their own (each already ran once above): the
+  # polynomial judge and the MLP training + exercise policy
+  t_poly = wall(lambda: lsm_lib._native_lsm(paths, ex, payoff, 2, dfp, n_cal,
+                                            torch.float64, paths.device))
+  t_mlp = wall(lambda: mlp_lsm.mlp_lsm_price(paths, ex, 100.0, df, n_cal,
+                                             judge=False))
+  mlp_flop = FLOP_MLP_SAMPLE_STEP * 65536 * 200 * (len(ex) - 1)
+  reg_flop = FLOP_LSM_DATE * (len(ex) - 1)
+  line = {
+      'seconds': total,
       'mlp_price': res['mlp_price'], 'mlp_se': res['mlp_se'],
       'poly_price': res['poly_price'], 'poly_se': res['poly_se'],
-      'gain': res['gain'], 'gain_se': res['gain_se'],
-      'accepted': res['accepted']}
+      'gain': res['gain'], 'gain_se_paired': res['gain_se'],
+      'gain_se_unpaired': res['gain_se_unpaired'],
+      'accepted': res['accepted'], 'price': res['price'],
+      'poly_seconds': t_poly, 'mlp_seconds': t_mlp,
+      'roofline': {'bound': 'mfma', 'achieved': mlp_flop / t_mlp / 1e12,
+                   'peak': PEAK_BF16 / 1e12, 'unit': 'TFLOP/s',
+                   'frac': mlp_flop / t_mlp / PEAK_BF16,
+                   'kernel': 'mlp training (mlp_grad_kernel + '
+                             'mlp_reduce_adam_kernel), 1600 Adam steps of '
+                             '65,536 samples',
+                   'flop_per_sample_step': FLOP_MLP_SAMPLE_STEP},
+      'roofline_regression': {
+          'bound': 'mfma', 'achieved': reg_flop / t_poly / 1e12,
+          'peak': PEAK_FP64 / 1e12, 'unit': 'TFLOP/s',
+          'frac': reg_flop / t_poly / PEAK_FP64,
+          'flop_per_date': FLOP_LSM_DATE}}
+  if pool is not None:
+    # the oracle's polynomial LSM on a subsample of the same paths: the
+    # first 2^14 calibration rows and the first 2^14 pricing rows
+    m = 2**14
+    sub = torch.cat([paths[:m], paths[n_cal:n_cal + m]]).contiguous()
+    sub_np = sub.cpu().numpy()
+    gpu_sub = float(lsm_lib._native_lsm(sub, ex, payoff, 2, dfp, m,
+                                        torch.float64, paths.device)[0])
+    item = [(sub_np, ex, df, m, pool.cores, True)]
+    t_cpu, ts = median_time(lambda: pool.map(_w_lsm, item))
+    want = pool.map(_w_lsm, item)[0]
+    raw = pool.map(_w_lsm, [(sub_np, ex, df, m, pool.cores, False)])[0]
+    scale = 2 * n_cal / (2 * m)
+    line['cpu_baseline'] = {
+        'value': t_cpu * scale, 'unit': 's (polynomial LSM, extrapolated)',
+        'cores': pool.cores, 'kind': 'port',
+        'sample': 'polynomial LSM only (K = 243, 8 regressions) on %d + %d '
+                  'rows of the same paths in %.2f s (median of 3 after a '
+                  'warm-up, BLAS on %d threads), extrapolated linearly in '
+                  'rows to 2^20 + 2^20 (x%d); the engine takes %.3f s for '
+                  'the same at full size' % (m, m, t_cpu, pool.cores, scale,
+                                             t_poly)}
+    line['check'] = {
+        'vs': 'oracle polynomial LSM (standardised basis, numpy pinv) on '
+              'the %d + %d-row subsample' % (m, m),
+        'engine_price': gpu_sub, 'oracle_price': want,
+        'rel_err': abs(gpu_sub - want) / abs(want),
+        'reference_semantics_price': raw,
+        'reference_semantics': 'pinv of the raw 243-column Gram with TF\'s '
+                               'cutoff (lsm.py:369-383) on the same rows'}
+  out['learned_lsm_5asset'] = line
   del paths
 
 
@@ -452,7 +749,10 @@ def _paths_and_points(out, timed, engine):
   out['paths_and_points'] = res
 
 
-def _basket16(out, timed, greeks):
+FLOP_BASKET16 = 384  # FP32 per path-step of 16 assets, forward
+
+
+def _basket16(out, timed, greeks, pool):
   # 16-asset basket with adjoint Greeks
   d = 16
   rs = np.random.RandomState(3)
@@ -462,8 +762,19 @@ def _basket16(out, timed, greeks):
   sig = rs.uniform(0.15, 0.45, d)
   t, _ = timed(lambda: greeks.basket_call_greeks(
       np.full(d, 100.0), sig, corr, 0.02, 1.0, 100.0, 64, 262144), 3)
-  out['basket16_greeks'] = {'value': 262144 * 64 / t, 'unit': 'path-steps/s',
-                            'seconds': t, 'greeks': 33}
+  ach = FLOP_BASKET16 * 262144 * 64 / t
+  out['basket16_greeks'] = {
+      'value': 262144 * 64 / t, 'unit': 'path-steps/s', 'seconds': t,
+      'greeks': 33,
+      'roofline': {'bound': 'valu', 'achieved': ach / 1e12,
+                   'peak': PEAK_FP32 / 1e12, 'unit': 'TFLOP/s',
+                   'frac': ach / PEAK_FP32, 'kernel': 'basket_kernel',
+                   'flop_per_path_step': FLOP_BASKET16,
+                   'note': 'forward FLOP only; the adjoint work of the 33 '
+                           'Greeks is not counted'}}
+  if pool is not None:
+    out['basket16_greeks']['cpu_baseline'] = cpu_baseline_basket(
+        pool, np.full(d, 100.0), sig, corr)
 
 
 def main():
@@ -472,7 +783,9 @@ def main():
   ap.add_argument('--steps', type=int, default=20)
   ap.add_argument('--warmup', type=int, default=3)
   ap.add_argument('--paths', type=int, default=2**20)
-  ap.add_argument('--cpu-paths', type=int, default=2**17)
+  ap.add_argument('--cpu-paths', type=int, default=2**20,
+                  help='paths of the headline CPU baseline (default: the '
+                       'full workload)')
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--no-check', action='store_true')
   ap.add_argument('--quick', action='store_true',
@@ -490,6 +803,16 @@ def main():
           file=sys.stderr)
     sys.exit(2)
 
+  # the CPU workers are spawned before this process touches the GPU
+  pool = None if (args.no_cpu or args.calib_copy) else CpuPool(cpu_cores())
+  try:
+    run(args, pool)
+  finally:
+    if pool is not None:
+      pool.__exit__(None, None, None)
+
+
+def run(args, pool):
   import torch
   if args.calib_copy:
     n = args.calib_copy << 18  # float32 elements
@@ -506,7 +829,8 @@ def main():
   dev = torch.device('cuda', 0)
   torch.cuda.set_device(dev)
   if args.only:
-    print(json.dumps({args.only: secondary_metrics(args.only)[args.only]}))
+    print(json.dumps({args.only: secondary_metrics(args.only,
+                                                   pool)[args.only]}))
     return
   launch = pricing.heston_ft_calls(num_paths=args.paths,
                                    num_time_steps=NUM_STEPS,
@@ -554,15 +878,15 @@ def main():
 
   secondary = None
   if not args.quick:
-    secondary = secondary_metrics()
+    secondary = secondary_metrics(pool=pool)
 
   cpu = None
-  if not args.no_cpu:
-    rate, secs = cpu_baseline(args.cpu_paths)
-    cpu = {'value': rate, 'unit': 'path-steps/s', 'cores': 1, 'kind': 'port',
-           'sample': '%d paths x %d steps, numpy FP32 restatement of the '
-                     'same scheme incl. Sobol normals (%.1f s)' %
-                     (args.cpu_paths, steps_per_path, secs)}
+  if pool is not None:
+    all_times, keep = pricing.heston_grid_for(MATURITIES, NUM_STEPS,
+                                              'reference', torch.float32)
+    shift = pricing.sobol_digital_shift(2 * (all_times.shape[0] - 1), 7)
+    cpu = cpu_baseline_headline(pool, np.asarray(all_times), keep,
+                                np.asarray(shift), args.cpu_paths)
 
   out = {
       'metric': METRIC,

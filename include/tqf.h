@@ -205,7 +205,8 @@ typedef struct tqf_rng {
 typedef struct tqf_output {
   int32_t mode;
   int32_t n_strikes;         /* PAYOFF                                      */
-  void* paths;               /* PATH_MAJOR / STEP_MAJOR                     */
+  void* paths;               /* PATH_MAJOR / STEP_MAJOR, aligned to one
+                                record (dim values of the dtype)           */
   const double* strikes;     /* PAYOFF: device [n_strikes] (calls)          */
   const double* discount;    /* PAYOFF: device [n_slots] discount factors   */
   double* workspace;         /* PAYOFF: tqf_euler_workspace_size() bytes    */
@@ -242,6 +243,47 @@ int tqf_euler(const tqf_model* model, const tqf_grid* grid,
               const tqf_rng* rng, const tqf_output* out, int64_t n_paths,
               const void* x0, int64_t x0_stride_path, int32_t dtype,
               void* stream);
+
+/* ---- Forward tangents of the fused Euler paths -------------------------- */
+
+#define TQF_TANGENT_VJP 0  /* gradient of sum(g * paths) w.r.t. the inputs */
+#define TQF_TANGENT_JVP 1  /* tangent paths in a direction of the inputs   */
+
+/* Inputs of a model, in order: the initial state x0[0..dim), then its
+ * parameters -- GBM: mu, sigma; CONST: a, b; HESTON_FT (either variant):
+ * r, kappa, theta, xi, rho.  The paths are those tqf_euler writes in
+ * TQF_OUT_PATH_MAJOR layout [N][k][dim] for the same descriptors. */
+typedef struct tqf_tangent {
+  int32_t mode;               /* TQF_TANGENT_*                              */
+  int32_t pad0;
+  const void* grad_paths;     /* VJP: device [N][k][dim] incoming gradient  */
+  double* grad_params;        /* VJP: device [dim + n_par] sums over paths;
+                                 the x0 entries are 0 with per-path x0      */
+  void* grad_x0;              /* VJP with x0_stride_path = dim: device
+                                 [N][dim] per-path x0 gradient              */
+  const double* direction;    /* JVP: device [dim + n_par]                  */
+  const void* dir_x0;         /* JVP: device [N][dim] per-path x0 direction,
+                                 or NULL to use direction[0..dim)           */
+  void* tangent_paths;        /* JVP: device [N][k][dim] output             */
+  void* workspace;            /* VJP: tqf_euler_tangent_workspace_size()    */
+} tqf_tangent;
+
+int64_t tqf_euler_tangent_workspace_size(int64_t n_paths);
+
+/* Device.  Re-simulates the paths of tqf_euler (same normals, same update,
+ * bit for bit) carrying the forward tangents of the state with respect to
+ * every input, in registers: memory does not grow with the number of steps.
+ * VJP: grad_params[q] = sum_{p, slot, c} g[p][slot][c] dX[p][slot][c] /
+ * dinput_q (fixed-order reduction).  JVP: tangent_paths = sum_q
+ * direction_q dX / dinput_q.  rng kinds: DRAWS, PHILOX, PHILOX_ANTITHETIC
+ * (Sobol draws are passed as DRAWS).  Replaces the forward-Jacobian loop of
+ * euler_sampling.sample with watch_params (tff/models/euler_sampling.py:
+ * 467-510, tff/math/custom_loops.py:20-214) and the tape gradients taken
+ * through it (tff/math/gradient.py:22-235). */
+int tqf_euler_tangent(const tqf_model* model, const tqf_grid* grid,
+                      const tqf_rng* rng, const tqf_tangent* tangent,
+                      int64_t n_paths, const void* x0, int64_t x0_stride_path,
+                      int32_t dtype, void* stream);
 
 /* ---- Exact geometric Brownian motion ----------------------------------- */
 
@@ -285,7 +327,10 @@ typedef struct tqf_lsm {
   int32_t basis_log;        /* 0: basis on the paths' values; 1: on their
                                logarithms (the payoff always uses values)    */
   const uint8_t* exponents; /* device [n_basis][dim] exponent table in the
-                               reference's meshgrid order (lsm.py:116-118)   */
+                               reference's meshgrid order (lsm.py:116-118);
+                               every entry <= degree and n_basis <=
+                               (degree + 1)^dim (tqf_lsm_price checks both
+                               and returns TQF_EINVAL)                       */
   double strike;
   int64_t num_calibration;  /* rows used for the regression (0 = all)       */
   double rcond;             /* pivot cutoff relative to the largest scaled
@@ -406,7 +451,10 @@ int64_t tqf_batch_workspace_size(int32_t n_items, int32_t n_blocks,
  * contract are contiguous rows; `order` (device [n_items], or NULL for row
  * order) is the processing order, longest item first), with
  * pathwise Greeks (Heston: checkpoints every 16 steps and a reverse sweep,
- * or forward tangents, per contract).  Philox stream: key = seed, sub-stream = the contract's
+ * or forward tangents, per contract).  The workspace holds checkpoints for
+ * max_steps steps (tqf_batch_workspace_size): a Heston contract with
+ * adjoint Greeks and n_steps > max_steps gets the forward tangents (the
+ * same estimator) instead of the reverse sweep.  Philox stream: key = seed, sub-stream = the contract's
  * stream_id, path index = first_path + offset.  out: device [n_contracts][16] doubles:
  * 8 means (price, Greeks) then their 8 standard errors.  The launch records
  * no memset, so it can be captured in a hipGraph.  Replaces sampling with

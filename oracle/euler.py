@@ -12,11 +12,14 @@
                     step i, component k sit at sequence dimension i*d + k.
 
 The drift/volatility callables take numpy arrays [N, d] and return [N, d] /
-[N, d, d].  Pinned by closed forms: with d=1, a = mu x, b = sigma x, the
-Euler path is S0 * prod(1 + mu dt_i + sigma sqrt(dt_i) z_i) (a product form,
-not this update); Wiener paths (a = 0, b = 1) are cumsum(sqrt(dt) z);
-E[X_t] = x0 + 2/3 mu t^1.5 for a = mu sqrt(t) (euler_sampling_test.py:206-294)
-checked statistically.
+[N, d, d].  Pinned by closed forms (tests/test_oracle.py): with d=1,
+a = mu x, b = sigma x, the Euler path is S0 * prod(1 + mu dt_i + sigma
+sqrt(dt_i) z_i) (a product form, not this update); Wiener paths (a = 0,
+b = 1) are cumsum(sqrt(dt) z); with b = 0 and a = mu sqrt(t) on a
+non-uniform grid the value is exactly the right Riemann sum
+x0 + sum dt_i mu sqrt(t_{i+1}) (coefficients at times[i+1]; the left sum
+differs by 1e-2); E[X_t] = x0 + 2/3 mu t^1.5 for a = mu sqrt(t),
+b = a t + b0 (euler_sampling_test.py:206-294) on the reference's grid.
 """
 import numpy as np
 

@@ -42,8 +42,13 @@ def make_polynomial_basis(degree, standardize=False):
     if standardize:
       sd = sl.std(axis=1, keepdims=True)
       c = c / np.where(sd > 0, sd, 1.0)
-    e = exponent_grid(degree, x.shape[-1]).astype(x.dtype)  # [K, d]
-    out = np.prod(c[:, :, None, :]**e[None, None], axis=-1)  # [B, N, K]
+    e = exponent_grid(degree, x.shape[-1])  # [K, d]
+    # prod_j c_j^e_kj, the factors multiplied in asset order as
+    # reduce_prod does, from a power table (no [B, N, K, d] intermediate)
+    pw = c[..., None]**np.arange(degree + 1).astype(x.dtype)  # [B, N, d, D]
+    out = pw[:, :, 0, e[:, 0]]
+    for j in range(1, x.shape[-1]):
+      out = out * pw[:, :, j, e[:, j]]  # [B, N, K]
     return np.transpose(out, (0, 2, 1))
   return basis
 
@@ -91,16 +96,19 @@ def _expected_exercise(design, calib, cont, ex):
     sub, y = masked, cont
   else:
     sub, y = masked[:, calib], cont[calib]
-  lhs = np.einsum('bnk,bnl->bkl', sub, sub)
-  rhs = np.einsum('bnk,nb->bk', sub, y)
-  beta = np.einsum('bkl,bl->bk', _pinv(lhs), rhs)
-  c = np.einsum('bnk,bk->bn', dt, beta)
+  subt = np.transpose(sub, (0, 2, 1))
+  lhs = np.matmul(subt, sub)  # X^T X, [B, K, K]
+  rhs = np.matmul(subt, np.transpose(y)[..., None])[..., 0]  # X^T y
+  beta = np.matmul(_pinv(lhs), rhs[..., None])[..., 0]
+  c = np.matmul(dt, beta[..., None])[..., 0]  # [B, N]
   return np.maximum(c.T, 0)
 
 
 def least_square_mc(sample_paths, exercise_times, payoff_fn, basis_fn,
                     discount_factors=None, num_calibration_samples=None,
-                    dtype=None, return_betas=False):
+                    dtype=None, return_path_values=False):
+  """Price per payoff batch entry; with return_path_values also the
+  discounted per-path values [N_price, B] whose mean is the price."""
   x = np.asarray(sample_paths, dtype=dtype)
   dtype = x.dtype
   ex_t = list(np.asarray(exercise_times).reshape(-1))
@@ -126,7 +134,6 @@ def least_square_mc(sample_paths, exercise_times, payoff_fn, basis_fn,
   calib = None
   if num_calibration_samples is not None:
     calib = np.arange(num_calibration_samples)
-  betas = []
   e = n_times - 1
   while e > 0:
     ti = ex_t[e - 1]
@@ -142,4 +149,6 @@ def least_square_mc(sample_paths, exercise_times, payoff_fn, basis_fn,
   pv = disc(cash + vals, 0)
   if num_calibration_samples is not None:
     pv = pv[num_calibration_samples:]
+  if return_path_values:
+    return pv.mean(axis=0), pv
   return pv.mean(axis=0)

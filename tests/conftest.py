@@ -2,6 +2,7 @@
 import os
 import sys
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -26,3 +27,16 @@ def load_golden(name):
   import json
   with open(os.path.join(GOLDEN, name)) as f:
     return json.load(f)
+
+
+# a(t, x) = mu sqrt(t) on a non-uniform grid, shared with the GPU tests
+COEF_TIMING_GRID = np.array([0.0, 0.03, 0.1, 0.12, 0.3, 0.31, 0.55, 0.8, 1.0])
+
+
+def right_riemann_drift(mu, x0, grid):
+  """x0 + sum_{i < j} dt_i mu sqrt(t_{i+1}) for every grid point j > 0, and
+  the same with sqrt(t_i) (what a step that read the coefficients at the
+  start of its interval would give)."""
+  dt = np.diff(grid)
+  return (x0 + np.cumsum(dt * mu * np.sqrt(grid[1:])),
+          x0 + np.cumsum(dt * mu * np.sqrt(grid[:-1])))

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import load_golden
+from conftest import COEF_TIMING_GRID, load_golden, right_riemann_drift
 from oracle import closed_form, euler as oeuler, gbm as ogbm
 from oracle import heston as oheston, normal as onormal, rng as orng
 from oracle import sobol as osobol
@@ -283,6 +283,160 @@ def test_gbm_euler_prices_within_3se_of_black_scholes():
   p, se = _np(res['prices'])[0], _np(res['stderrs'])[0]
   bs = closed_form.bs_price(100, strikes, 1.0, 0.05, 0.2)
   assert np.all(np.abs(p - bs) <= 3 * se), (p - bs) / se
+
+
+def test_generic_loop_and_local_vol_kernel_read_coefficients_at_next_time():
+  # zero draws: each value is the right Riemann sum of the drift, which
+  # tells times[i + 1] (euler_sampling.py:519) from times[i]
+  mu, x0 = 0.2, 0.1
+  n_steps = COEF_TIMING_GRID.shape[0] - 1
+  zeros = np.zeros((3, n_steps, 1))
+  right, left = right_riemann_drift(mu, x0, COEF_TIMING_GRID)
+  got = tqf.models.euler_sampling.sample(
+      1, lambda t, x: mu * torch.sqrt(t) * torch.ones_like(x),
+      lambda t, x: torch.zeros(x.shape + (1,), dtype=x.dtype, device=x.device),
+      [0.3, 1.0], times_grid=COEF_TIMING_GRID, initial_state=[x0],
+      normal_draws=zeros, dtype=torch.float64)
+  np.testing.assert_allclose(_np(got)[:, :, 0], [[right[3], right[-1]]] * 3,
+                             rtol=1e-14)
+  # the fused local-volatility kernel on a time-only surface whose nodes are
+  # the grid times: log S_T = log S_0 + sum dt_i (r - q - sigma(t_{i+1})^2/2)
+  sig = 0.2 + 0.3 * COEF_TIMING_GRID
+  xs = np.linspace(np.log(50.), np.log(200.), 5)
+  surf = tqf.models.LocalVolSurface(COEF_TIMING_GRID, xs[0], xs[1] - xs[0],
+                                    np.repeat(sig[:, None], 5, 1))
+  model = tqf.models.LocalVolatilityModel(
+      1, risk_free_rate=0.03, dividend_yield=0.01, local_volatility_fn=surf,
+      dtype=torch.float64)
+  assert engine.spec_of(model.drift_fn(), model.volatility_fn()) is not None
+  s = model.sample_paths([0.3, 1.0], initial_state=[100.0],
+                         times_grid=COEF_TIMING_GRID, normal_draws=zeros)
+  dt = np.diff(COEF_TIMING_GRID)
+  want = np.log(100.) + np.cumsum(dt * (0.02 - 0.5 * sig[1:]**2))
+  want_left = np.log(100.) + np.cumsum(dt * (0.02 - 0.5 * sig[:-1]**2))
+  np.testing.assert_allclose(np.log(_np(s))[:, :, 0],
+                             [[want[3], want[-1]]] * 3, rtol=1e-13)
+  assert abs(want[-1] - want_left[-1]) > 1e-3
+
+
+def test_generic_loop_time_dependent_process_mean():
+  # euler_sampling_test.py:206-294: dX = mu sqrt(t) dt + (a t + b) dW has
+  # E[X_t] = x0 + 2/3 mu t^1.5
+  mu, a, b, x0 = 0.2, 0.4, 0.33, 0.1
+  times = np.array([0.0, 0.1, 0.21, 0.32, 0.43, 0.55])
+  kw = dict(num_samples=10000, initial_state=[x0],
+            random_type=mvn.RandomType.PSEUDO_ANTITHETIC, seed=42,
+            time_step=0.01, dtype=torch.float64)
+  paths = _np(tqf.models.euler_sampling.sample(
+      1, lambda t, x: mu * torch.sqrt(t) * torch.ones_like(x),
+      lambda t, x: (a * t + b) * torch.ones(x.shape + (1,), dtype=x.dtype,
+                                            device=x.device), times, **kw))
+  assert paths.shape == (10000, 6, 1)
+  np.testing.assert_allclose(paths.mean(0)[:, 0],
+                             x0 + 2.0 / 3.0 * mu * times**1.5, rtol=1e-2,
+                             atol=1e-2)
+  # antithetic pairs cancel the noise: the mean is the right Riemann sum
+  from oracle import grid as ogrid
+  t, _, idx = ogrid.prepare_grid(times, np.float64, time_step=0.01)
+  right, _ = right_riemann_drift(mu, x0, t)
+  np.testing.assert_allclose(paths.mean(0)[1:, 0], right[idx[1:] - 1],
+                             rtol=1e-12)
+
+
+def test_euler_float32_parity_on_the_headline_grid():
+  # the headline workload's own grid (num_time_steps=365, maturities .25 ..
+  # 1: 366 points) in float32 with injected draws, 2^14 paths
+  print('euler float32 parity, headline grid', flush=True)
+  spec, fns, x0 = _models()['heston_ft']
+  times = np.array([0.25, 0.5, 0.75, 1.0], np.float32)
+  from oracle import grid as ogrid
+  t, _, _ = ogrid.prepare_grid(times, np.float32, num_time_steps=365)
+  assert t.shape[0] == 366
+  n = 2**14
+  z = np.random.RandomState(29).standard_normal((n, 365, 2)).astype(
+      np.float32)
+  want = oeuler.sample(2, fns[0], fns[1], times, num_samples=n,
+                       initial_state=np.asarray(x0, np.float32),
+                       normal_draws=z, num_time_steps=365, dtype=np.float32)
+  drift = engine.tag(lambda t, x: None, spec)
+  vol = engine.tag(lambda t, x: None, spec)
+  got = _np(tqf.models.euler_sampling.sample(
+      2, drift, vol, times, num_time_steps=365,
+      initial_state=np.asarray(x0, np.float32), normal_draws=z,
+      dtype=torch.float32))
+  assert got.shape == want.shape == (n, 4, 2)
+  for c in range(2):
+    _close(got[..., c], want[..., c], 2e-5)
+
+
+def test_heston_qe_sample_paths_price_the_smile():
+  # the reference's cross-check of HestonModel.sample_paths against the
+  # semi-analytic price (heston_model_test.py:290-380), here at rho = -0.7
+  # and xi = 0.6 where the QE spot update's correlation terms carry the
+  # skew, 2^20 antithetic paths, every strike within 3 standard errors
+  print('heston QE vs semi-analytic', flush=True)
+  kappa, theta, xi, rho, v0 = 1.5, 0.04, 0.6, -0.7, 0.04
+  model = tqf.models.HestonModel(kappa, theta, xi, rho, dtype=torch.float64)
+  paths = model.sample_paths([0.5, 1.0], [np.log(100.), v0],
+                             num_samples=2**20, num_time_steps=100,
+                             random_type=mvn.RandomType.PSEUDO_ANTITHETIC,
+                             seed=5)
+  s = np.exp(_np(paths)[:, -1, 0])
+  for k in (70.0, 85.0, 100.0, 115.0, 130.0):
+    pay = np.maximum(s - k, 0)
+    want = oheston.european_call(100.0, k, 1.0, 0.0, v0, kappa, theta, xi,
+                                 rho)
+    se = pay.std(ddof=1) / np.sqrt(s.size)
+    assert abs(pay.mean() - want) < 3 * se, (k, pay.mean(), want, se)
+  assert abs(s.mean() - 100.0) < 3 * s.std() / np.sqrt(s.size)
+
+
+def test_batch_shape_generic_loop_reference_case():
+  # euler_sampling_test.py:206-294 with use_batch: initial_state [2, 1, 1]
+  # and a volatility with batch axes [2, 1, 1, 1]; paths [2, N, 6, 1]
+  mu, a, b = 0.2, 0.4, 0.33
+  times = np.array([0.0, 0.1, 0.21, 0.32, 0.43, 0.55])
+  x0 = np.array([[[0.1]], [[0.1]]])
+  paths = _np(tqf.models.euler_sampling.sample(
+      1, lambda t, x: mu * torch.sqrt(t) * torch.ones_like(x),
+      lambda t, x: (a * t + b) * torch.ones([2, 1, 1, 1], dtype=x.dtype,
+                                            device=x.device),
+      times, num_samples=10000, initial_state=x0,
+      random_type=mvn.RandomType.PSEUDO_ANTITHETIC, seed=42, time_step=0.01,
+      dtype=torch.float64))
+  assert paths.shape == (2, 10000, 6, 1)
+  means = paths.mean(1).reshape(2, 1, 6)
+  np.testing.assert_allclose(means, x0 + 2.0 / 3.0 * mu * times**1.5,
+                             rtol=1e-2, atol=1e-2)
+  # the batch members have independent draws
+  assert not np.allclose(paths[0], paths[1])
+
+
+def test_batch_shape_fused_equals_one_run_per_member():
+  # a batch of initial states and batched draws for one fused model: one
+  # launch over the flattened paths, equal to a run per member
+  rs = np.random.RandomState(12)
+  n, steps = 500, 20
+  z = rs.standard_normal((3, n, steps, 1))
+  x0 = np.array([90.0, 100.0, 110.0]).reshape(3, 1, 1)
+  gbm = tqf.models.GeometricBrownianMotion(0.05, 0.2, dtype=torch.float64)
+  got = _np(tqf.models.euler_sampling.sample(
+      1, gbm.drift_fn(), gbm.volatility_fn(), [0.5, 1.0],
+      num_time_steps=steps, initial_state=x0, normal_draws=z,
+      dtype=torch.float64))
+  assert got.shape == (3, n, 2, 1)
+  for b in range(3):
+    one = _np(tqf.models.euler_sampling.sample(
+        1, gbm.drift_fn(), gbm.volatility_fn(), [0.5, 1.0],
+        num_time_steps=steps, initial_state=x0[b, 0], normal_draws=z[b],
+        dtype=torch.float64))
+    np.testing.assert_array_equal(got[b], one)
+  # and the same batch through the generic loop (plain callables)
+  loop = _np(tqf.models.euler_sampling.sample(
+      1, lambda t, x: 0.05 * x, lambda t, x: 0.2 * x.unsqueeze(-1),
+      [0.5, 1.0], num_time_steps=steps, initial_state=x0, normal_draws=z,
+      dtype=torch.float64))
+  np.testing.assert_allclose(loop, got, rtol=1e-12)
 
 
 def test_generic_callables_match_fused_kernel():

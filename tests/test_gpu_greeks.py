@@ -108,6 +108,24 @@ def test_batch_tangent_greeks_equal_adjoint_greeks():
       np.max(np.abs(tan[:, means] - adj[:, means]) / scale))
 
 
+def test_batch_contract_longer_than_the_checkpoint_workspace_takes_tangents():
+  # a C caller may size the checkpoint workspace for fewer steps than an
+  # adjoint-mode Heston contract takes: that contract falls back to the
+  # forward tangents (same estimator) and nothing is written past the
+  # workspace; the others keep the reverse sweep
+  cs = [_heston(0, steps=200, paths=3000), _heston(1, steps=40, paths=3000)]
+  short = batch_lib.BatchPlan(cs, tile=512, seed=6)
+  assert short.max_steps == 200
+  short.max_steps = 48  # checkpoints for 3 segments of 16 steps
+  got = short.launch().cpu().numpy()
+  tan = batch_lib.BatchPlan(cs, tile=512, seed=6,
+                            greeks='tangent').launch().cpu().numpy()
+  adj = batch_lib.BatchPlan(cs, tile=512, seed=6).launch().cpu().numpy()
+  np.testing.assert_array_equal(got[:, 0], adj[:, 0])
+  np.testing.assert_allclose(got[0], tan[0], rtol=1e-6, atol=1e-9)
+  np.testing.assert_array_equal(got[1], adj[1])
+
+
 def test_batch_equals_loop_of_single_contracts():
   cs = [_gbm(0), _heston(1), _gbm(2, sigma=0.4, steps=77, paths=5000),
         _heston(3, steps=121, paths=3000)]
@@ -226,3 +244,179 @@ def test_watch_params_gradients_through_the_generic_loop():
   assert abs(vega - fd) <= 1e-5 * abs(fd) + 1e-6, (vega, fd)
   bs = closed_form.bs_vega(100.0, k, 1.0, r, 0.2)
   assert abs(vega - bs) <= 0.03 * bs, (vega, bs)
+
+
+def test_notebook_greeks_through_generic_ito_process_and_fwd_gradient():
+  # Monte_Carlo_Euler_Scheme.ipynb (cells 10 and 22-27): log-GBM through
+  # GenericItoProcess with callables that close over sigma, watch_params=
+  # [sigma], PSEUDO_ANTITHETIC draws, time_step 0.01, 200,000 paths,
+  # expiries [0.5, 1.0] x strikes [600, 650, 680]; delta and vega by
+  # tqf.math.fwd_gradient against Black-Scholes.  The notebook's own
+  # maximum relative errors (delta 1.71e-3, vega 7.78e-2, :917-920) set the
+  # scale of the bound: a different random stream, the same estimator.
+  print('notebook Greeks via fwd_gradient', flush=True)
+  import tqf
+  from conftest import load_golden
+  from tqf.math.random import multivariate_normal as mvn
+  g = load_golden('notebook_euler.json')
+  dev = 'cuda'
+  f64 = dict(dtype=torch.float64, device=dev)
+  rate = 0.03
+  expiries = torch.tensor([0.5, 1.0], **f64)
+  strikes = torch.tensor([600.0, 650.0, 680.0], **f64)
+  spot = torch.tensor(700.0, **f64)
+  sigma = torch.tensor(0.1, **f64)
+
+  def price_eu_options(strikes, spot, sigma):
+    def drift_fn(t, x):
+      del t, x
+      return rate - 0.5 * sigma**2
+
+    def vol_fn(t, x):
+      del t, x
+      return torch.reshape(sigma, [1, 1])
+    process = tqf.models.GenericItoProcess(dim=1, drift_fn=drift_fn,
+                                           volatility_fn=vol_fn,
+                                           dtype=torch.float64)
+    paths = process.sample_paths(
+        [0.5, 1.0], num_samples=200000,
+        initial_state=torch.log(torch.mean(spot)), watch_params=[sigma],
+        random_type=mvn.RandomType.PSEUDO_ANTITHETIC, seed=42, time_step=0.01)
+    return (torch.exp(-rate * expiries).unsqueeze(-1) *
+            torch.mean(torch.relu(torch.exp(paths) - strikes), 0))
+
+  prices = price_eu_options(strikes, spot, sigma)
+  assert prices.shape == (2, 3)
+  delta = tqf.math.fwd_gradient(
+      lambda s: price_eu_options(strikes, s, sigma), spot)
+  vega = tqf.math.fwd_gradient(
+      lambda v: price_eu_options(strikes, spot, v), sigma)
+  t = np.array([[0.5], [1.0]])
+  k = np.array([[600.0, 650.0, 680.0]])
+  bs = closed_form.bs_price(700.0, k, t, rate, 0.1)
+  bs_delta = closed_form.bs_delta(700.0, k, t, rate, 0.1)
+  bs_vega = closed_form.bs_vega(700.0, k, t, rate, 0.1)
+  np.testing.assert_allclose(prices.detach().cpu().numpy(), bs, rtol=2e-3)
+  err_d = np.max(np.abs(delta.cpu().numpy() / bs_delta - 1))
+  err_v = np.max(np.abs(vega.cpu().numpy() / bs_vega - 1))
+  print('max relative error: delta %.2e (notebook %.2e), vega %.2e '
+        '(notebook %.2e)' % (err_d, g['max_rel_err_delta'], err_v,
+                             g['max_rel_err_vega']), flush=True)
+  assert err_d <= 3 * g['max_rel_err_delta']
+  assert err_v <= 3 * g['max_rel_err_vega']
+  # the reverse-mode route gives the same sums over the output grid
+  d_sum = tqf.math.gradients(lambda s: price_eu_options(strikes, s, sigma),
+                             spot)
+  np.testing.assert_allclose(float(d_sum), float(delta.sum()), rtol=1e-10)
+
+
+def test_generic_ito_process_routes_model_callables_to_the_fused_kernel():
+  import tqf
+  gbm = tqf.models.GeometricBrownianMotion(0.05, 0.2, dtype=torch.float64)
+  proc = tqf.models.GenericItoProcess(1, gbm.drift_fn(), gbm.volatility_fn(),
+                                      dtype=torch.float64)
+  z = np.random.RandomState(4).standard_normal((1000, 20, 1))
+  a = proc.sample_paths([0.5, 1.0], initial_state=[100.0], num_time_steps=20,
+                        normal_draws=z)
+  b = tqf.models.euler_sampling.sample(
+      1, lambda t, x: 0.05 * x, lambda t, x: 0.2 * x.unsqueeze(-1),
+      [0.5, 1.0], initial_state=[100.0], num_time_steps=20, normal_draws=z,
+      dtype=torch.float64)
+  np.testing.assert_allclose(a.cpu().numpy(), b.cpu().numpy(), rtol=1e-12)
+  with pytest.raises(NotImplementedError):
+    proc.fd_solver_backward(1.0, 0.0, None, None)
+  with pytest.raises(ValueError):
+    tqf.models.GenericItoProcess(0, gbm.drift_fn(), gbm.volatility_fn())
+
+
+def test_gbm_sample_paths_gradients_match_the_per_path_derivatives():
+  # GeometricBrownianMotion.sample_paths is differentiable in S0, mu and
+  # sigma (the reference tapes it for Greeks): against the per-path
+  # derivatives of the exact scheme on the same draws
+  import tqf
+  rs = np.random.RandomState(6)
+  n, times = 4000, np.array([0.25, 0.6, 1.0])
+  z = rs.standard_normal((n, 3, 1))
+  w = rs.standard_normal((n, 3, 1))
+  f64 = dict(dtype=torch.float64, device='cuda')
+  mu = torch.tensor(0.04, requires_grad=True, **f64)
+  sig = torch.tensor(0.3, requires_grad=True, **f64)
+  s0 = torch.tensor(100.0, requires_grad=True, **f64)
+  gbm = tqf.models.GeometricBrownianMotion(mu, sig, dtype=torch.float64)
+  p = gbm.sample_paths(times, initial_state=s0, normal_draws=z)
+  got = torch.autograd.grad((p * torch.as_tensor(w, device='cuda')).sum(),
+                            [s0, mu, sig])
+  s = p.detach().cpu().numpy()
+  t = times.reshape(1, -1, 1)
+  wt = np.cumsum(np.sqrt(np.diff(np.concatenate([[0.0], times]))).reshape(
+      1, -1, 1) * z, axis=1)
+  want = [np.sum(w * s / 100.0), np.sum(w * s * t),
+          np.sum(w * s * (wt - 0.3 * t))]
+  np.testing.assert_allclose([float(g) for g in got], want, rtol=1e-10)
+  # a call's pathwise vega from the taped sampler against Black-Scholes
+  sig2 = torch.tensor(0.2, requires_grad=True, **f64)
+  g2 = tqf.models.GeometricBrownianMotion(0.03, sig2, dtype=torch.float64)
+  st = g2.sample_paths([1.0], initial_state=100.0, num_samples=2**20,
+                       seed=4)[:, -1, 0]
+  price = torch.exp(torch.tensor(-0.03, **f64)) * torch.relu(st - 100).mean()
+  vega = float(torch.autograd.grad(price, sig2)[0])
+  bs = closed_form.bs_vega(100.0, 100.0, 1.0, 0.03, 0.2)
+  assert abs(vega - bs) < 0.01 * bs, (vega, bs)
+
+
+@pytest.mark.parametrize('greeks', ['tangent', 'adjoint'])
+def test_batch_table_sample_price_and_greeks_vs_oracle(greeks):
+  # 32 contracts of the heterogeneous 4096-contract table (every 128th:
+  # GBM and Heston, 50-500 steps, maturities 0.1-3), at most 2048 paths
+  # each: price and every Greek against the float64 oracle on the same
+  # Philox draws (Heston: forward tangents of the step loop; GBM: the
+  # closed-form pathwise terms)
+  print('batch table sample vs oracle', greeks, flush=True)
+  table = batch_lib.random_contracts(4096)
+  cs = []
+  for i in range(0, 4096, 128):
+    c = dict(table[i])
+    c['n_paths'] = min(c['n_paths'], 2048)
+    c['stream_id'] = i
+    cs.append(c)
+  plan = batch_lib.BatchPlan(cs, tile=1024, seed=5, greeks=greeks)
+  plan.launch()
+  res = plan.results()
+  for c, r in zip(cs, res):
+    n, m = c['n_steps'], c['n_paths']
+    if c['kind'] == batch_lib.KIND_GBM:
+      z = orng.normals(m, n, 5, stream=c['stream_id'],
+                       dtype=np.float32).astype(np.float64)
+      dt = c['maturity'] / n
+      w = np.sqrt(dt) * z.sum(1)
+      s = c['s0'] * np.exp((c['rate'] - 0.5 * c['sigma']**2) * c['maturity'] +
+                           c['sigma'] * w)
+      disc = np.exp(-c['rate'] * c['maturity'])
+      itm = s > c['strike']
+      pay = disc * np.where(itm, s - c['strike'], 0)
+      sb = disc * np.where(itm, s, 0)
+      g = np.stack([pay, sb / c['s0'], sb * (w - c['sigma'] * c['maturity']),
+                    -c['maturity'] * pay + sb * c['maturity']])
+      want, want_se = g.mean(1), g.std(1, ddof=1) / np.sqrt(m)
+    else:
+      z = orng.normals(m, 2 * n, 5, stream=c['stream_id'],
+                       dtype=np.float32).astype(np.float64).reshape(m, n, 2)
+      want, want_se = oheston.ft_euler_call_greeks(
+          np.log(c['s0']), c['v0'], c['kappa'], c['theta'], c['xi'], c['rho'],
+          c['rate'], c['strike'], np.linspace(0.0, c['maturity'], n + 1), z)
+      # kernel order: price, delta, d/dv0, rho_r, kappa, theta, xi, rho
+    names = batch_lib.GREEK_NAMES[c['kind']]
+    if (c['kind'] == batch_lib.KIND_HESTON and
+        2 * c['kappa'] * c['theta'] < c['xi']**2):
+      # Feller condition violated: the variance reaches 0 and the pathwise
+      # variance-parameter Greeks carry 1/sqrt(v) factors with unbounded
+      # tails (float32 and float64 then differ by more than their standard
+      # errors); price, delta and rate rho stay well conditioned
+      names = names[:2] + names[3:4]
+    price = want[0]
+    for name in names:
+      q = batch_lib.GREEK_NAMES[c['kind']].index(name)
+      got, se = r[name]
+      scale = max(abs(want[q]), 1e-2 * abs(price), 1e-6)
+      assert abs(got - want[q]) <= 2e-3 * scale + 0.1 * want_se[q], (
+          c['kind'], name, got, want[q], want_se[q])

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 from scipy import special, stats
 
-from conftest import load_golden
+from conftest import COEF_TIMING_GRID, load_golden, right_riemann_drift
 from oracle import closed_form, euler, gbm, grid, heston, lsm, normal, rng, sobol
 
 
@@ -167,6 +167,47 @@ def test_euler_wiener_is_cumsum_and_records():
   np.testing.assert_allclose(out[:, :, 0], w[:, idx], rtol=1e-12, atol=1e-15)
 
 
+def test_euler_coefficients_are_evaluated_at_the_next_grid_time():
+  # zero volatility, drift mu sqrt(t): the Euler value is exactly the right
+  # Riemann sum of the drift (euler_sampling.py:519 reads times[i + 1]);
+  # the left sum differs by ~1e-2 here, far outside the tolerance
+  mu, x0 = 0.2, 0.1
+  drift = lambda t, x: mu * np.sqrt(t) * np.ones_like(x)
+  vol = lambda t, x: np.zeros(x.shape + (1,))
+  n_steps = COEF_TIMING_GRID.shape[0] - 1
+  out = euler.sample(1, drift, vol, [0.3, 1.0], num_samples=3,
+                     initial_state=[x0], normal_draws=np.zeros((3, n_steps, 1)),
+                     times_grid=COEF_TIMING_GRID)
+  right, left = right_riemann_drift(mu, x0, COEF_TIMING_GRID)
+  np.testing.assert_allclose(out[:, 0, 0], right[3], rtol=1e-14)
+  np.testing.assert_allclose(out[:, 1, 0], right[-1], rtol=1e-14)
+  assert abs(right[-1] - left[-1]) > 1e-2
+
+
+def test_euler_time_dependent_process_mean():
+  # euler_sampling_test.py:206-294: dX = mu sqrt(t) dt + (a t + b) dW has
+  # E[X_t] = x0 + 2/3 mu t^1.5; antithetic draws, the reference's grid
+  mu, a, b, x0 = 0.2, 0.4, 0.33, 0.1
+  times = np.array([0.0, 0.1, 0.21, 0.32, 0.43, 0.55])
+  drift = lambda t, x: mu * np.sqrt(t) * np.ones_like(x)
+  vol = lambda t, x: (a * t + b) * np.ones(x.shape + (1,))
+  t, _, _ = grid.prepare_grid(times, np.float64, time_step=0.01)
+  assert t.shape[0] - 1 == 55
+  z = np.random.RandomState(42).standard_normal((5000, 55, 1))
+  z = np.concatenate([z, -z], 0)
+  out = euler.sample(1, drift, vol, times, num_samples=10000,
+                     initial_state=[x0], normal_draws=z, time_step=0.01)
+  assert out.shape == (10000, 6, 1)
+  np.testing.assert_allclose(out.mean(0)[:, 0],
+                             x0 + 2.0 / 3.0 * mu * times**1.5, rtol=1e-2,
+                             atol=1e-2)
+  # with antithetic draws the mean is the deterministic right Riemann sum
+  right, _ = right_riemann_drift(mu, x0, t)
+  _, _, idx = grid.prepare_grid(times, np.float64, time_step=0.01)
+  np.testing.assert_allclose(out.mean(0)[1:, 0], right[idx[1:] - 1],
+                             rtol=1e-12)
+
+
 # ---------------------------------------------------------------- Heston ---
 
 def test_heston_coefficient_goldens():
@@ -218,6 +259,57 @@ def test_heston_qe_small_volvol_is_gbm():
   v = out[:, -1, 1]
   np.testing.assert_allclose(v, 0.04, rtol=2e-2)
   assert abs(v.mean() - 0.04) < 4 * v.std() / np.sqrt(n)
+
+
+# strongly correlated, high vol-of-vol Heston: every rho / xi term of the
+# QE log-spot update (heston_model.py:554-572) moves these prices
+QE_CASE = dict(kappa=1.5, theta=0.04, xi=0.6, rho=-0.7, v0=0.04, s0=100.0,
+               t=1.0, strikes=(70.0, 85.0, 100.0, 115.0, 130.0))
+
+
+def _qe_terminal_spots(n_half, steps, seed, rho_spot_sign=1.0):
+  """QE terminal spots with antithetic draws; `rho_spot_sign = -1` plants a
+  sign error in the log-spot update's correlation terms only."""
+  c = QE_CASE
+  dt = c['t'] / steps
+  z = np.random.RandomState(seed).standard_normal((n_half, steps, 2))
+  z = np.concatenate([z, -z], 0)
+  x = np.full(2 * n_half, np.log(c['s0']))
+  v = np.full(2 * n_half, c['v0'])
+  for i in range(steps):
+    vn = heston.update_variance(c['kappa'], c['theta'], c['xi'], v, dt,
+                                z[:, i, 0])
+    x = heston.update_log_spot(c['kappa'], c['theta'], c['xi'],
+                               rho_spot_sign * c['rho'], v, vn, x, dt,
+                               z[:, i, 1])
+    v = vn
+  return np.exp(x)
+
+
+def _qe_call_errors_in_se(spots):
+  c = QE_CASE
+  errs = []
+  for k in c['strikes']:
+    pay = np.maximum(spots - k, 0)
+    want = heston.european_call(c['s0'], k, c['t'], 0.0, c['v0'], c['kappa'],
+                                c['theta'], c['xi'], c['rho'])
+    errs.append((pay.mean() - want) / (pay.std(ddof=1) / np.sqrt(spots.size)))
+  return np.array(errs)
+
+
+def test_heston_qe_log_spot_prices_match_semi_analytic_with_correlation():
+  # the reference's own cross-check (heston_model_test.py:290-380), at
+  # rho = -0.7 and xi = 0.6 where the QE spot update's correlation terms
+  # carry the skew: the QE oracle's call prices over the smile and its
+  # martingale mean agree with the independent Lewis integral
+  spots = _qe_terminal_spots(2**16, 50, 11)
+  errs = _qe_call_errors_in_se(spots)
+  assert np.all(np.abs(errs) < 3), errs
+  assert abs(spots.mean() - 100.0) < 3 * spots.std() / np.sqrt(spots.size)
+  # a sign error in those terms (planted here, in the oracle's inputs) moves
+  # the smile by tens of standard errors: the check discriminates
+  bad = _qe_call_errors_in_se(_qe_terminal_spots(2**16, 50, 11, -1.0))
+  assert np.max(np.abs(bad)) > 10, bad
 
 
 # ---------------------------------------------------------------- GBM ------
@@ -414,3 +506,38 @@ def test_halton_oracle_known_values_and_radical_inverse():
   np.testing.assert_array_equal(
       halton.sample(3, num_results=40, perms=perms),
       halton.sample(3, num_results=40))
+
+
+def test_heston_ft_tangent_greeks_match_central_differences():
+  # forward tangents through the full-truncation step loop against central
+  # differences of the price on the same draws (Feller condition kept so
+  # the truncation kinks do not dominate)
+  rs = np.random.RandomState(8)
+  n, steps = 20000, 40
+  z = rs.standard_normal((n, steps, 2))
+  t = np.linspace(0.0, 1.0, steps + 1)
+  keep = np.zeros(steps + 1, bool)
+  keep[-1] = True
+  p = dict(x0=np.log(100.0), v0=0.04, kappa=2.0, theta=0.05, xi=0.3,
+           rho=-0.6, r=0.03)
+  mean, se = heston.ft_euler_call_greeks(
+      p['x0'], p['v0'], p['kappa'], p['theta'], p['xi'], p['rho'], p['r'],
+      100.0, t, z)
+
+  def price(**kw):
+    q = dict(p, **kw)
+    pr, _ = heston.ft_euler_calls(q['x0'], q['v0'], q['kappa'], q['theta'],
+                                  q['xi'], q['rho'], q['r'], [100.0], t, keep,
+                                  z)
+    return pr[-1, 0]
+  np.testing.assert_allclose(mean[0], price(), rtol=1e-12)
+  for k, (name, h) in enumerate([('s0', 1e-4), ('v0', 1e-7), ('r', 1e-7),
+                                 ('kappa', 1e-6), ('theta', 1e-7),
+                                 ('xi', 1e-6), ('rho', 1e-6)]):
+    if name == 's0':
+      fd = (price(x0=np.log(100.0 + h)) - price(x0=np.log(100.0 - h))) / (2 * h)
+    else:
+      fd = (price(**{name: p[name] + h}) - price(**{name: p[name] - h})) / (
+          2 * h)
+    assert abs(mean[k + 1] - fd) <= 1e-4 * abs(fd) + 0.01 * se[k + 1], (
+        name, mean[k + 1], fd)

@@ -291,7 +291,7 @@ batch_kernel(const tqf_contract* __restrict__ contracts,
              const int32_t* __restrict__ order, int n_items,
              uint32_t k0, uint32_t k1, int* __restrict__ counter,
              float* __restrict__ ckpt, int64_t ckpt_per_block,
-             double* __restrict__ item_sums, int use_counter) {
+             int max_seg, double* __restrict__ item_sums, int use_counter) {
   __shared__ int s_item;
   __shared__ double red[kBlockB / kWave][2 * kStats];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -317,7 +317,11 @@ batch_kernel(const tqf_contract* __restrict__ contracts,
       float g[kStats];
       if (c.kind == TQF_CONTRACT_GBM)
         gbm_path(c, p, c.stream, k0, k1, g);
-      else if (!kAdjoint || c.greeks_mode == TQF_GREEKS_TANGENT)
+      else if (!kAdjoint || c.greeks_mode == TQF_GREEKS_TANGENT ||
+               (c.n_steps + kSeg - 1) / kSeg > max_seg)
+        // a contract with more steps than the checkpoint workspace holds
+        // (n_steps > max_steps) takes the forward tangents, the same
+        // estimator, rather than write past its checkpoint column
         heston_path_tangent(c, p, c.stream, k0, k1, g);
       else
         heston_path(c, p, c.stream, k0, k1, ck, kBlockB, g);
@@ -403,7 +407,7 @@ extern "C" int tqf_batch_price(const tqf_contract* contracts,
   int* counter = (int*)w;
   w += 256;
   float* ckpt = (float*)w;
-  const int64_t nseg = (max_steps + kSeg - 1) / kSeg;
+  const int nseg = (max_steps + kSeg - 1) / kSeg;
   const int64_t per_block = (int64_t)kBlockB * nseg * 2;
   hipLaunchKernelGGL(zero_counter_kernel, 1, 1, 0, s, counter);
   const int use_counter = flags & TQF_BATCH_COUNTER;
@@ -411,12 +415,12 @@ extern "C" int tqf_batch_price(const tqf_contract* contracts,
     hipLaunchKernelGGL(batch_kernel<false>, n_blocks, kBlockB, 0, s, contracts,
                        items, order, n_items, (uint32_t)seed,
                        (uint32_t)(seed >> 32), counter, ckpt, per_block,
-                       item_sums, use_counter);
+                       nseg, item_sums, use_counter);
   else
     hipLaunchKernelGGL(batch_kernel<true>, n_blocks, kBlockB, 0, s, contracts,
                        items, order, n_items, (uint32_t)seed,
                        (uint32_t)(seed >> 32), counter, ckpt, per_block,
-                       item_sums, use_counter);
+                       nseg, item_sums, use_counter);
   hipLaunchKernelGGL(batch_finalize_kernel, (n_contracts + 255) / 256, 256, 0,
                      s, item_sums, contract_first_item, contract_n_items,
                      contracts, n_contracts, out);

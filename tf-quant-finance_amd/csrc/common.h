@@ -80,9 +80,13 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& z0,
 
 // ---------------------------------------------------------------- erfinv --
 // float32: erfinv(x) = x P(w), w = -ln(1 - x^2); P is a degree-8 polynomial
-// in w - 2.5 (w < 5) or sqrt(w) - 3 (w >= 5), fitted for this library by
-// least squares against scipy (csrc/fit_erfinv.py); max relative error
-// 2.9e-7 over float32 inputs with 1 - x^2 rounded once.
+// in w - 2.5 (w < 5) or sqrt(w) - 3 (w >= 5).  The form (the variable w, the
+// split at w = 5 and the two shifted polynomial variables) is the one M.
+// Giles published for single precision ("Approximating the erfinv
+// function", GPU Computing Gems Jade Edition, 2011); the coefficients were
+// refitted for this library by least squares against scipy
+// (csrc/fit_erfinv.py); max relative error 2.9e-7 over float32 inputs with
+// 1 - x^2 rounded once.
 __device__ __forceinline__ float erfinv_f32(float x) {
   // w = -ln(1 - x^2) = -ln2 log2(1 - x^2); 1 - x^2 >= 2^-24 here, so the
   // raw v_log_f32 needs no denormal scaling

@@ -300,21 +300,43 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
   Normals<R, RNG> nrm;
   nrm.init(a.rng, pc, D, lds, a.n_steps);
 
-  // path-major staging (block-uniform bookkeeping; flushes synchronise
-  // the block, so they only happen at the uniform record steps)
+  // path-major staging (block-uniform bookkeeping, flushed at the uniform
+  // record steps).  Each wave flushes the rows of its own 64 paths, so no
+  // workgroup barrier is needed: a wave's LDS accesses execute in order.
+  // One store moves one whole record (D values); the lanes of a store cover
+  // 64 / nrec consecutive paths, each as one contiguous run of nrec
+  // records.  A full flush has nrec = stage_recs, a power of two.
   int nrec = 0, first_slot = 0;
   auto flush = [&]() {
-    __syncthreads();
-    const int seg = nrec * D;
-    const int64_t p0 = (int64_t)blockIdx.x * kBlock;
-    for (int e = threadIdx.x; e < kBlock * seg; e += kBlock) {
-      const int pl = e / seg, rem = e - pl * seg;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t p0 = (int64_t)blockIdx.x * kBlock + wave * kWave;
+    const R* rows = lds_pay + wave * kWave * kStageStride;
+    const bool pow2 = (nrec & (nrec - 1)) == 0;
+    const int sh = 31 - __builtin_clz(nrec);
+    for (int it = 0; it < nrec; ++it) {
+      const int u = it * kWave + lane;
+      const int pl = pow2 ? u >> sh : u / nrec;
+      const int rec = u - pl * nrec;
       const int64_t pg = p0 + pl;
-      if (pg < a.n_paths)
-        a.paths[(pg * a.n_slots + first_slot) * D + rem] =
-            lds_pay[pl * kStageStride + rem];
+      const R* src = rows + pl * kStageStride + rec * D;
+      R* dst = a.paths + (pg * a.n_slots + first_slot + rec) * D;
+      if (pg < a.n_paths) {
+        if constexpr (D == 2 && sizeof(R) == 4) {
+          // one 8-byte store per record
+          typedef float f32x2v __attribute__((ext_vector_type(2)));
+          *(f32x2v*)dst = f32x2v{src[0], src[1]};
+        } else {
+#pragma unroll
+          for (int c = 0; c < D; ++c) dst[c] = src[c];
+        }
+      }
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     nrec = 0;
   };
   auto put = [&](int slot, R off) {
@@ -604,6 +626,270 @@ int launch_typed(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
   return rc;
 }
 
+// ---- forward tangents (differentiable sampling) ----------------------------
+// The fused paths as a function of (x0, model parameters): one thread per
+// path re-simulates it with the forward kernel's own normals and Stepper
+// (the same path bit for bit) and carries the tangents of the state with
+// respect to every input, as custom_loops.for_loop carries the Jacobian of
+// the loop state (tff/math/custom_loops.py:20-214, euler_sampling.py:
+// 467-510).  Memory does not grow with the number of steps.
+//   VJP: sum over paths, records and components of g * dX/dtheta (g the
+//        incoming gradient of the [N][k][d] paths), per-block partials
+//        reduced in a fixed order.
+//   JVP: the tangent paths sum_q u_q dX/dtheta_q in the paths' layout.
+// Inputs theta: x0[0..d) then the model's parameters
+//   GBM (mu, sigma); CONST (a, b); HESTON_FT (r, kappa, theta, xi, rho).
+constexpr int kMaxTan = 7;
+
+template <typename R>
+struct TangentArgs {
+  int mode;
+  int n_par;                 // model parameters (inputs after x0)
+  R xi, rho, kappa, theta;   // Heston raw parameters
+  const R* grad_paths;       // VJP [N][k][d]
+  double* partials;          // VJP [blocks][kMaxTan]
+  R* grad_x0;                // VJP, per-path x0: [N][d] or null
+  const double* direction;   // JVP [d + n_par]
+  const R* dir_x0;           // JVP, per-path x0 direction [N][d] or null
+  R* tangent_paths;          // JVP [N][k][d]
+};
+
+template <typename R, int MODEL, int RNG>
+__global__ void __launch_bounds__(kBlock)
+euler_tangent_kernel(EulerArgs<R> a, TangentArgs<R> t) {
+  constexpr int D = ModelDim<MODEL>::D;
+  constexpr bool kHeston =
+      MODEL == TQF_MODEL_HESTON_FT || MODEL == kModelHestonOwn;
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ double red[kWaves][kMaxTan];
+  const int64_t path = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool active = path < a.n_paths;
+  const int64_t pc = active ? path : a.n_paths - 1;
+  const int m = D + t.n_par;  // inputs
+  R x[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) x[c] = a.x0[pc * a.x0_stride + c];
+  // T[c][q] = dX_c / dtheta_q
+  R T[D][kMaxTan];
+#pragma unroll
+  for (int c = 0; c < D; ++c)
+#pragma unroll
+    for (int q = 0; q < kMaxTan; ++q) T[c][q] = (c == q) ? (R)1 : (R)0;
+  double acc[kMaxTan];
+#pragma unroll
+  for (int q = 0; q < kMaxTan; ++q) acc[q] = 0;
+  double accx0[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) accx0[c] = 0;
+  // JVP direction of this path
+  R u[kMaxTan];
+  if (t.mode == TQF_TANGENT_JVP) {
+#pragma unroll
+    for (int q = 0; q < kMaxTan; ++q)
+      u[q] = q < m ? (R)ld_uniform(t.direction, q) : (R)0;
+    if (t.dir_x0) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) u[c] = t.dir_x0[pc * D + c];
+    }
+  }
+  // the r-input of the Heston log-spot: the recorded value carries + r t
+  const int q_r = D;  // HESTON_FT parameter 0 is r
+  auto rec = [&](int slot, R time) {
+    if (!active) return;
+    const int64_t o = (path * a.n_slots + slot) * D;
+    if (t.mode == TQF_TANGENT_VJP) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const double g = (double)t.grad_paths[o + c];
+#pragma unroll
+        for (int q = 0; q < kMaxTan; ++q) acc[q] += g * (double)T[c][q];
+        if (kHeston && c == 0) acc[q_r] += g * (double)time;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        R v = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxTan; ++q) v += u[q] * T[c][q];
+        if (kHeston && c == 0) v += u[q_r] * time;
+        t.tangent_paths[o + c] = v;
+      }
+    }
+  };
+  if (a.init_slot >= 0) rec(a.init_slot, (R)0);
+
+  Normals<R, RNG> nrm;
+  nrm.init(a.rng, pc, D, lds, a.n_steps);
+  const int n_dims = a.n_steps * D;
+  for (int c0 = 0; c0 < a.n_steps; c0 += kSobolChunk) {
+    nrm.refill(a.rng, c0, n_dims);
+    const int c1 = c0 + kSobolChunk < a.n_steps ? c0 + kSobolChunk : a.n_steps;
+    for (int i = c0; i < c1; ++i) {
+      R z[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) z[k] = nrm.get(a.rng, i, k);
+      StepRow<R> row;
+      R sdt;
+      int s;
+      const R dt = ld_uniform(a.dt, i);
+      if constexpr (kHasStepRow<MODEL>) {
+        row = ld_row(a.coef, i);
+        sdt = row.c[4];
+        s = row_slot(row.c[5]);
+      } else {
+        sdt = ld_uniform(a.sqrt_dt, i);
+        s = ld_uniform(a.slot, i);
+      }
+      // tangents from the state before the step
+      if (MODEL == TQF_MODEL_GBM) {
+        const R dw = z[0] * sdt;
+        const R f = ((R)1 + dt * a.p[0]) + a.p[1] * dw;
+#pragma unroll
+        for (int q = 0; q < kMaxTan; ++q) T[0][q] *= f;
+        T[0][D + 0] += dt * x[0];   // mu
+        T[0][D + 1] += x[0] * dw;   // sigma
+      } else if (MODEL == TQF_MODEL_CONST) {
+        T[0][D + 0] += dt;          // a
+        T[0][D + 1] += z[0] * sdt;  // b
+      } else if (kHeston) {
+        const R v = x[1];
+        const bool ft = MODEL == TQF_MODEL_HESTON_FT;
+        const bool pos = v > (R)0;
+        const R vp = ft ? (pos ? v : (R)0) : v;
+        const R sq = sqrt(ft ? vp : fabs(v));
+        // d sq / d v
+        const R hs = sq > (R)0 ? (ft ? (R)0.5 / sq : (pos ? (R)0.5 : (R)-0.5) / sq)
+                               : (R)0;
+        const R dw0 = z[0] * sdt, dw1 = z[1] * sdt;
+        const R s1r = sqrt((R)1 - t.rho * t.rho);
+        const R cz = a.p[3] * dw0 + a.p[4] * dw1;  // xi rho dw0 + xi s dw1
+#pragma unroll
+        for (int q = 0; q < kMaxTan; ++q) {
+          const R tvp = ft ? (pos ? T[1][q] : (R)0) : T[1][q];
+          const R tsq = T[1][q] * hs;
+          T[0][q] += (R)-0.5 * dt * tvp + tsq * dw0;
+          T[1][q] += -t.kappa * dt * tvp + tsq * cz;
+        }
+        T[1][D + 1] += dt * (t.theta - vp);                    // kappa
+        T[1][D + 2] += dt * t.kappa;                           // theta
+        T[1][D + 3] += sq * (t.rho * dw0 + s1r * dw1);         // xi
+        T[1][D + 4] += t.xi * sq * (dw0 - t.rho / s1r * dw1);  // rho
+      }
+      Stepper<R, MODEL>::step(a, x, z, dt, sdt, nullptr, 0, (R)0, i, row);
+      if (s >= 0) rec(s, ld_uniform(a.t_next, i));
+    }
+  }
+  if (t.mode != TQF_TANGENT_VJP) return;
+  if (t.grad_x0) {
+    // per-path initial states: their gradient is per path
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) t.grad_x0[path * D + c] = (R)acc[c];
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kMaxTan; ++q) {
+    double v = acc[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxTan) {
+    double v = 0;
+    for (int w = 0; w < kWaves; ++w) v += red[w][threadIdx.x];
+    t.partials[(int64_t)blockIdx.x * kMaxTan + threadIdx.x] = v;
+  }
+}
+
+// Fixed-order sum of the per-block tangent partials: thread q of one
+// workgroup per input adds the blocks in order through a 256-lane tree.
+__global__ void __launch_bounds__(256)
+tangent_reduce_kernel(const double* __restrict__ partials, int64_t n_blocks,
+                      int m, double* __restrict__ out) {
+  __shared__ double r[256];
+  const int q = blockIdx.x;
+  double s = 0;
+  for (int64_t b = threadIdx.x; b < n_blocks; b += 256)
+    s += partials[b * kMaxTan + q];
+  r[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) r[threadIdx.x] += r[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && q < m) out[q] = r[0];
+}
+
+template <typename R>
+int launch_tangent(const tqf_model* m, const tqf_grid* g, const tqf_rng* r,
+                   const tqf_tangent* tg, int64_t n_paths, const void* x0,
+                   int64_t x0_stride, hipStream_t stream) {
+  EulerArgs<R> a{};
+  for (int i = 0; i < 8; ++i) a.p[i] = (R)m->p[i];
+  TangentArgs<R> t{};
+  const int kind = (m->kind == TQF_MODEL_HESTON_FT && m->p[5] != 0.0)
+                       ? kModelHestonOwn
+                       : m->kind;
+  const int D = m->kind == TQF_MODEL_HESTON_FT ? 2 : 1;
+  if (m->kind == TQF_MODEL_HESTON_FT) {
+    const double xi = m->p[3], rho = m->p[4];
+    a.p[3] = (R)(xi * rho);
+    a.p[4] = (R)(xi * std::sqrt(1.0 - rho * rho));
+    t.kappa = (R)m->p[1];
+    t.theta = (R)m->p[2];
+    t.xi = (R)xi;
+    t.rho = (R)rho;
+    t.n_par = 5;
+  } else {
+    t.n_par = 2;
+  }
+  a.coef = (const R*)m->step_coef;
+  a.dt = (const R*)g->dt;
+  a.sqrt_dt = (const R*)g->sqrt_dt;
+  a.t_next = (const R*)g->t_next;
+  a.slot = g->slot;
+  a.n_steps = g->n_steps;
+  a.n_slots = g->n_slots;
+  a.init_slot = g->init_slot;
+  fill_rng_args(a.rng, r, n_paths);
+  a.n_paths = n_paths;
+  a.x0 = (const R*)x0;
+  a.x0_stride = x0_stride;
+  t.mode = tg->mode;
+  t.grad_paths = (const R*)tg->grad_paths;
+  t.partials = (double*)tg->workspace;
+  t.grad_x0 = (R*)tg->grad_x0;
+  t.direction = tg->direction;
+  t.dir_x0 = (const R*)tg->dir_x0;
+  t.tangent_paths = (R*)tg->tangent_paths;
+  void (*kern)(EulerArgs<R>, TangentArgs<R>) = nullptr;
+#define TQF_PICK_T(MODEL, RNG) \
+  if (kind == MODEL && r->kind == RNG) kern = euler_tangent_kernel<R, MODEL, RNG>;
+#define TQF_PICK_T_RNG(MODEL)                   \
+  TQF_PICK_T(MODEL, TQF_RNG_DRAWS)              \
+  TQF_PICK_T(MODEL, TQF_RNG_PHILOX)             \
+  TQF_PICK_T(MODEL, TQF_RNG_PHILOX_ANTITHETIC)
+  TQF_PICK_T_RNG(TQF_MODEL_GBM)
+  TQF_PICK_T_RNG(TQF_MODEL_CONST)
+  TQF_PICK_T_RNG(TQF_MODEL_HESTON_FT)
+  TQF_PICK_T_RNG(kModelHestonOwn)
+#undef TQF_PICK_T_RNG
+#undef TQF_PICK_T
+  if (!kern)
+    return set_error(TQF_ENOTIMPL, "no tangent kernel for this model/rng");
+  const unsigned n_blocks = (unsigned)((n_paths + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(kern, dim3(n_blocks), dim3(kBlock), 16, stream, a, t);
+  int rc = check_launch("tqf_euler_tangent");
+  if (rc || tg->mode != TQF_TANGENT_VJP) return rc;
+  hipLaunchKernelGGL(tangent_reduce_kernel, dim3(D + t.n_par), dim3(256), 0,
+                     stream, (const double*)tg->workspace, (int64_t)n_blocks,
+                     D + t.n_par, tg->grad_params);
+  return check_launch("tqf_euler_tangent reduce");
+}
+
 }  // namespace
 
 extern "C" int64_t tqf_euler_workspace_size(int64_t n_paths, int32_t n_slots,
@@ -695,6 +981,9 @@ extern "C" int tqf_euler(const tqf_model* m, const tqf_grid* g,
                 (int)(48 * 1024 / (kWaves * 2 * dtype_size(dtype))));
   } else {
     TQF_REQUIRE(out->paths != nullptr, "null output paths");
+    TQF_REQUIRE(((uintptr_t)out->paths % (D * dtype_size(dtype))) == 0,
+                "output paths must be aligned to one record (%d bytes)",
+                (int)(D * dtype_size(dtype)));
   }
   const hipStream_t s = (hipStream_t)stream;
   if (dtype == TQF_F32)
@@ -715,4 +1004,58 @@ extern "C" int tqf_payoff_finalize(double* workspace, int64_t n_paths,
               "null payoff buffers");
   return launch_finalize(workspace, n_paths, n_slots, n_strikes, discount,
                          prices, stderrs, sums, (hipStream_t)stream);
+}
+
+extern "C" int64_t tqf_euler_tangent_workspace_size(int64_t n_paths) {
+  const int64_t n_blocks = (n_paths + kBlock - 1) / kBlock;
+  return n_blocks * kMaxTan * (int64_t)sizeof(double);
+}
+
+extern "C" int tqf_euler_tangent(const tqf_model* m, const tqf_grid* g,
+                                 const tqf_rng* r, const tqf_tangent* tg,
+                                 int64_t n_paths, const void* x0,
+                                 int64_t x0_stride_path, int32_t dtype,
+                                 void* stream) {
+  clear_error();
+  TQF_REQUIRE(m && g && r && tg, "null descriptor");
+  TQF_REQUIRE(dtype == TQF_F32 || dtype == TQF_F64, "bad dtype %d", dtype);
+  TQF_REQUIRE(n_paths >= 1, "num_samples must be positive");
+  TQF_REQUIRE(m->kind == TQF_MODEL_GBM || m->kind == TQF_MODEL_CONST ||
+                  m->kind == TQF_MODEL_HESTON_FT,
+              "tangents exist for the GBM, constant-coefficient and Heston "
+              "Euler models, not kind %d", m->kind);
+  TQF_REQUIRE(r->kind == TQF_RNG_DRAWS || r->kind == TQF_RNG_PHILOX ||
+                  r->kind == TQF_RNG_PHILOX_ANTITHETIC,
+              "tangents read injected draws or the Philox stream, not rng "
+              "kind %d", r->kind);
+  const int D = m->kind == TQF_MODEL_HESTON_FT ? 2 : 1;
+  TQF_REQUIRE(m->dim == D, "model dim %d does not match kind", m->dim);
+  TQF_REQUIRE(m->kind != TQF_MODEL_HESTON_FT || m->step_coef ||
+                  g->n_steps == 0,
+              "Heston models need their per-step constants (step_coef)");
+  TQF_REQUIRE(g->n_steps >= 0 && g->n_slots >= 1, "bad grid sizes");
+  TQF_REQUIRE(g->init_slot >= -1 && g->init_slot < g->n_slots, "bad init_slot");
+  TQF_REQUIRE(g->n_steps == 0 || (g->dt && g->sqrt_dt && g->slot && g->t_next),
+              "null grid arrays");
+  TQF_REQUIRE(x0 != nullptr, "null initial state");
+  TQF_REQUIRE(x0_stride_path == 0 || x0_stride_path == D,
+              "x0_stride_path must be 0 or dim");
+  TQF_REQUIRE(r->kind != TQF_RNG_DRAWS || r->draws, "null normal_draws");
+  TQF_REQUIRE(r->kind != TQF_RNG_PHILOX_ANTITHETIC || n_paths % 2 == 0,
+              "antithetic sampling needs an even number of paths");
+  if (tg->mode == TQF_TANGENT_VJP) {
+    TQF_REQUIRE(tg->grad_paths && tg->grad_params && tg->workspace,
+                "VJP needs grad_paths, grad_params and a workspace");
+    TQF_REQUIRE(x0_stride_path == 0 || tg->grad_x0,
+                "per-path initial states need grad_x0");
+  } else {
+    TQF_REQUIRE(tg->mode == TQF_TANGENT_JVP, "unknown tangent mode %d",
+                tg->mode);
+    TQF_REQUIRE(tg->direction && tg->tangent_paths,
+                "JVP needs a direction and tangent_paths");
+  }
+  const hipStream_t s = (hipStream_t)stream;
+  if (dtype == TQF_F32)
+    return launch_tangent<float>(m, g, r, tg, n_paths, x0, x0_stride_path, s);
+  return launch_tangent<double>(m, g, r, tg, n_paths, x0, x0_stride_path, s);
 }

@@ -5,7 +5,11 @@ erfinv(x) = x * P(w) with w = -ln(1 - x^2).  Two ranges:
   tail:    w >= 5, P_t(sqrt(w) - 3)       (up to w = 17, i.e. 1 - x^2 >= 4e-8)
 Each P is a degree-8 polynomial fitted by weighted least squares on
 Chebyshev nodes against scipy.special.erfinv (relative error), then checked
-by evaluating the Horner form in float32.  Run: python fit_erfinv.py
+by evaluating the Horner form in float32.  The form of the approximation
+(w, the split at w = 5, the variables w - 2.5 and sqrt(w) - 3) follows
+M. Giles, "Approximating the erfinv function", GPU Computing Gems Jade
+Edition (2011); the coefficients are this script's own fit.
+Run: python fit_erfinv.py
 """
 import numpy as np
 from scipy import special

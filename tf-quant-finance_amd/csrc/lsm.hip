@@ -1230,10 +1230,27 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
     for (int i = 0; i < tiles; ++i)
       for (int j = i; j < tiles; ++j) hp[q++] = int2{i, j};
   }
+  // the exponent table (at most 255 x 8 bytes) comes back in the same
+  // synchronisation: basis_col indexes the power table by exponent, so an
+  // entry above the degree would read another asset's powers
+  uint8_t he_expo[255 * kMaxAssets];
   hipError_t he = hipMemcpyAsync(pairs, hp, n_pairs * sizeof(int2),
                                  hipMemcpyHostToDevice, s);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(he_expo, l->exponents, (size_t)K * l->dim,
+                        hipMemcpyDeviceToHost, s);
   if (he == hipSuccess) he = hipStreamSynchronize(s);
-  if (he != hipSuccess) return set_error((int)he, "copy pairs");
+  if (he != hipSuccess) return set_error((int)he, "copy pairs / exponents");
+  {
+    double grid = 1;
+    for (int j = 0; j < l->dim; ++j) grid *= l->degree + 1;
+    TQF_REQUIRE(K <= grid, "%d basis functions exceed (degree + 1)^dim = %.0f",
+                K, grid);
+    for (int e = 0; e < K * l->dim; ++e)
+      TQF_REQUIRE(he_expo[e] <= l->degree,
+                  "exponent %d of basis function %d exceeds the degree %d",
+                  (int)he_expo[e], e / l->dim, l->degree);
+  }
 
   tqf_lsm l2 = *l;
   MomentPlan mp{};

@@ -5,6 +5,9 @@ Mirrors of the reference call surface:
   tqf.math.random.mv_normal_sample        <- tff.math.random.mv_normal_sample
   tqf.math.qmc.sobol_sample (+ nets)      <- tff.math.qmc
   tqf.models.euler_sampling.sample        <- tff.models.euler_sampling.sample
+  tqf.models.GenericItoProcess            <- tff.models.GenericItoProcess
+  tqf.math.fwd_gradient / gradients / value_and_gradient
+                                          <- tff.math (gradient.py)
   tqf.models.GeometricBrownianMotion      <- tff.models.GeometricBrownianMotion
   tqf.models.MultivariateGeometricBrownianMotion
   tqf.models.HestonModel                  <- tff.models.HestonModel

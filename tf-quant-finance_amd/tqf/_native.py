@@ -57,6 +57,16 @@ class Output(ctypes.Structure):
               ('stderrs', _c_void_p), ('sums', _c_void_p)]
 
 
+TANGENT_VJP, TANGENT_JVP = 0, 1
+
+
+class Tangent(ctypes.Structure):
+  _fields_ = [('mode', _i32), ('pad0', _i32), ('grad_paths', _c_void_p),
+              ('grad_params', _c_void_p), ('grad_x0', _c_void_p),
+              ('direction', _c_void_p), ('dir_x0', _c_void_p),
+              ('tangent_paths', _c_void_p), ('workspace', _c_void_p)]
+
+
 class Gbm(ctypes.Structure):
   _fields_ = [('dim', _i32), ('drift', _c_void_p), ('vol', _c_void_p),
               ('chol', _c_void_p), ('s0', _c_void_p),
@@ -108,6 +118,10 @@ _SIGS = {
     'tqf_payoff_finalize': (_i32, [_c_void_p, _i64, _i32, _i32, _c_void_p,
                                    _c_void_p, _c_void_p, _c_void_p,
                                    _c_void_p]),
+    'tqf_euler_tangent_workspace_size': (_i64, [_i64]),
+    'tqf_euler_tangent': (_i32, [ctypes.POINTER(Model), ctypes.POINTER(Grid),
+                                 ctypes.POINTER(Rng), ctypes.POINTER(Tangent),
+                                 _i64, _c_void_p, _i64, _i32, _c_void_p]),
     'tqf_gbm_exact': (_i32, [ctypes.POINTER(Gbm), ctypes.POINTER(Grid),
                              ctypes.POINTER(Rng), _i32, _c_void_p, _i64, _i32,
                              _c_void_p]),

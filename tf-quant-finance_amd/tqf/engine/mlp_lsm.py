@@ -10,6 +10,10 @@ of every path.  The price is the discounted mean over the independent
 pricing paths, a lower bound for any exercise policy.  The polynomial LSM
 price on the same paths is the judge: the MLP price is accepted only when
 it is at least poly - 2 SE; otherwise the polynomial price is reported.
+Both prices are means over the same pricing paths, so their errors are
+strongly correlated: the gain and its standard error come from the per-path
+differences of the two policies' discounted cashflows (a paired estimate),
+not from the two prices' own standard errors.
 """
 import ctypes
 
@@ -113,13 +117,16 @@ def _lsm_struct(paths, payoff, strike, degree=2):
 def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
                   num_calibration, payoff=lsm_lib.PAYOFF_MAX_CALL,
                   steps_per_date=200, batch=65536, lr=1e-3, seed=0,
-                  poly_degree=2, judge=True):
+                  poly_degree=2, judge=True, return_path_values=False):
   """Prices with the learned continuation value (and the polynomial judge).
 
   paths: float64 device tensor [N, T, d], rows [0, num_calibration) train
   the models, the rest price.  discount_factors: [n_ex] (unpadded).
-  Returns a dict with mlp / poly prices and standard errors, the accepted
-  price and the per-date final training losses.
+  Returns a dict with mlp / poly prices and standard errors, the paired
+  gain mlp - poly and its standard error, the accepted price and the
+  per-date final training losses; with return_path_values the per-path
+  values of both policies at the first exercise date (float64 [N], before
+  the final discount).
   """
   dev = paths.device
   paths = paths.contiguous()
@@ -182,18 +189,32 @@ def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
   out = {'mlp_price': mlp_price, 'mlp_se': mlp_se,
          'losses': [float(t.item()) for t in losses]}
   if judge:
+    mlp_values = cash + vals
+    poly_values = torch.empty(n, **f64)
     poly, poly_se, _ = lsm_lib._native_lsm(
         paths, ex, (payoff, np.array([float(strike)])), poly_degree,
         np.asarray(df).reshape(-1, 1, 1), n_cal, torch.float64, dev,
-        return_details=True)
+        return_details=True, path_values=poly_values)
     poly_price = float(poly[0])
     poly_se = float(poly_se[0])
-    comb = np.hypot(mlp_se, poly_se)
-    accepted = mlp_price >= poly_price - 2 * comb
+    # paired: mean and standard error of df0 (mlp_p - poly_p) over the
+    # pricing rows, by the same kernel that reduces each price
+    diff = mlp_values - poly_values
+    _native.call('tqf_lsm_final', ctypes.c_void_p(diff.data_ptr()),
+                 ctypes.c_void_p(torch.zeros_like(diff).data_ptr()),
+                 ctypes.c_double(df[1] / df[0]), ctypes.c_int64(n_cal),
+                 ctypes.c_int64(n), ctypes.c_void_p(fin.data_ptr()),
+                 ctypes.c_void_p(res.data_ptr()), s)
+    gain, gain_se = [float(v) for v in res.cpu().numpy()]
+    accepted = gain >= -2 * gain_se
     out.update({'poly_price': poly_price, 'poly_se': poly_se,
                 'accepted': bool(accepted),
                 'price': mlp_price if accepted else poly_price,
-                'gain': mlp_price - poly_price, 'gain_se': float(comb)})
+                'gain': gain, 'gain_se': gain_se,
+                'gain_se_unpaired': float(np.hypot(mlp_se, poly_se))})
+    if return_path_values:
+      out['mlp_path_values'] = mlp_values
+      out['poly_path_values'] = poly_values
   return out
 
 

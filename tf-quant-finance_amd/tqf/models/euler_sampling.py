@@ -3,15 +3,20 @@
 Same signature and semantics as tff/models/euler_sampling.py:27-332.
 Coefficient callables produced by the tqf model classes (GBM, Heston,
 local volatility) run in one fused HIP kernel (tqf_euler) with the state
-in registers.  Any other callable pair runs as a per-step loop of torch
-ops on the same GPU, fed by the engine's HIP normal generators; that loop is
-also the differentiable route for `watch_params`.
+in registers.  When the initial state or a parameter of a GBM, constant-
+coefficient or Heston model needs gradients (`watch_params`, or tensors
+that require grad), the fused kernel samples and the forward-tangent
+kernel (tqf_euler_tangent) differentiates, with memory flat in the number
+of steps (tqf.engine.differentiable).  Any other callable pair runs as a
+per-step loop of torch ops on the same GPU, fed by the engine's HIP normal
+generators; autograd through that loop serves `watch_params` there.
 """
 import numpy as np
 import torch
 
 from tqf import _native
 from tqf import errors
+from tqf.engine import differentiable
 from tqf.engine import euler as engine
 from tqf.math.random.multivariate_normal import RandomType
 from tqf.models import utils
@@ -73,59 +78,111 @@ def sample(dim, drift_fn, volatility_fn, times, time_step=None,
 
   dev = _native.device()
   draws = None
+  batch = ()
   if normal_draws is not None:
-    draws = torch.as_tensor(normal_draws, device=dev).to(dtype)
-    if draws.dim() != 3:
-      raise NotImplementedError('batched normal_draws are not supported')
+    draws = utils.device_tensor(normal_draws, dtype, dev)
+    if draws.dim() < 3:
+      raise ValueError('`normal_draws` must have shape batch_shape + '
+                       '[num_samples, num_time_steps, dim]')
     if draws.shape[-1] != dim:
       raise ValueError(
           '`dim` should be equal to `normal_draws.shape[2]` but are '
           '{0} and {1} respectively'.format(dim, draws.shape[-1]))
-    if validate_args and draws.shape[1] != steps:
+    if validate_args and draws.shape[-2] != steps:
       raise errors.InvalidArgumentError(
           '`num_time_steps` should be equal to `tf.shape(normal_draws)[1]`')
-    num_samples = draws.shape[0]
+    batch = tuple(draws.shape[:-3])
+    num_samples = draws.shape[-3]
   num_samples = int(num_samples)
 
   if initial_state is None:
     initial_state = torch.zeros(dim, dtype=dtype, device=dev)
-  x0 = torch.as_tensor(initial_state, device=dev).to(dtype)
+  x0 = utils.device_tensor(initial_state, dtype, dev)
+  # batch_shape = shape(initial_state)[:-2] (euler_sampling.py:251), or the
+  # leading axes of batched normal_draws
   if x0.dim() > 2:
-    raise NotImplementedError('batch_shape in initial_state is not supported')
+    batch = tuple(torch.broadcast_shapes(tuple(x0.shape[:-2]), batch))
 
   spec = engine.spec_of(drift_fn, volatility_fn)
-  if (spec is not None and draws is None and random_type in
-      (RandomType.HALTON, RandomType.HALTON_RANDOMIZED)):
+  fused = spec is not None and engine.model_dim(spec) == dim
+  if fused and batch:
+    # a batch of initial states (or of draws) for one model: the batch
+    # members are consecutive blocks of paths of a single launch
+    nb = int(np.prod(batch))
+    xb = (x0 + torch.zeros(batch + (num_samples, dim), dtype=dtype,
+                           device=dev)).reshape(nb * num_samples, dim)
+    if draws is not None:
+      draws = draws.expand(batch + tuple(draws.shape[-3:])).reshape(
+          (nb * num_samples,) + tuple(draws.shape[-2:]))
+    out = _fused(dim, spec, all_times, keep_mask, k, xb, nb * num_samples,
+                 random_type, seed, skip, draws, watch_params, dtype, dev,
+                 steps)
+    if out is not None:
+      return out.reshape(batch + (num_samples, k, dim))
+  elif fused:
+    out = _fused(dim, spec, all_times, keep_mask, k, x0, num_samples,
+                 random_type, seed, skip, draws, watch_params, dtype, dev,
+                 steps)
+    if out is not None:
+      return out
+  return _generic_loop(dim, drift_fn, volatility_fn, all_times, keep_mask, k,
+                       x0, num_samples, random_type, seed, skip, draws, dtype,
+                       dev, batch)
+
+
+def _fused(dim, spec, all_times, keep_mask, k, x0, num_samples, random_type,
+           seed, skip, draws, watch_params, dtype, dev, steps):
+  """The fused kernels for a model spec; None when the call needs the
+  per-step loop (watch_params on a model without tangents)."""
+  if draws is None and random_type in (RandomType.HALTON,
+                                       RandomType.HALTON_RANDOMIZED):
     # Halton draws are produced up front ([N, steps, d], the reference's
     # quasi layout) and read by the fused kernel as injected draws
     draws = utils.generate_mc_normal_draws(
         num_normal_draws=dim, num_time_steps=steps,
         num_sample_paths=num_samples, random_type=random_type, seed=seed,
         skip=skip, dtype=dtype).permute(1, 0, 2)
-  if spec is not None and watch_params is None and engine.model_dim(spec) == dim:
+  grads = (differentiable.needs_gradients(spec, x0) or
+           (watch_params is not None and
+            spec[0] in differentiable.TANGENT_KINDS))
+  if grads:
+    # fused forward, forward-tangent backward (memory flat in steps);
+    # Sobol draws are generated up front so both passes read the same
+    if draws is None and random_type not in (
+        None, RandomType.PSEUDO, RandomType.STATELESS,
+        RandomType.PSEUDO_ANTITHETIC, RandomType.STATELESS_ANTITHETIC):
+      draws = utils.generate_mc_normal_draws(
+          num_normal_draws=dim, num_time_steps=steps,
+          num_sample_paths=num_samples, random_type=random_type, seed=seed,
+          skip=skip, dtype=dtype).permute(1, 0, 2)
     rng = engine.RngSpec.from_random_type(random_type, seed, skip, draws)
-    return engine.run(spec, all_times, keep_mask, k, x0, num_samples, rng,
-                      dtype, mode='paths', dev=dev)
-  return _generic_loop(dim, drift_fn, volatility_fn, all_times, keep_mask, k,
-                       x0, num_samples, random_type, seed, skip, draws, dtype,
-                       dev)
+    return differentiable.sample_paths(spec, all_times, keep_mask, k, x0,
+                                       num_samples, rng, dtype, dev)
+  if watch_params is not None:
+    return None
+  rng = engine.RngSpec.from_random_type(random_type, seed, skip, draws)
+  return engine.run(spec, all_times, keep_mask, k, x0, num_samples, rng,
+                    dtype, mode='paths', dev=dev)
 
 
 def _generic_loop(dim, drift_fn, volatility_fn, all_times, keep_mask, k, x0,
-                  num_samples, random_type, seed, skip, draws, dtype, dev):
-  """`_while_loop` of euler_sampling.py:405-464 as torch ops on the GPU."""
+                  num_samples, random_type, seed, skip, draws, dtype, dev,
+                  batch=()):
+  """`_while_loop` of euler_sampling.py:405-464 as torch ops on the GPU;
+  the state keeps its batch axes, batch + [N, d]."""
   steps = all_times.shape[0] - 1
   if draws is None:
     draws = utils.generate_mc_normal_draws(
         num_normal_draws=dim, num_time_steps=steps,
         num_sample_paths=num_samples, random_type=random_type, seed=seed,
-        skip=skip, dtype=dtype)  # [steps, N, d]
+        skip=skip, dtype=dtype, batch_shape=batch)  # [steps] + batch + [N, d]
   else:
-    draws = draws.permute(1, 0, 2)
+    r = draws.dim()
+    draws = draws.permute((r - 2,) + tuple(range(r - 2)) + (r - 1,))
   times_t = torch.as_tensor(all_times, dtype=dtype, device=dev)
   dt = times_t[1:] - times_t[:-1]
   sqrt_dt = torch.sqrt(dt)
-  x = x0 + torch.zeros((num_samples, dim), dtype=dtype, device=dev)
+  x = x0 + torch.zeros(batch + (num_samples, dim), dtype=dtype, device=dev)
   slots, init_slot, executed, _ = utils.record_plan(keep_mask, k)
   outs = [None] * k
   if init_slot >= 0:
@@ -138,5 +195,5 @@ def _generic_loop(dim, drift_fn, volatility_fn, all_times, keep_mask, k, x0,
     x = x + inc + torch.matmul(vol, dw.unsqueeze(-1)).squeeze(-1)
     if slots[i] >= 0:
       outs[slots[i]] = x
-  zero = torch.zeros((num_samples, dim), dtype=dtype, device=dev)
-  return torch.stack([o if o is not None else zero for o in outs], dim=1)
+  zero = torch.zeros_like(x)
+  return torch.stack([o if o is not None else zero for o in outs], dim=-2)

@@ -65,7 +65,7 @@ def run_exact(times, drift, vol, chol, s0, num_samples, rng, dtype,
   g.drift = dv(np.asarray(drift, np.float64)).data_ptr()
   g.vol = dv(np.asarray(vol, np.float64)).data_ptr()
   g.chol = 0 if chol is None else dv(np.asarray(chol, np.float64)).data_ptr()
-  s0_t = torch.as_tensor(s0, device=dev).to(dtype)
+  s0_t = utils.device_tensor(s0, dtype, dev)
   if s0_t.dim() == 0:
     s0_t = s0_t.reshape(1)
   if s0_t.dim() == 1:
@@ -138,8 +138,10 @@ class GeometricBrownianMotion:
     if np.ndim(utils.to_numpy(mean)) or np.ndim(utils.to_numpy(volatility)):
       raise NotImplementedError(
           'batched or piecewise-constant GBM parameters are not supported')
-    self._mean = float(utils.to_numpy(mean))
-    self._volatility = float(utils.to_numpy(volatility))
+    # tensors that require grad are kept: euler_sampling.sample then
+    # differentiates with respect to them
+    self._mean = utils.scalar_param(mean)
+    self._volatility = utils.scalar_param(volatility)
 
   def dim(self):
     return 1
@@ -175,20 +177,65 @@ class GeometricBrownianMotion:
     times_np = utils.to_numpy(times, utils.np_dtype(self._dtype)).reshape(-1)
     if initial_state is None:
       initial_state = 1.0
-    s0 = torch.as_tensor(initial_state, device=dev).to(self._dtype)
+    s0 = utils.device_tensor(initial_state, self._dtype, dev)
     if s0.dim() > 1 or (s0.dim() == 1 and s0.shape[0] not in (1,)):
       s0 = s0.reshape(-1, 1)
     draws = None
     if normal_draws is not None:
-      draws = torch.as_tensor(normal_draws, device=dev).to(self._dtype)
+      draws = utils.device_tensor(normal_draws, self._dtype, dev)
       if draws.dim() != 3 or draws.shape[2] != 1:
         raise ValueError('`dim` should be equal to `1` but is {0}'.format(
             draws.shape[-1]))
       num_samples = draws.shape[0]
     rng = engine.RngSpec.from_random_type(random_type, seed, skip, draws)
-    mu, sig = self._mean, self._volatility
-    return run_exact(times_np, [mu - sig * sig / 2], [sig], None, s0,
-                     int(num_samples), rng, self._dtype, dev=dev)
+    mu, sig = float(self._mean), float(self._volatility)
+
+    def run():
+      return run_exact(times_np, [mu - sig * sig / 2], [sig], None,
+                       s0.detach(), int(num_samples), rng, self._dtype,
+                       dev=dev)
+    params = [self._mean, self._volatility]
+    if torch.is_grad_enabled() and (s0.requires_grad or any(
+        torch.is_tensor(p) and p.requires_grad for p in params)):
+      t = torch.as_tensor(times_np, dtype=self._dtype, device=dev)
+      mu_t, sig_t = (p if torch.is_tensor(p) else
+                     torch.tensor(p, dtype=self._dtype, device=dev)
+                     for p in params)
+      return _ExactPaths.apply(run, t, s0, mu_t, sig_t)
+    return run()
+
+
+class _ExactPaths(torch.autograd.Function):
+  """Exact GBM paths S = S0 exp((mu - sigma^2/2) t + sigma W_t) from
+  tqf_gbm_exact, differentiable in S0, mu and sigma.  The backward needs
+  only the paths: dS/dS0 = S / S0, dS/dmu = S t, dS/dsigma = S (W_t -
+  sigma t) with W_t = (ln(S / S0) - (mu - sigma^2/2) t) / sigma; at
+  sigma = 0 the sigma-derivative is 0, the reference's convention for
+  sqrt(sigma^2 dt) (univariate_geometric_brownian_motion.py:712-718)."""
+
+  @staticmethod
+  def forward(ctx, run, t, s0, mu, sig):
+    out = run()
+    ctx.save_for_backward(out, t, s0, mu, sig)
+    return out
+
+  @staticmethod
+  def backward(ctx, g):
+    out, t, s0, mu, sig = ctx.saved_tensors
+    n = out.shape[0]
+    s0b = s0.reshape(-1, 1, 1) if s0.numel() > 1 else s0.reshape(1, 1, 1)
+    gs = g * out
+    tt = t.reshape(1, -1, 1)
+    g_s0 = (gs / s0b).sum(dim=(1, 2))
+    g_s0 = (g_s0.reshape(s0.shape) if s0.numel() == n and s0.numel() > 1
+            else g_s0.sum().reshape(s0.shape))
+    g_mu = (gs * tt).sum().reshape(mu.shape)
+    if float(sig) != 0.0:
+      w = (torch.log(out / s0b) - (mu - 0.5 * sig * sig) * tt) / sig
+      g_sig = (gs * (w - sig * tt)).sum().reshape(sig.shape)
+    else:
+      g_sig = torch.zeros_like(sig)
+    return None, None, g_s0, g_mu, g_sig
 
 
 class MultivariateGeometricBrownianMotion:
@@ -253,10 +300,10 @@ class MultivariateGeometricBrownianMotion:
     times_np = utils.to_numpy(times, utils.np_dtype(self._dtype)).reshape(-1)
     if initial_state is None:
       initial_state = np.ones(self._dim)
-    s0 = torch.as_tensor(initial_state, device=dev).to(self._dtype)
+    s0 = utils.device_tensor(initial_state, self._dtype, dev)
     draws = None
     if normal_draws is not None:
-      draws = torch.as_tensor(normal_draws, device=dev).to(self._dtype)
+      draws = utils.device_tensor(normal_draws, self._dtype, dev)
       if draws.dim() != 3 or draws.shape[2] != self._dim:
         raise ValueError(
             '`dim` should be equal to `normal_draws.shape[2]` but are '

@@ -74,7 +74,7 @@ class HestonModel:
     def conv(p):
       if isinstance(p, PiecewiseConstantFunc):
         return p
-      return float(utils.to_numpy(p))
+      return utils.scalar_param(p)
     self._mean_reversion = conv(mean_reversion)
     self._theta = conv(theta)
     self._volvol = conv(volvol)
@@ -96,8 +96,8 @@ class HestonModel:
   def _spec(self, r, variant):
     if not self._constant:
       return None
-    return ('heston', float(r), self._mean_reversion, self._theta,
-            self._volvol, self._rho, float(variant))
+    return ('heston', r, self._mean_reversion, self._theta, self._volvol,
+            self._rho, float(variant))
 
   def drift_fn(self):
     params = self._params
@@ -129,8 +129,9 @@ class HestonModel:
     """(drift_fn, volatility_fn) of the full-truncation log-Euler scheme."""
     if not self._constant:
       raise NotImplementedError('full truncation needs constant parameters')
-    r = float(risk_free_rate)
+    r = utils.scalar_param(risk_free_rate)
     k, th, xi, rho = self._params
+    s1r = (1 - rho * rho)**0.5
 
     def _drift(t, x):
       vp = torch.clamp(x[..., 1], min=0)
@@ -140,7 +141,7 @@ class HestonModel:
       vp = torch.sqrt(torch.clamp(x[..., 1], min=0))
       z = torch.zeros_like(vp)
       c1 = torch.stack([vp, xi * rho * vp], -1)
-      c2 = torch.stack([z, xi * np.sqrt(1 - rho * rho) * vp], -1)
+      c2 = torch.stack([z, xi * s1r * vp], -1)
       return torch.stack([c1, c2], -1)
     spec = self._spec(r, 0)
     return engine.tag(_drift, spec), engine.tag(_vol, spec)
@@ -160,7 +161,7 @@ class HestonModel:
     times_np = utils.to_numpy(times, npd).reshape(-1)
     draws = None
     if normal_draws is not None:
-      draws = torch.as_tensor(normal_draws, device=dev).to(dtype)
+      draws = utils.device_tensor(normal_draws, dtype, dev)
       num_samples = draws.shape[0]
     if times_grid is None:
       if time_step is None:

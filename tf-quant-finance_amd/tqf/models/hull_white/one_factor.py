@@ -233,7 +233,7 @@ class HullWhiteModel1F:
     steps = all_times.shape[0] - 1
     draws = None
     if normal_draws is not None:
-      draws = torch.as_tensor(normal_draws, device=dev).to(self._dtype)
+      draws = utils.device_tensor(normal_draws, self._dtype, dev)
       if draws.shape[-1] != 1:
         raise ValueError(
             '`dim` should be equal to `normal_draws.shape[2]` but are '
@@ -304,7 +304,7 @@ class HullWhiteModel1F:
     batch (vector_hull_white.py:594-635)."""
     del name
     dev = _native.device()
-    r = torch.as_tensor(short_rate, device=dev).to(self._dtype)
+    r = utils.device_tensor(short_rate, self._dtype, dev)
     t = np.asarray(utils.to_numpy(times), np.float64)
     big_t = np.asarray(utils.to_numpy(maturities), np.float64)
     f0, p0big, g, y = self._bond_constants(t, big_t)

@@ -465,8 +465,7 @@ class LocalVolatilityModel:
     dev = _native.device()
     x0 = None
     if initial_state is not None:
-      x0 = torch.log(torch.as_tensor(initial_state, device=dev).to(
-          self._dtype))
+      x0 = torch.log(utils.device_tensor(initial_state, self._dtype, dev))
     x = euler_sampling.sample(
         self._dim, self.drift_fn(), self.volatility_fn(), times,
         time_step=time_step, num_time_steps=num_time_steps,

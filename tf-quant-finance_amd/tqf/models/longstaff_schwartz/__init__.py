@@ -161,9 +161,15 @@ def least_square_mc(sample_paths, exercise_times, payoff_fn, basis_fn,
 
 
 def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
-                return_details=False, basis_log=False, exponents=None):
+                return_details=False, basis_log=False, exponents=None,
+                path_values=None):
   """exponents: an optional [K, d] table of monomial exponents (each at
-  most `degree`) in place of the tensor grid of make_polynomial_basis."""
+  most `degree`) in place of the tensor grid of make_polynomial_basis.
+  path_values: an optional float64 device tensor [N] that receives, for the
+  last payoff batch entry, every path's value at the first exercise date
+  (cashflow + discounted later values, before the final discount df[1] /
+  df[0]): the per-path terms whose mean over the pricing rows is the
+  price."""
   kind, strikes = pspec
   paths_b = x.unsqueeze(0) if x.dim() == 3 else x  # [Bp, N, T, d]
   bp = paths_b.shape[0]
@@ -241,6 +247,8 @@ def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
     prices.append(res[0])
     ses.append(res[1])
     ranks_all.append(ranks)
+    if path_values is not None:
+      torch.add(cash, vals, out=path_values)
   out = torch.stack(prices).to(dtype)
   if return_details:
     return out, torch.stack(ses), torch.stack(ranks_all)

@@ -30,6 +30,26 @@ def np_dtype(dtype):
   return np.dtype(dtype).type
 
 
+def scalar_param(x):
+  """A scalar model parameter: a tensor that requires grad stays a tensor
+  (the engine can differentiate with respect to it); anything else becomes
+  a float."""
+  if isinstance(x, torch.Tensor) and x.requires_grad:
+    if x.numel() != 1:
+      raise NotImplementedError('batched model parameters are not supported')
+    return x.reshape(())
+  return float(to_numpy(x))
+
+
+def device_tensor(x, dtype, dev):
+  """x on the device in `dtype`.  Python numbers and lists are converted in
+  `dtype` directly (torch would build them in float32 first and round a
+  float64 input); a tensor keeps its autograd history."""
+  if isinstance(x, torch.Tensor):
+    return x.to(device=dev, dtype=dtype)
+  return torch.as_tensor(np.asarray(x, dtype=np_dtype(dtype)), device=dev)
+
+
 def to_numpy(x, dtype=None):
   if isinstance(x, torch.Tensor):
     x = x.detach().cpu().numpy()
@@ -124,10 +144,18 @@ def record_plan(keep_mask, num_requested, initial_written=None):
 def generate_mc_normal_draws(num_normal_draws, num_time_steps,
                              num_sample_paths, random_type, batch_shape=None,
                              skip=0, seed=None, dtype=None, name=None):
-  """Draws [num_time_steps, N, d] laid out as the reference lays them out."""
+  """Draws [num_time_steps] + batch_shape + [N, d], laid out as the
+  reference lays them out (models/utils.py:20-128); batch member b takes the
+  stream's paths [b N, (b + 1) N)."""
   del name
-  if batch_shape:
-    raise NotImplementedError('batch_shape is not supported')
+  batch = tuple(int(b) for b in (batch_shape or ()))
+  if batch:
+    nb = int(np.prod(batch))
+    flat = generate_mc_normal_draws(num_normal_draws, num_time_steps,
+                                    nb * int(num_sample_paths), random_type,
+                                    skip=skip, seed=seed, dtype=dtype)
+    steps, d = flat.shape[0], flat.shape[-1]
+    return flat.reshape((steps,) + batch + (int(num_sample_paths), d))
   dtype = dtype or torch.float32
   if isinstance(dtype, np.dtype) or dtype in (np.float32, np.float64):
     dtype = torch.float64 if np.dtype(dtype) == np.float64 else torch.float32
