@@ -244,6 +244,24 @@ int tqf_euler(const tqf_model* model, const tqf_grid* grid,
               const void* x0, int64_t x0_stride_path, int32_t dtype,
               void* stream);
 
+/* Allocation-free variant of tqf_euler in TQF_OUT_PAYOFF mode.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under stream or graph capture, tight serving loops).
+ * Same arguments and results as tqf_euler with out->mode == TQF_OUT_PAYOFF,
+ * except that out->workspace must be NULL: the payoff partials live in a
+ * fixed device array of TQF_NOALLOC_SCRATCH_BYTES that the library holds
+ * on each GPU it runs on until the process ends.  Calls whose
+ * tqf_euler_workspace_size(n_paths, n_slots, n_strikes) exceeds it return
+ * TQF_EINVAL.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (two streams, two threads) race.  Makes no hipMalloc, hipFree
+ * or other allocating call; the stream must belong to the current device. */
+#define TQF_NOALLOC_SCRATCH_BYTES (32 << 20)
+int tqf_euler_payoff_noalloc(const tqf_model* model, const tqf_grid* grid,
+                             const tqf_rng* rng, const tqf_output* out,
+                             int64_t n_paths, const void* x0,
+                             int64_t x0_stride_path, int32_t dtype,
+                             void* stream);
+
 /* ---- Forward tangents of the fused Euler paths -------------------------- */
 
 #define TQF_TANGENT_VJP 0  /* gradient of sum(g * paths) w.r.t. the inputs */

@@ -22,6 +22,8 @@
 // its 64 paths without any cross-lane reduction tree.  Per-wave sums go to
 // LDS, the block combines its 4 waves in float64 and one finalize kernel
 // sums the blocks in a fixed order (deterministic).
+#include <mutex>
+
 #include "common.h"
 #include "normals.h"
 
@@ -1058,4 +1060,71 @@ extern "C" int tqf_euler_tangent(const tqf_model* m, const tqf_grid* g,
   if (dtype == TQF_F32)
     return launch_tangent<float>(m, g, r, tg, n_paths, x0, x0_stride_path, s);
   return launch_tangent<double>(m, g, r, tg, n_paths, x0, x0_stride_path, s);
+}
+
+// ---- allocation-free payoff pricing ----------------------------------------
+// For callers that cannot allocate device memory (code under stream or graph
+// capture, tight serving loops): tqf_euler in TQF_OUT_PAYOFF mode with its
+// workspace (per-block payoff partials and chunk sums) in a fixed device
+// array owned by the library.  Each GPU the library runs on holds its own
+// copy for the life of the process; every call on one GPU shares it, so
+// calls that overlap on one GPU (two streams, two threads) race.  Nothing
+// has to start at zero: every block writes all of its partial sums before
+// the reductions read them.
+static __device__ __align__(256) double scratch[TQF_NOALLOC_SCRATCH_BYTES / 8];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once
+// per device (the array's name is only a host-side stand-in).
+int scratch_on_current_device(void** out) {
+  int dev = -1;
+  hipError_t he = hipGetDevice(&dev);
+  if (he != hipSuccess) return set_error((int)he, "hipGetDevice");
+  TQF_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index %d out of range",
+              dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (he != hipSuccess || p == nullptr)
+      return set_error(he != hipSuccess ? (int)he : (int)hipErrorUnknown,
+                       "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TQF_OK;
+}
+}  // namespace
+
+extern "C" int tqf_euler_payoff_noalloc(const tqf_model* m, const tqf_grid* g,
+                                        const tqf_rng* r,
+                                        const tqf_output* out, int64_t n_paths,
+                                        const void* x0, int64_t x0_stride_path,
+                                        int32_t dtype, void* stream) {
+  clear_error();
+  TQF_REQUIRE(m && g && r && out, "null descriptor");
+  TQF_REQUIRE(out->mode == TQF_OUT_PAYOFF,
+              "the allocation-free entry point prices payoffs "
+              "(TQF_OUT_PAYOFF)");
+  TQF_REQUIRE(out->workspace == nullptr,
+              "workspace must be NULL: the library's scratch array is used");
+  TQF_REQUIRE(n_paths >= 1 && g->n_slots >= 1 && out->n_strikes >= 1,
+              "bad sizes");
+  const int64_t need =
+      tqf_euler_workspace_size(n_paths, g->n_slots, out->n_strikes);
+  TQF_REQUIRE(need <= (int64_t)TQF_NOALLOC_SCRATCH_BYTES,
+              "%lld paths x %d dates x %d strikes need %lld bytes of scratch; "
+              "the allocation-free entry point holds %lld",
+              (long long)n_paths, g->n_slots, out->n_strikes, (long long)need,
+              (long long)TQF_NOALLOC_SCRATCH_BYTES);
+  void* ws = nullptr;
+  int rc = scratch_on_current_device(&ws);
+  if (rc) return rc;
+  tqf_output o = *out;
+  o.workspace = (double*)ws;
+  return tqf_euler(m, g, r, &o, n_paths, x0, x0_stride_path, dtype, stream);
 }

@@ -118,6 +118,11 @@ _SIGS = {
     'tqf_payoff_finalize': (_i32, [_c_void_p, _i64, _i32, _i32, _c_void_p,
                                    _c_void_p, _c_void_p, _c_void_p,
                                    _c_void_p]),
+    'tqf_euler_payoff_noalloc': (_i32, [ctypes.POINTER(Model),
+                                        ctypes.POINTER(Grid),
+                                        ctypes.POINTER(Rng),
+                                        ctypes.POINTER(Output), _i64,
+                                        _c_void_p, _i64, _i32, _c_void_p]),
     'tqf_euler_tangent_workspace_size': (_i64, [_i64]),
     'tqf_euler_tangent': (_i32, [ctypes.POINTER(Model), ctypes.POINTER(Grid),
                                  ctypes.POINTER(Rng), ctypes.POINTER(Tangent),
