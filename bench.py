@@ -489,9 +489,22 @@ def _batch(out, timed, batch_lib, pool):
   b = plan2.out.cpu().numpy()
   means = a[:, :8]
   scale = np.maximum(np.abs(means), 1e-3 * np.abs(a[:, :1]))
+  fin = np.isfinite(means) & np.isfinite(b[:, :8])
+  # Feller-violating Heston contracts: the variance reaches 0 on some
+  # paths, where the pathwise variance-parameter derivatives carry
+  # 1/sqrt(v) factors that can overflow float32
+  feller = np.array([c['kind'] == batch_lib.KIND_GBM or
+                     2 * c['kappa'] * c['theta'] >= c['xi']**2 for c in cs])
   acc['greeks_%s_vs_%s' % (other, cfg['greeks'])] = {
       'contracts': len(cs),
-      'max_rel_diff': float(np.max(np.abs(b[:, :8] - means) / scale)),
+      'max_rel_diff': float(np.max(np.where(fin, np.abs(b[:, :8] - means) /
+                                            scale, 0.0))),
+      'max_rel_diff_feller_contracts': float(np.max(np.where(
+          fin[feller], np.abs(b[feller, :8] - means[feller]) / scale[feller],
+          0.0))),
+      'contracts_with_nonfinite_greeks': {
+          cfg['greeks']: int(np.sum(~np.all(np.isfinite(means), 1))),
+          other: int(np.sum(~np.all(np.isfinite(b[:, :8]), 1)))},
       'prices_identical': bool(np.array_equal(a[:, 0], b[:, 0]))}
   out['batch_4096'] = {'value': 4096 / t, 'unit': 'options+Greeks/s',
                        ('%s_seconds' % other): t2,

@@ -334,7 +334,8 @@ int tqf_gbm_exact(const tqf_gbm* gbm, const tqf_grid* grid, const tqf_rng* rng,
                                     payoff_table[t_idx][p]                    */
 
 typedef struct tqf_lsm {
-  const double* paths;      /* device [n_paths][n_times][dim] float64        */
+  const double* paths;      /* device float64, [n_paths][n_times][dim] or the
+                               layout the strides below give               */
   int64_t n_paths;
   int32_t n_times;
   int32_t dim;              /* assets, 1..8                                  */
@@ -360,6 +361,12 @@ typedef struct tqf_lsm {
                                    padded discount factors of every path
                                    (row 0 = 1); when set they replace df in
                                    tqf_lsm_price                             */
+  int64_t stride_path;  /* element strides of `paths`: value (p, t, j) at   */
+  int64_t stride_time;  /* paths[p stride_path + t stride_time + j          */
+  int64_t stride_asset; /* stride_asset]; all 0 = the reference layout
+                           [n_paths][n_times][dim].  The step-major layout
+                           [n_times][dim][n_paths] (stride_path = 1) makes
+                           every per-date read coalesced.                   */
 } tqf_lsm;
 
 int64_t tqf_lsm_workspace_size(int64_t n_paths, int32_t n_basis);

@@ -5,8 +5,10 @@ There is no counterpart in the reference (its continuation value is the
 polynomial regression of tff/models/longstaff_schwartz/lsm.py:328-383).
 The restatement is pinned by central finite differences of its own loss
 (tests/test_oracle.py), which a wrong transpose, sign or chain-rule factor
-fails.  `bf16=True` rounds the operands of the 64 x 64 products to bfloat16
-as the MFMA kernels do, so GPU gradients can be compared tightly.
+fails.  `bf=True` rounds the operands of every matrix product the MFMA
+kernels run in bfloat16 -- layer 1 (inputs and W1), layer 2 (H1 and W2),
+dW1 (inputs and dZ1), dW2 and dH1 (dZ2) -- so GPU gradients can be compared
+tightly; biases, activations and the 64 -> 1 layer stay float.
 
 Parameter vector layout: W1[6][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3.
 """
@@ -59,7 +61,8 @@ def init_params(seed=0):
 
 def forward(p, x, bf=False):
   w1, b1, w2, b2, w3, b3 = unpack(p)
-  z1 = x[:, :N_IN] @ w1 + b1
+  xin = x[:, :N_IN]
+  z1 = (bf16(xin) @ bf16(w1) if bf else xin @ w1) + b1
   h1 = silu(z1)
   w2e = bf16(w2) if bf else w2
   h1e = bf16(h1) if bf else h1
@@ -84,7 +87,8 @@ def loss_and_grad(p, x, t, w, inv_norm, bf=False):
   db2 = dz2.sum(0)
   dh1 = dz2e @ w2e.T
   dz1 = dh1 * silu_grad(z1)
-  dw1 = x[:, :N_IN].T @ dz1
+  xin = x[:, :N_IN]
+  dw1 = bf16(xin).T @ bf16(dz1) if bf else xin.T @ dz1
   db1 = dz1.sum(0)
   return loss, pack(dw1, db1, dw2, db2, dw3, db3)
 

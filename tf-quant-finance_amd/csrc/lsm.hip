@@ -35,7 +35,9 @@ constexpr int kSubPaths = 32;   // paths staged per LDS round
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 struct LsmArgs {
-  const double* paths;      // [n_paths][n_times][d] (path-major) float64
+  const double* paths;      // float64, element (p, t, j) at
+                            // paths[p sp + t st + j sa]
+  int64_t sp, st, sa;
   int64_t n_paths;
   int n_times, d;
   int t_idx;                // time index of the exercise date
@@ -59,6 +61,12 @@ struct LsmArgs {
 __device__ __forceinline__ double disc_ratio(const LsmArgs& a, double ratio,
                                              int64_t p) {
   return a.dnum ? a.dnum[p] / a.dden[p] : ratio;
+}
+
+// Asset j of path p at the date's time index.  With the step-major layout
+// (sp = 1) consecutive threads read consecutive doubles.
+__device__ __forceinline__ double path_x(const LsmArgs& a, int64_t p, int j) {
+  return a.paths[p * a.sp + (int64_t)a.t_idx * a.st + (int64_t)j * a.sa];
 }
 
 // The variable the basis is built on: x, or ln x when the caller regresses
@@ -100,8 +108,7 @@ lsm_prepare_kernel(LsmArgs a, double df_ratio, const double* __restrict__ cash,
   double x[kMaxAssets];
   for (int j = 0; j < kMaxAssets; ++j) x[j] = 0;
   if (p < a.n_paths) {
-    const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
-    for (int j = 0; j < a.d; ++j) x[j] = row[j];
+    for (int j = 0; j < a.d; ++j) x[j] = path_x(a, p, j);
     exv[p] = payoff_value(a, x, p);
     y[p] = disc_ratio(a, df_ratio, p) * (cash[p] + vals[p]);
   }
@@ -214,9 +221,9 @@ lsm_gram_kernel(LsmArgs a, const double* __restrict__ exv,
       const int64_t p = p0 + s;
       double w = 0;
       if (p < p_end) {
-        const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
         for (int j = 0; j < a.d; ++j) {
-          const double c = (basis_input(a, row[j]) - a.mean[j]) * a.scale[j];
+          const double c =
+              (basis_input(a, path_x(a, p, j)) - a.mean[j]) * a.scale[j];
           double v = 1.0;
           for (int e = 0; e <= a.deg; ++e) {
             pw[s][j * deg1 + e] = v;
@@ -429,7 +436,7 @@ lsm_moment_kernel(LsmArgs a, MomentPlan mp, const double* __restrict__ exv,
     in_e = 0.0;
     if (pthread && i < nsub && p < p_end) {
       in_e = exv[p];
-      in_x = a.paths[(p * a.n_times + a.t_idx) * a.d + pj];
+      in_x = path_x(a, p, pj);
       if (pj == 0) in_y = y[p];
     }
   };
@@ -547,7 +554,7 @@ lsm_moment_small_kernel(LsmArgs a, MomentPlan mp,
   if (p_end > n_calib) p_end = n_calib;
   for (int64_t p = p_begin + threadIdx.x; p < p_end; p += 256) {
     if (!(exv[p] > 0)) continue;  // in-the-money rows only
-    const double x = a.paths[p * a.n_times + a.t_idx];
+    const double x = path_x(a, p, 0);
     const double c = (basis_input(a, x) - a.mean[0]) * a.scale[0];
     double v = 1.0, u = y[p];
 #pragma unroll
@@ -864,9 +871,9 @@ lsm_update_kernel(LsmArgs a, double df_ratio, const double* __restrict__ exv,
   const int64_t p = (int64_t)blockIdx.x * nt + tid;
   if (p >= a.n_paths) return;
   const int deg1 = a.deg + 1;
-  const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
   for (int j = 0; j < a.d; ++j) {
-    const double c = (basis_input(a, row[j]) - a.mean[j]) * a.scale[j];
+    const double c =
+        (basis_input(a, path_x(a, p, j)) - a.mean[j]) * a.scale[j];
     double v = 1.0;
     for (int e = 0; e <= a.deg; ++e) {
       pws[(j * deg1 + e) * nt + tid] = v;
@@ -960,12 +967,11 @@ lsm_update_dense_kernel(LsmArgs a, double df_ratio,
                         double* __restrict__ cash, double* __restrict__ vals) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= a.n_paths) return;
-  const double* row = a.paths + (p * a.n_times + a.t_idx) * D;
   double c[D];
   int st[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    c[j] = (basis_input(a, row[j]) - a.mean[j]) * a.scale[j];
+    c[j] = (basis_input(a, path_x(a, p, j)) - a.mean[j]) * a.scale[j];
     st[j] = j == 0 ? 1 : st[j - 1] * (a.deg + 1);
   }
   double cont = DenseHorner<D - 1>::eval(dense, c, st, a.deg);
@@ -1004,8 +1010,7 @@ lsm_terminal_kernel(LsmArgs a, double* __restrict__ cash,
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= a.n_paths) return;
   double x[kMaxAssets];
-  const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
-  for (int j = 0; j < a.d; ++j) x[j] = row[j];
+  for (int j = 0; j < a.d; ++j) x[j] = path_x(a, p, j);
   cash[p] = payoff_value(a, x, p);
   vals[p] = 0;
 }
@@ -1075,13 +1080,13 @@ lsm_features_kernel(LsmArgs a, const double* __restrict__ exv,
                     float* __restrict__ tgt, float* __restrict__ w) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= a.n_paths) return;
-  const double* row = a.paths + (p * a.n_times + a.t_idx) * a.d;
   // the row of 8 floats in registers, stored as two 16-byte halves (a
   // wave's stores then cover whole lines)
   float v[kMlpStride];
 #pragma unroll
   for (int j = 0; j < kMlpIn - 1; ++j)
-    v[j] = j < a.d ? (float)((basis_input(a, row[j]) - a.mean[j]) * a.scale[j])
+    v[j] = j < a.d ? (float)((basis_input(a, path_x(a, p, j)) - a.mean[j]) *
+                             a.scale[j])
                    : 0.f;
   const double e = exv[p];
   v[kMlpIn - 1] = (float)(e / a.strike);
@@ -1122,6 +1127,11 @@ LsmArgs make_args(const tqf_lsm* l, int t_idx) {
   a.n_paths = l->n_paths;
   a.n_times = l->n_times;
   a.d = l->dim;
+  // zero strides: the reference's [n_paths][n_times][dim] layout
+  const bool given = l->stride_path || l->stride_time || l->stride_asset;
+  a.sp = given ? l->stride_path : (int64_t)l->n_times * l->dim;
+  a.st = given ? l->stride_time : l->dim;
+  a.sa = given ? l->stride_asset : 1;
   a.t_idx = t_idx;
   a.payoff = l->payoff;
   a.strike = l->strike;

@@ -1,15 +1,19 @@
 // Small MLP (6 -> 64 -> 64 -> 1, SiLU) for the Longstaff-Schwartz
 // continuation value: forward, backward and Adam as hand-written kernels.
 //
-// The 64x64 layer runs on MFMA (v_mfma_f32_16x16x16_bf16: bf16 operands,
-// float32 accumulation); the 6->64 and 64->1 layers are VALU.  One wave
-// handles 16 samples per MFMA group and keeps the whole layer-2 weight
-// block as MFMA operands in registers:
+// The 6->64 and 64x64 layers run on MFMA (v_mfma_f32_16x16x16_bf16: bf16
+// operands, float32 accumulation), the 64->1 layer on VALU.  Layer 1 is
+// computed transposed, Z1^T = W1^T X^T (inputs padded to k = 16, the bias
+// as the accumulator's initial value), so its output lands in the A-operand
+// layout layer 2 needs; dW1 = X^T dZ1 is one more MFMA per 16-column tile.
+// One wave handles 16 samples per MFMA group and keeps the layer-1 and
+// layer-2 weights as MFMA operands in registers:
 //   A operand (16 x 16, K chunk c): lane l holds row l % 16, k = c*16 +
 //     4*(l/16) + {0..3};   B operand: column l % 16, same k;
 //   C (16 x 16 float32): lane l holds column l % 16, rows 4*(l/16) + i.
-// So the VALU layer-1 output of lane l (sample l % 16, features
-// c*16 + 4*(l/16) + j) is already the A operand of layer 2, and the layer-2
+// So the transposed layer-1 output of lane l (C layout of Z1^T: sample
+// l % 16, features c*16 + 4*(l/16) + j) is already the A operand of
+// layer 2, and the layer-2
 // output in C layout is already the B operand of the weight-gradient
 // product dW2 = H1^T dZ2.  Two small LDS transposes remain (H1^T and dZ2 as
 // an A operand).
@@ -31,11 +35,21 @@ constexpr int oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH,
               oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH;
 static_assert(oB3 + 1 == kMlpParams, "parameter layout");
 
-__device__ __forceinline__ short to_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-  return (short)(u >> 16);
+// float -> bf16, round to nearest even: gfx950's v_cvt_pk_bf16_f32
+// converts two values per instruction (in place of a shift-add-round
+// sequence and a pack per value)
+typedef __bf16 bf16x2h __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ bf16x4 pack_bf16(float a, float b, float c,
+                                            float d) {
+  const bf16x2h lo = {(__bf16)a, (__bf16)b};
+  const bf16x2h hi = {(__bf16)c, (__bf16)d};
+  const uint32_t ul = __builtin_bit_cast(uint32_t, lo);
+  const uint32_t uh = __builtin_bit_cast(uint32_t, hi);
+  return bf16x4{(short)(ul & 0xffffu), (short)(ul >> 16),
+                (short)(uh & 0xffffu), (short)(uh >> 16)};
 }
+
 
 // logistic sigmoid through the hardware reciprocal (1 ulp; the network runs
 // its products in bf16)
@@ -81,35 +95,42 @@ __device__ __forceinline__ float sum16(float v) {
 // Forward for one group.  Returns z1 (A layout, 16), h1 bf16 A operands,
 // z2 (C layout, 16), y (4 samples of the lane's row block).
 struct Fwd {
-  float z1a[4][4];    // [chunk c][j]: sample l%16, feature c*16+4*(l/16)+j
-  float s1a[4][4];    // sigmoid(z1a), same layout
-  bf16x4 h1a[4];      // A operands of layer 2
+  bf16x4 h1a[4];      // A operands of layer 2: sample l%16, features
+                      // c*16 + 4*(l/16) + j
   f32x4 z2[4];        // [n-tile t]: rows 4*(l/16)+i, column t*16 + l%16
   f32x4 s2[4];        // sigmoid(z2), same layout
   float y[4];         // outputs of rows 4*(l/16)+i
 };
 
+// g1 (optional, per-wave LDS [sample][kG1Stride]) receives SiLU'(z1) for
+// the backward pass, written as soon as it is known so z1 and its sigmoid
+// need no registers past layer 1.
 __device__ __forceinline__ void forward_group(const float* __restrict__ P,
-                                              const bf16x4* w2b, const float* xa,
-                                              Fwd& f) {
+                                              const bf16x4* w1a,
+                                              const bf16x4* w2b, bf16x4 xb,
+                                              Fwd& f, float* g1 = nullptr) {
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, r = lane & 15;
-  // layer 1 (VALU): features c*16 + 4q + j of sample r
+  // layer 1 (MFMA, transposed): tile c of Z1^T = W1^T[c] X^T + b1, A = the
+  // W1^T tile (rows features c*16 + r, k inputs 4q + j), B = X^T (column
+  // sample r, k inputs 4q + j); the C layout (column sample r, rows
+  // features c*16 + 4q + j) is the A layout of layer 2
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    short h[4];
+    f32x4 z;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) z[j] = P[oB1 + c * 16 + 4 * q + j];
+    z = mfma(w1a[c], xb, z);
+    float h[4];
+    f32x4 gv;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int fo = c * 16 + 4 * q + j;
-      float z = P[oB1 + fo];
-#pragma unroll
-      for (int i = 0; i < kIn; ++i) z += xa[i] * P[oW1 + i * kH + fo];
-      f.z1a[c][j] = z;
-      const float sg = sigmoid(z);
-      f.s1a[c][j] = sg;
-      h[j] = to_bf16(z * sg);
+      const float sg = sigmoid(z[j]);
+      h[j] = z[j] * sg;
+      gv[j] = silu_grad_s(z[j], sg);
     }
-    f.h1a[c] = bf16x4{h[0], h[1], h[2], h[3]};
+    f.h1a[c] = pack_bf16(h[0], h[1], h[2], h[3]);
+    if (g1) *(f32x4*)(g1 + r * kG1Stride + c * 16 + 4 * q) = gv;
   }
   // layer 2 (MFMA): Z2[16 x 64] = H1[16 x 64] W2[64 x 64]
 #pragma unroll
@@ -136,6 +157,35 @@ __device__ __forceinline__ void forward_group(const float* __restrict__ P,
   }
 }
 
+// W1^T as layer-1 A operands: tile c, row (feature) c*16 + r, k (input)
+// 4q + j; inputs past kIn are zero.
+__device__ __forceinline__ void load_w1(const float* __restrict__ P,
+                                        bf16x4* w1a) {
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * q + j;
+      v[j] = k < kIn ? P[oW1 + k * kH + c * 16 + r] : 0.f;
+    }
+    w1a[c] = pack_bf16(v[0], v[1], v[2], v[3]);
+  }
+}
+
+// X^T as the layer-1 B operand: sample row s (column r of the tile),
+// inputs 4q + j (zero past kIn or past the end)
+__device__ __forceinline__ void load_x_op(const float* __restrict__ x,
+                                          int64_t s, bool ok, float* v) {
+  const int q = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = 4 * q + j;
+    v[j] = (ok && k < kIn) ? x[s * kMlpStride + k] : 0.f;
+  }
+}
+
 // W2 as layer-2 B operands: tile (t, c): column t*16 + r, rows
 // c*16 + 4q + {0..3}.  W2 is stored [in][out].
 __device__ __forceinline__ void load_w2(const float* __restrict__ P,
@@ -145,11 +195,11 @@ __device__ __forceinline__ void load_w2(const float* __restrict__ P,
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      short v[4];
+      float v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        v[j] = to_bf16(P[oW2 + (c * 16 + 4 * q + j) * kH + t * 16 + r]);
-      w2b[t * 4 + c] = bf16x4{v[0], v[1], v[2], v[3]};
+        v[j] = P[oW2 + (c * 16 + 4 * q + j) * kH + t * 16 + r];
+      w2b[t * 4 + c] = pack_bf16(v[0], v[1], v[2], v[3]);
     }
 }
 
@@ -162,11 +212,11 @@ __device__ __forceinline__ void load_w2t(const float* __restrict__ P,
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      short v[4];
+      float v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        v[j] = to_bf16(P[oW2 + (t * 16 + r) * kH + c * 16 + 4 * q + j]);
-      w2t[t * 4 + c] = bf16x4{v[0], v[1], v[2], v[3]};
+        v[j] = P[oW2 + (t * 16 + r) * kH + c * 16 + 4 * q + j];
+      w2t[t * 4 + c] = pack_bf16(v[0], v[1], v[2], v[3]);
     }
 }
 
@@ -201,27 +251,25 @@ mlp_forward_kernel(const float* __restrict__ params,
   __shared__ float P[kMlpParams];
   stage_params(params, P, nullptr);
   __syncthreads();
-  bf16x4 w2b[16];
+  bf16x4 w1a[4], w2b[16];
+  load_w1(P, w1a);
   load_w2(P, w2b);
   const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
   const int64_t waves = (int64_t)gridDim.x * 4;
   const int64_t wid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  // the next group's rows are loaded while this one is processed
-  float nx[kIn];
+  // the next group's inputs are loaded while this one is processed
+  float nx[4];
   auto fetch = [&](int64_t g) {
     const int64_t sa = g * 16 + r;
-#pragma unroll
-    for (int i = 0; i < kIn; ++i) nx[i] = sa < n ? x[sa * kMlpStride + i] : 0.f;
+    load_x_op(x, sa, sa < n, nx);
   };
   fetch(wid);
   for (int64_t g = wid; g * 16 < n; g += waves) {
     const int64_t base = g * 16;
-    float xa[kIn];
-#pragma unroll
-    for (int i = 0; i < kIn; ++i) xa[i] = nx[i];
+    const bf16x4 xb = pack_bf16(nx[0], nx[1], nx[2], nx[3]);
     fetch(g + waves);
     Fwd f;
-    forward_group(P, w2b, xa, f);
+    forward_group(P, w1a, w2b, xb, f);
     if (r == 0) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -234,35 +282,46 @@ mlp_forward_kernel(const float* __restrict__ params,
 
 // Gradient of L = sum_s w_s (y_s - t_s)^2 / norm over [0, n), per-block
 // partial sums of the kMlpParams gradient entries (and the loss at the end).
-__global__ void __launch_bounds__(256)
+template <int kF>
+__global__ void __launch_bounds__(256, kF == 1 ? 2 : 1)
 mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
                 const float* __restrict__ tgt, const float* __restrict__ wts,
                 int64_t first, int64_t n, float inv_norm,
                 float* __restrict__ partial) {
-  __shared__ float P[kMlpParams];
-  __shared__ float Wsum[4][kMlpParams + 1];  // per-wave gradient rows
-  // per-wave transpose buffers (bf16) of the two groups in flight
-  __shared__ short tr[4][2][16 * 64];
-  // per-wave SiLU'(z1) of the two groups, [sample][feature] with a row
+  // the parameters, then (after the group loop) the block's gradient row:
+  // the four waves add their rows into it in wave order
+  __shared__ float P[kMlpParams + 1];
+  // per-wave transpose buffers (bf16) of the kF groups in flight
+  __shared__ short tr[4][kF][16 * 64];
+  // per-wave SiLU'(z1) of the kF groups, [sample][feature] with a row
   // stride of 68 floats (16-byte rows, the four row groups of a read in
   // distinct banks)
-  __shared__ float g1s[4][2][16 * kG1Stride];
+  __shared__ float g1s[4][kF][16 * kG1Stride];
+  __shared__ bf16x4 w2t_s[16 * 64];
   stage_params(params, P, nullptr);
   __syncthreads();
-  bf16x4 w2b[16], w2t[16];
-  load_w2(P, w2b);
-  load_w2t(P, w2t);
   const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
   const int wave = threadIdx.x >> 6;
-  f32x4 dw2[16];
+  bf16x4 w1a[4], w2b[16];
+  load_w1(P, w1a);
+  load_w2(P, w2b);
+  // the W2^T operands stay in LDS in operand order (one 8-byte read per
+  // lane and tile in the backward pass), written once by wave 0
+  if (wave == 0) {
+    bf16x4 w2t[16];
+    load_w2t(P, w2t);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) w2t_s[k * 64 + lane] = w2t[k];
+  }
+  __syncthreads();
+  f32x4 dw2[16], dw1t[4];
 #pragma unroll
   for (int e = 0; e < 16; ++e) dw2[e] = f32x4{0.f, 0.f, 0.f, 0.f};
-  float dw1[kIn][4], db1[4], db2[4], dw3[4], db3 = 0.f, loss = 0.f;
+  float db1[4], db2[4], dw3[4], db3 = 0.f, loss = 0.f;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     db1[t] = db2[t] = dw3[t] = 0.f;
-#pragma unroll
-    for (int i = 0; i < kIn; ++i) dw1[i][t] = 0.f;
+    dw1t[t] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
   const int64_t waves = (int64_t)gridDim.x * 4;
   const int64_t wid = (int64_t)blockIdx.x * 4 + wave;
@@ -273,67 +332,59 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
   // accumulator takes group g's terms before group g + waves', the order
   // of one group at a time; a partner past the end has zero inputs and
   // weight and adds exact zeros.
-  // inputs of a group: lane l holds sample base + l % 16 (its row, weight
-  // and target; zero past the end), loaded one pair ahead; the rows 4q + i
-  // the backward pass needs come by shuffles.
-  float nx[2][kIn], nw[2], nt[2];
+  // inputs of a group, loaded one pair ahead: lane l holds, of sample
+  // base + l % 16, inputs 4q .. 4q + 3 (the layer-1 B operand), its weight
+  // and target (zero past the end), and, of samples base + 4q + j, input
+  // l % 16 (the B operand of dW1 = X^T dZ1; zero past kIn or the end)
+  float nx[kF][4], nxt[kF][4], nw[kF], nt[kF];
   auto fetch = [&](int u, int64_t g) {
     const int64_t sa = first + g * 16 + r;
     const bool ok = g * 16 < n && sa < end;
-#pragma unroll
-    for (int i = 0; i < kIn; ++i) nx[u][i] = ok ? x[sa * kMlpStride + i] : 0.f;
+    load_x_op(x, sa, ok, nx[u]);
     nw[u] = ok ? wts[sa] : 0.f;
     nt[u] = ok ? tgt[sa] : 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t sb = first + g * 16 + 4 * q + j;
+      nxt[u][j] = (g * 16 < n && sb < end && r < kIn) ? x[sb * kMlpStride + r]
+                                                       : 0.f;
+    }
   };
-  fetch(0, wid);
-  fetch(1, wid + waves);
-  for (int64_t g = wid; g * 16 < n; g += 2 * waves) {
-    float xa[2][kIn], wa[2], ta[2];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+  for (int u = 0; u < kF; ++u) fetch(u, wid + u * waves);
+  for (int64_t g = wid; g * 16 < n; g += kF * waves) {
+    bf16x4 xb[kF], xtb[kF];
+    float wa[kF], ta[kF];
 #pragma unroll
-      for (int i = 0; i < kIn; ++i) xa[u][i] = nx[u][i];
+    for (int u = 0; u < kF; ++u) {
+      xb[u] = pack_bf16(nx[u][0], nx[u][1], nx[u][2], nx[u][3]);
+      xtb[u] = pack_bf16(nxt[u][0], nxt[u][1], nxt[u][2], nxt[u][3]);
       wa[u] = nw[u];
       ta[u] = nt[u];
     }
-    fetch(0, g + 2 * waves);
-    fetch(1, g + 3 * waves);
-    Fwd f[2];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) forward_group(P, w2b, xa[u], f[u]);
+    for (int u = 0; u < kF; ++u) fetch(u, g + (kF + u) * waves);
+    Fwd f[kF];
+#pragma unroll
+    for (int u = 0; u < kF; ++u)
+      forward_group(P, w1a, w2b, xb[u], f[u], g1s[wave][u]);
     // dy for the lane's rows
-    float dy[2][4], xr[2][4][kIn];
+    float dy[kF][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < kF; ++u)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int src = 4 * q + i;
-#pragma unroll
-        for (int k = 0; k < kIn; ++k) xr[u][i][k] = __shfl(xa[u][k], src, 64);
         const float w = __shfl(wa[u], src, 64), t = __shfl(ta[u], src, 64);
         const float e = f[u].y[i] - t;
         dy[u][i] = 2.f * w * e * inv_norm;
         if (r == 0) loss += w * e * e * inv_norm;
         db3 += r == 0 ? dy[u][i] : 0.f;
       }
-    // SiLU'(z1) from the forward pass, to the C layout of the layer-1
-    // backward through LDS (in place of recomputing z1 and its sigmoid)
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      float* G1 = g1s[wave][u];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        f32x4 gv;
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          gv[j] = silu_grad_s(f[u].z1a[c][j], f[u].s1a[c][j]);
-        *(f32x4*)(G1 + r * kG1Stride + c * 16 + 4 * q) = gv;
-      }
-    }
     // layer 3 and SiLU' of layer 2: dZ2 in C layout (rows 4q+i, col t*16+r)
-    f32x4 dz2[2][4];
+    f32x4 dz2[kF][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < kF; ++u)
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const float w3 = P[oW3 + t * 16 + r];
@@ -350,16 +401,16 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
     // dW2 += H1^T dZ2: A = H1^T tile (features mt*16 + r, samples 4q + j)
     // from the transpose of the A-layout h1 via LDS.
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < kF; ++u)
 #pragma unroll
       for (int c = 0; c < 4; ++c)
 #pragma unroll
         for (int j = 0; j < 4; ++j)  // [sample][feature]
           tr[wave][u][r * 64 + c * 16 + 4 * q + j] = f[u].h1a[c][j];
     wave_lds_sync();
-    bf16x4 h1t[2][4], dzb[2][4];
+    bf16x4 h1t[kF][4], dzb[kF][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kF; ++u) {
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) {
         short v[4];
@@ -371,11 +422,11 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
       // dZ2 as B operand: col t*16+r, k (samples) 4q + i
 #pragma unroll
       for (int t = 0; t < 4; ++t)
-        dzb[u][t] = bf16x4{to_bf16(dz2[u][t][0]), to_bf16(dz2[u][t][1]),
-                           to_bf16(dz2[u][t][2]), to_bf16(dz2[u][t][3])};
+        dzb[u][t] = pack_bf16(dz2[u][t][0], dz2[u][t][1], dz2[u][t][2],
+                              dz2[u][t][3]);
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < kF; ++u)
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
@@ -384,16 +435,16 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
     // dH1 = dZ2 W2^T: A = dZ2 (rows = samples r, k = n c*16 + 4q + j) via LDS
     wave_lds_sync();
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < kF; ++u)
 #pragma unroll
       for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int i = 0; i < 4; ++i)  // [sample][n]
           tr[wave][u][(4 * q + i) * 64 + t * 16 + r] = dzb[u][t][i];
     wave_lds_sync();
-    bf16x4 dza[2][4];
+    bf16x4 dza[kF][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < kF; ++u)
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         short v[4];
@@ -404,74 +455,94 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
       }
     wave_lds_sync();
     // dH1 tiles (C layout): rows (samples) 4q + i, features t*16 + r
-    f32x4 dh[2][4];
+    f32x4 dh[kF][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < kF; ++u)
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc = mfma(dza[u][c], w2t[t * 4 + c], acc);
+        for (int c = 0; c < 4; ++c)
+          acc = mfma(dza[u][c], w2t_s[(t * 4 + c) * 64 + lane], acc);
         dh[u][t] = acc;
       }
+    // dZ1 = dH1 SiLU'(z1) (C layout: rows samples 4q + i, column feature
+    // t*16 + r) is the A operand of dW1^T (row feature, k sample): tile t
+    // of dW1^T += dZ1^T X
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kF; ++u) {
       const float* G1 = g1s[wave][u];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int fo = t * 16 + r;
+        float d[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const float d = dh[u][t][i] * G1[(4 * q + i) * kG1Stride + fo];
-          db1[t] += d;
-#pragma unroll
-          for (int k = 0; k < kIn; ++k) dw1[k][t] += xr[u][i][k] * d;
+          d[i] = dh[u][t][i] * G1[(4 * q + i) * kG1Stride + fo];
+          db1[t] += d[i];
         }
+        dw1t[t] = mfma(pack_bf16(d[0], d[1], d[2], d[3]), xtb[u], dw1t[t]);
       }
     }
   }
-  // block reduction in a fixed order: each wave adds its four lane groups
-  // (q = lane / 16, which hold the same features) with shuffles and writes
-  // one row of LDS; the block adds the four rows in wave order.  (Every W2
-  // entry has exactly one owning lane per wave.)
-  float* Wr = Wsum[wave];
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        Wr[oW2 + (mt * 16 + 4 * q + i) * kH + t * 16 + r] = dw2[mt * 4 + t][i];
+  // block reduction in a fixed order: each wave folds its four lane groups
+  // (q = lane / 16, which hold the same features) with shuffles where
+  // needed, and the waves add their rows into P (free once every wave has
+  // left the group loop) in wave order: every entry has exactly one owning
+  // lane per wave.
   auto qsum = [](float v) {
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
     return v;
   };
+  float sb1[4], sb2[4], sw3[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    const int fo = t * 16 + r;
-    const float sb1 = qsum(db1[t]), sb2 = qsum(db2[t]), sw3 = qsum(dw3[t]);
-    float sw1[kIn];
-#pragma unroll
-    for (int k = 0; k < kIn; ++k) sw1[k] = qsum(dw1[k][t]);
-    if (q == 0) {
-      Wr[oB1 + fo] = sb1;
-      Wr[oB2 + fo] = sb2;
-      Wr[oW3 + fo] = sw3;
-#pragma unroll
-      for (int k = 0; k < kIn; ++k) Wr[oW1 + k * kH + fo] = sw1[k];
-    }
+    sb1[t] = qsum(db1[t]);
+    sb2[t] = qsum(db2[t]);
+    sw3[t] = qsum(dw3[t]);
   }
   // db3 and the loss live on the lanes with r == 0
   const float sb3 = qsum(db3), sl = qsum(loss);
-  if (lane == 0) {
-    Wr[oB3] = sb3;
-    Wr[kMlpParams] = sl;
-  }
   __syncthreads();
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+      auto put = [&](int e, float v) { P[e] = w == 0 ? v : P[e] + v; };
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            put(oW2 + (mt * 16 + 4 * q + i) * kH + t * 16 + r,
+                dw2[mt * 4 + t][i]);
+      // dW1^T tile t (C layout): column input r, rows features
+      // t*16 + 4q + i
+      if (r < kIn) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            put(oW1 + r * kH + t * 16 + 4 * q + i, dw1t[t][i]);
+      }
+      if (q == 0) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int fo = t * 16 + r;
+          put(oB1 + fo, sb1[t]);
+          put(oB2 + fo, sb2[t]);
+          put(oW3 + fo, sw3[t]);
+        }
+      }
+      if (lane == 0) {
+        put(oB3, sb3);
+        put(kMlpParams, sl);
+      }
+    }
+    __syncthreads();
+  }
   for (int e = threadIdx.x; e < kMlpParams + 1; e += 256)
-    partial[(int64_t)blockIdx.x * (kMlpParams + 1) + e] =
-        ((Wsum[0][e] + Wsum[1][e]) + Wsum[2][e]) + Wsum[3][e];
+    partial[(int64_t)blockIdx.x * (kMlpParams + 1) + e] = P[e];
 }
 
 // The block partials summed and the Adam update in one kernel: workgroup
@@ -479,6 +550,9 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
 // the read); half-wave q = 2 w + (lane >> 5) of its kRedSlices adds the rows
 // of blocks [q per, (q + 1) per) in block order (consecutive lanes read
 // consecutive parameters), then the slice sums are added in slice order.
+// groups of 16 samples each wave keeps in flight in the gradient kernel
+constexpr int kGroupsInFlight = 2;
+
 constexpr int kRedWaves = 16;
 constexpr int kRedParams = 32;
 constexpr int kRedSlices = kRedWaves * 64 / kRedParams;
@@ -540,8 +614,8 @@ int mlp_grad_step(float* params, float* m, float* v, const float* x,
                   float inv_norm, float lr, int step, float* partial,
                   int n_blocks, float* grad_out, float* loss_out,
                   hipStream_t s) {
-  hipLaunchKernelGGL(mlp_grad_kernel, n_blocks, 256, 0, s, params, x, tgt, w,
-                     first, n, inv_norm, partial);
+  hipLaunchKernelGGL(mlp_grad_kernel<kGroupsInFlight>, n_blocks, 256, 0, s,
+                     params, x, tgt, w, first, n, inv_norm, partial);
   hipLaunchKernelGGL(mlp_reduce_adam_kernel,
                      (kMlpParams + 1 + kRedParams - 1) / kRedParams,
                      kRedWaves * 64, 0, s, params, m, v, partial, n_blocks, lr,

@@ -78,7 +78,9 @@ class Lsm(ctypes.Structure):
               ('dim', _i32), ('payoff', _i32), ('degree', _i32),
               ('n_basis', _i32), ('basis_log', _i32), ('exponents', _c_void_p),
               ('strike', _f64), ('num_calibration', _i64), ('rcond', _f64),
-              ('payoff_table', _c_void_p), ('path_discounts', _c_void_p)]
+              ('payoff_table', _c_void_p), ('path_discounts', _c_void_p),
+              ('stride_path', _i64), ('stride_time', _i64),
+              ('stride_asset', _i64)]
 
 
 class Contract(ctypes.Structure):

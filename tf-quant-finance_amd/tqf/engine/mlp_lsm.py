@@ -98,11 +98,13 @@ class MlpModel:
 
 
 def _lsm_struct(paths, payoff, strike, degree=2):
+  """paths: float64 [N, T, d] with any strides."""
   n, nt, d = paths.shape
   expo = lsm_lib.exponent_grid(degree, d).astype(np.uint8)
   expo_t = torch.as_tensor(expo, device=paths.device).contiguous()
   l = _native.Lsm()
   l.paths = paths.data_ptr()
+  l.stride_path, l.stride_time, l.stride_asset = paths.stride()
   l.n_paths = n
   l.n_times = nt
   l.dim = d
@@ -129,7 +131,7 @@ def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
   the final discount).
   """
   dev = paths.device
-  paths = paths.contiguous()
+  paths = lsm_lib.step_major_f64(paths)
   n = paths.shape[0]
   ex = [int(v) for v in exercise_idx]
   df = np.concatenate([[1.0], np.asarray(discount_factors, np.float64)])
@@ -222,9 +224,9 @@ def bermudan_max_call_paths(n_paths, dim=5, s0=100.0, sigma=0.2, corr=0.3,
                             r=0.05, q=0.1, maturity=3.0, n_dates=9,
                             steps_per_date=20, seed=17):
   """Correlated GBM paths at t = 0 and the exercise dates, float64
-  [n_paths, n_dates + 1, dim], simulated over n_dates * steps_per_date
-  exact steps by tqf_gbm_exact (Philox).  Also returns the discount factors
-  of the exercise dates."""
+  [n_paths, n_dates + 1, dim] (a view of step-major memory [T][d][N]),
+  simulated over n_dates * steps_per_date exact steps by tqf_gbm_exact
+  (Philox).  Also returns the discount factors of the exercise dates."""
   from tqf.engine import euler as engine
   from tqf.models.geometric_brownian_motion import run_exact
   steps = n_dates * steps_per_date
@@ -237,6 +239,6 @@ def bermudan_max_call_paths(n_paths, dim=5, s0=100.0, sigma=0.2, corr=0.3,
   out = run_exact(grid[1:], drift, vol, np.linalg.cholesky(c),
                   np.full(dim, s0), int(n_paths),
                   engine.RngSpec('philox', seed=seed), torch.float32,
-                  keep_mask=keep)
+                  keep_mask=keep, mode='store')  # [T][d][N]
   dates = grid[keep][1:]
-  return out.to(torch.float64), np.exp(-r * dates)
+  return out.to(torch.float64).permute(2, 0, 1), np.exp(-r * dates)

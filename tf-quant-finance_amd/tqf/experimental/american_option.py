@@ -61,18 +61,23 @@ def bs_lsm_price(spots, expiry_times, strikes, volatility, discount_factors,
                           random_type=RandomType.STATELESS_ANTITHETIC)
   t = torch.as_tensor(times, dtype=torch.float64, device=dev).reshape(1, -1)
   w = torch.log(unit[..., 0]) + 0.5 * t  # W(t), [N, T]
+  # the same in date-major memory [T, N]: the regression kernels read one
+  # date of every path at a time
+  w_t = w.t().contiguous()
+  t_t = t.t()
   ex = list(range(int(num_exercise_times)))
   if basis_fn is None:
     basis_fn = lsm_lib.make_polynomial_basis(2)
   bspec = getattr(basis_fn, '_tqf_basis', None)
   prices = []
   for b in range(n_opt):
-    tt = t * expiry[b]
-    log_s = (vol[b] * np.sqrt(expiry[b])) * w + (rate[b] - 0.5 * vol[b]**2) * tt
     dfs = np.exp(-rate[b] * times * expiry[b])
     kind = lsm_lib.PAYOFF_MAX_CALL if call[b] else lsm_lib.PAYOFF_BASKET_PUT
     if bspec is not None and bspec[0] == 'poly':
-      paths = torch.exp(log_s).unsqueeze(-1).contiguous()  # [N, T, 1]
+      log_s_t = ((vol[b] * np.sqrt(expiry[b])) * w_t +
+                 (rate[b] - 0.5 * vol[b]**2) * (t_t * expiry[b]))
+      # [N, T, 1] view of [T][1][N] memory
+      paths = torch.exp(log_s_t).unsqueeze(1).permute(2, 0, 1)
       p = lsm_lib._native_lsm(
           paths, ex, (kind, np.array([strike[b]])), bspec[1],
           np.concatenate([[1.0], dfs]).reshape(-1, 1, 1),
@@ -80,6 +85,8 @@ def bs_lsm_price(spots, expiry_times, strikes, volatility, discount_factors,
           torch.float64, dev, basis_log=True)
       prices.append(float(p[0]))
     else:
+      log_s = ((vol[b] * np.sqrt(expiry[b])) * w +
+               (rate[b] - 0.5 * vol[b]**2) * (t * expiry[b]))
       k, is_call = strike[b], bool(call[b])
 
       def payoff_fn(log_paths, time_index, k=k, is_call=is_call):

@@ -160,6 +160,20 @@ def least_square_mc(sample_paths, exercise_times, payoff_fn, basis_fn,
                     dtype)
 
 
+def step_major_f64(paths):
+  """[N, T, d] paths as a float64 view whose memory is [T][d][N] (the
+  kernels read one date of every path per pass: with this layout those
+  reads are coalesced).  Returns the input when it already is one."""
+  if paths.dtype == torch.float64 and paths.stride(0) == 1 and (
+      paths.stride(2) == paths.shape[0] and
+      paths.stride(1) == paths.shape[0] * paths.shape[2]):
+    return paths
+  n, t, d = paths.shape
+  out = torch.empty((t, d, n), dtype=torch.float64, device=paths.device)
+  out.copy_(paths.permute(1, 2, 0))
+  return out.permute(2, 0, 1)
+
+
 def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
                 return_details=False, basis_log=False, exponents=None,
                 path_values=None):
@@ -200,9 +214,10 @@ def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
   exi = (ctypes.c_int32 * len(ex))(*ex)
   prices, ses, ranks_all = [], [], []
   for b in range(n_batch):
-    p64 = paths_b[b if bp > 1 else 0].to(torch.float64).contiguous()
+    p64 = step_major_f64(paths_b[b if bp > 1 else 0])
     l = _native.Lsm()
     l.paths = p64.data_ptr()
+    l.stride_path, l.stride_time, l.stride_asset = p64.stride()
     l.n_paths = n
     l.n_times = p64.shape[1]
     l.dim = d
