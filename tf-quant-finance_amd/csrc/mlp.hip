@@ -288,16 +288,29 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
                 const float* __restrict__ tgt, const float* __restrict__ wts,
                 int64_t first, int64_t n, float inv_norm,
                 float* __restrict__ partial) {
-  // the parameters, then (after the group loop) the block's gradient row:
-  // the four waves add their rows into it in wave order
-  __shared__ float P[kMlpParams + 1];
-  // per-wave transpose buffers (bf16) of the kF groups in flight
-  __shared__ short tr[4][kF][16 * 64];
-  // per-wave SiLU'(z1) of the kF groups, [sample][feature] with a row
-  // stride of 68 floats (16-byte rows, the four row groups of a read in
-  // distinct banks)
-  __shared__ float g1s[4][kF][16 * kG1Stride];
-  __shared__ bf16x4 w2t_s[16 * 64];
+  // LDS: the parameters P (after the group loop: wave 0's gradient row),
+  // then the loop buffers, which waves 1-3 reuse for their gradient rows
+  // once every wave has left the loop
+  //   tr   per-wave transpose buffers (bf16) of the kF groups in flight
+  //   g1s  per-wave SiLU'(z1) of the kF groups, [sample][feature] with a
+  //        row stride of 68 floats (16-byte rows, the four row groups of a
+  //        read in distinct banks)
+  //   w2t  the W2^T operands in operand order
+  constexpr int kRow = kMlpParams + 1;
+  constexpr int kTrBytes = 4 * kF * 16 * 64 * 2;
+  constexpr int kG1Bytes = 4 * kF * 16 * kG1Stride * 4;
+  constexpr int kW2tBytes = 16 * 64 * 8;
+  constexpr int kLoopBytes = kTrBytes + kG1Bytes + kW2tBytes;
+  constexpr int kRowsBytes = 3 * kRow * 4;
+  constexpr int kPBytes = (kRow * 4 + 15) / 16 * 16;
+  __shared__ __attribute__((aligned(16)))
+      char smem[kPBytes + (kLoopBytes > kRowsBytes ? kLoopBytes : kRowsBytes)];
+  float* P = (float*)smem;
+  short (*tr)[kF][16 * 64] = (short (*)[kF][16 * 64])(smem + kPBytes);
+  float (*g1s)[kF][16 * kG1Stride] =
+      (float (*)[kF][16 * kG1Stride])(smem + kPBytes + kTrBytes);
+  bf16x4* w2t_s = (bf16x4*)(smem + kPBytes + kTrBytes + kG1Bytes);
+  float* rows = (float*)(smem + kPBytes);
   stage_params(params, P, nullptr);
   __syncthreads();
   const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
@@ -487,9 +500,10 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
   }
   // block reduction in a fixed order: each wave folds its four lane groups
   // (q = lane / 16, which hold the same features) with shuffles where
-  // needed, and the waves add their rows into P (free once every wave has
-  // left the group loop) in wave order: every entry has exactly one owning
-  // lane per wave.
+  // needed and writes its gradient row (every entry has exactly one owning
+  // lane per wave): wave 0 into P, waves 1-3 into the loop buffers, all
+  // free once every wave has left the group loop.  The rows are then added
+  // in wave order, one entry per thread.
   auto qsum = [](float v) {
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
@@ -505,44 +519,40 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
   // db3 and the loss live on the lanes with r == 0
   const float sb3 = qsum(db3), sl = qsum(loss);
   __syncthreads();
-  for (int w = 0; w < 4; ++w) {
-    if (wave == w) {
-      auto put = [&](int e, float v) { P[e] = w == 0 ? v : P[e] + v; };
+  float* row = wave == 0 ? P : rows + (wave - 1) * kRow;
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt)
+  for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
-            put(oW2 + (mt * 16 + 4 * q + i) * kH + t * 16 + r,
-                dw2[mt * 4 + t][i]);
-      // dW1^T tile t (C layout): column input r, rows features
-      // t*16 + 4q + i
-      if (r < kIn) {
+      for (int i = 0; i < 4; ++i)
+        row[oW2 + (mt * 16 + 4 * q + i) * kH + t * 16 + r] =
+            dw2[mt * 4 + t][i];
+  // dW1^T tile t (C layout): column input r, rows features t*16 + 4q + i
+  if (r < kIn) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
-            put(oW1 + r * kH + t * 16 + 4 * q + i, dw1t[t][i]);
-      }
-      if (q == 0) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int fo = t * 16 + r;
-          put(oB1 + fo, sb1[t]);
-          put(oB2 + fo, sb2[t]);
-          put(oW3 + fo, sw3[t]);
-        }
-      }
-      if (lane == 0) {
-        put(oB3, sb3);
-        put(kMlpParams, sl);
-      }
-    }
-    __syncthreads();
+      for (int i = 0; i < 4; ++i)
+        row[oW1 + r * kH + t * 16 + 4 * q + i] = dw1t[t][i];
   }
-  for (int e = threadIdx.x; e < kMlpParams + 1; e += 256)
-    partial[(int64_t)blockIdx.x * (kMlpParams + 1) + e] = P[e];
+  if (q == 0) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int fo = t * 16 + r;
+      row[oB1 + fo] = sb1[t];
+      row[oB2 + fo] = sb2[t];
+      row[oW3 + fo] = sw3[t];
+    }
+  }
+  if (lane == 0) {
+    row[oB3] = sb3;
+    row[kMlpParams] = sl;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kRow; e += 256)
+    partial[(int64_t)blockIdx.x * kRow + e] =
+        ((P[e] + rows[e]) + rows[kRow + e]) + rows[2 * kRow + e];
 }
 
 // The block partials summed and the Adam update in one kernel: workgroup
