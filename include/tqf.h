@@ -344,7 +344,9 @@ typedef struct tqf_lsm {
                                (degree + 1) * dim <= 64                       */
   int32_t n_basis;          /* (degree + 1)^dim, at most 255                 */
   int32_t basis_log;        /* 0: basis on the paths' values; 1: on their
-                               logarithms (the payoff always uses values)    */
+                               logarithms (the payoff on the values); 2:
+                               the paths hold log prices: basis on the
+                               stored values, payoff on their exp          */
   const uint8_t* exponents; /* device [n_basis][dim] exponent table in the
                                reference's meshgrid order (lsm.py:116-118);
                                every entry <= degree and n_basis <=

@@ -342,3 +342,40 @@ def test_total_degree_table_takes_the_dense_gram_and_basis_order_update():
                         lsm._padded_discounts(df, 4, torch.float64), None,
                         torch.float64, dev, exponents=table)
   np.testing.assert_allclose(_np(got), want, rtol=2e-6)
+
+
+@pytest.mark.parametrize('dim', [1, 2])
+def test_log_price_paths_equal_the_log_basis_on_prices(dim):
+  # basis_log = 2 (paths hold ln S: basis on them, payoff on exp, the
+  # one-asset in-the-money test as ln S < ln K) against basis_log = 1 on
+  # exp of the same paths, with half the rows as calibration samples; one
+  # asset runs the two-launch date chain, two the general one
+  from tqf.models import longstaff_schwartz as lsm_lib
+  rs = np.random.RandomState(5 + dim)
+  n, t = 40000, 9
+  z = rs.standard_normal((n, t, dim))
+  times = np.linspace(0.1, 0.9, t)
+  log_s = np.log(ogbm.sample_paths(times, 0.03, 0.25, 1.0, z))
+  dfs = np.exp(-0.03 * times)
+  dev = torch.device('cuda')
+  lg = torch.as_tensor(log_s, device=dev)
+  df = np.concatenate([[1.0], dfs]).reshape(-1, 1, 1)
+  for kind, k in ((lsm_lib.PAYOFF_BASKET_PUT, 1.05),
+                  (lsm_lib.PAYOFF_MAX_CALL, 0.95)):
+    got = lsm_lib._native_lsm(lg, list(range(t)), (kind, np.array([k])), 2,
+                              df, n // 2, torch.float64, dev, basis_log=2)
+    want = lsm_lib._native_lsm(torch.exp(lg), list(range(t)),
+                               (kind, np.array([k])), 2, df, n // 2,
+                               torch.float64, dev, basis_log=1)
+    np.testing.assert_allclose(_np(got), _np(want), rtol=1e-10)
+    # and the oracle on the same rows (regression on the first half)
+    def payoff(x, ti, kind=kind, k=k):
+      s = np.exp(x[:, ti, :])
+      v = (k - s.mean(-1) if kind == lsm_lib.PAYOFF_BASKET_PUT
+           else s.max(-1) - k)
+      return np.maximum(v, 0)[:, None]
+    ref = olsm.least_square_mc(log_s, list(range(t)), payoff,
+                               olsm.make_polynomial_basis(2, True),
+                               discount_factors=dfs,
+                               num_calibration_samples=n // 2)[0]
+    np.testing.assert_allclose(float(_np(got)[0]), ref, rtol=2e-6)

@@ -48,7 +48,10 @@ struct LsmArgs {
   const uint8_t* expo;      // [K][d] exponents
   int K;
   int deg;
-  int basis_log;            // basis built on ln(x) (payoff still on x)
+  int basis_log;            // 0: basis on the stored values x; 1: on
+                            // ln x (payoff on x); 2: the stored values
+                            // are ln x (basis on them, payoff on exp)
+  double log_strike;        // ln(strike) (-inf for strike <= 0)
   const double* table;      // TQF_PAYOFF_TABLE: [n_times][n_paths]
   // per-path discount rows of the current date (numerator e + 1 and
   // denominator e of the padded factors), or null for the scalar ratio
@@ -74,24 +77,40 @@ __device__ __forceinline__ double path_x(const LsmArgs& a, int64_t p, int j) {
 // ln S: tff/experimental/pricing_platform/framework/equity_instruments/
 // american_option/utils.py:150-167).
 __device__ __forceinline__ double basis_input(const LsmArgs& a, double x) {
-  return a.basis_log ? log(x) : x;
+  return a.basis_log == 1 ? log(x) : x;
 }
 
+// The payoff of the stored values x[0 .. d) of path p (exp of them when
+// the paths hold log prices).
 __device__ __forceinline__ double payoff_value(const LsmArgs& a,
                                                const double* x, int64_t p) {
-  if (a.payoff == TQF_PAYOFF_TABLE) {
+  if (a.payoff == TQF_PAYOFF_TABLE)
     return a.table[(int64_t)a.t_idx * a.n_paths + p];
-  } else if (a.payoff == TQF_PAYOFF_BASKET_PUT) {
+  auto value = [&](int j) { return a.basis_log == 2 ? exp(x[j]) : x[j]; };
+  if (a.payoff == TQF_PAYOFF_BASKET_PUT) {
     double s = 0;
-    for (int j = 0; j < a.d; ++j) s += x[j];
+    for (int j = 0; j < a.d; ++j) s += value(j);
     const double v = a.strike - s / a.d;
     return v > 0 ? v : 0;
   } else {  // TQF_PAYOFF_MAX_CALL
-    double m = x[0];
-    for (int j = 1; j < a.d; ++j) m = x[j] > m ? x[j] : m;
+    double m = value(0);
+    for (int j = 1; j < a.d; ++j) {
+      const double v = value(j);
+      m = v > m ? v : m;
+    }
     const double v = m - a.strike;
     return v > 0 ? v : 0;
   }
+}
+
+// payoff_value(a, x, p) > 0.  One asset held as a log price: the put is in
+// the money iff ln S < ln K, the call iff ln S > ln K, with no exp.
+__device__ __forceinline__ bool in_the_money(const LsmArgs& a,
+                                             const double* x, int64_t p) {
+  if (a.basis_log == 2 && a.d == 1 && a.payoff != TQF_PAYOFF_TABLE)
+    return a.payoff == TQF_PAYOFF_BASKET_PUT ? x[0] < a.log_strike
+                                             : x[0] > a.log_strike;
+  return payoff_value(a, x, p) > 0;
 }
 
 // Exercise values, continuation targets and the per-block asset moments
@@ -1119,6 +1138,283 @@ lsm_update_cont_kernel(int64_t n, double strike, double df_ratio,
   }
 }
 
+// ---------------------------------------------------------------------------
+// One-asset date chain, two launches per exercise date.  With one asset and
+// the full polynomial basis the regression needs only the power sums
+// c^0 .. c^(2 deg) and y c^0 .. y c^deg of the in-the-money calibration rows
+// (the moment plan's small case), and the dense update evaluates the
+// continuation by Horner in c.  So one pass over the paths (pass r)
+//   - updates the cashflows of the date regressed before (date r + 1),
+//   - accumulates the power sums of date r from the updated cashflows, and
+//   - sums the basis input of date r - 1 and its square (its centring),
+// and one workgroup (solve r) reduces the pass's per-block rows, builds the
+// (K + 1)^2 extended Gram from the sums, solves it with the column-scaled,
+// truncating Cholesky of lsm_solve_kernel (serially: K <= 13) and writes
+// date r's dense coefficients and rank and date r - 1's centring.  The
+// general chain takes eight launches per date for the same work.
+
+constexpr int kSmallMaxDeg = 4;
+constexpr int kSmallMaxK = 2 * kSmallMaxDeg + 1;  // exponent rows (K)
+constexpr int kSmallMaxRows = 2048;  // workgroups (partial rows) per pass
+constexpr int kSmallRows = 4;        // rows per thread and round of a pass
+
+// One-asset payoff of the stored value x (exp of it for log prices) and
+// the in-the-money test (ln S against ln K for log prices, no exp).
+__device__ __forceinline__ double payoff1(const LsmArgs& a, double x,
+                                          int64_t p) {
+  if (a.payoff == TQF_PAYOFF_TABLE)
+    return a.table[(int64_t)a.t_idx * a.n_paths + p];
+  const double s = a.basis_log == 2 ? exp(x) : x;
+  const double v = a.payoff == TQF_PAYOFF_BASKET_PUT ? a.strike - s
+                                                     : s - a.strike;
+  return v > 0 ? v : 0;
+}
+__device__ __forceinline__ bool itm1(const LsmArgs& a, double x, int64_t p) {
+  if (a.basis_log == 2 && a.payoff != TQF_PAYOFF_TABLE)
+    return a.payoff == TQF_PAYOFF_BASKET_PUT ? x < a.log_strike
+                                             : x > a.log_strike;
+  return payoff1(a, x, p) > 0;
+}
+
+// The per-date values one pass reads: the date's column of the paths
+// (value of path p at col[p * sp]), its centring and discount rows.
+struct SmallDate {
+  const double* col;
+  int t_idx;            // time index (the table payoff's row)
+  double ratio;         // scalar discount ratio (when dnum is null)
+  const double* dnum;
+  const double* dden;
+};
+
+__device__ __forceinline__ double small_ratio(const SmallDate& d,
+                                              int64_t p) {
+  return d.dnum ? d.dnum[p] / d.dden[p] : d.ratio;
+}
+
+// Partial row of workgroup b: [0, NX) power sums c^m, [NX, NM) y c^m, then
+// the sum and the sum of squares of date r - 1's basis input.  do_update
+// = 0 skips the update (the first regressed date follows the terminal
+// payoff); n_calib = 0 skips the power sums; do_stats = 0 the centring
+// sums.  The centrings of dates r + 1 and r come from device slots
+// (mean_u / mean_m, written by the solves).
+template <int DEG>
+__global__ void __launch_bounds__(256)
+lsm_small_pass_kernel(LsmArgs a, SmallDate du, const double* __restrict__ cu,
+                      const double* __restrict__ dense_u, int do_update,
+                      SmallDate dm, const double* __restrict__ cm,
+                      int64_t n_calib, SmallDate ds, int do_stats,
+                      int64_t rows_per_block, double* __restrict__ cash,
+                      double* __restrict__ vals, double* __restrict__ partial) {
+  constexpr int NX = 2 * DEG + 1, NM = NX + DEG + 1, W = NM + 2;
+  __shared__ double red[4][W];
+  double acc[W];
+#pragma unroll
+  for (int m = 0; m < W; ++m) acc[m] = 0.0;
+  const int64_t n = a.n_paths, sp = a.sp;
+  const double mu = do_update ? cu[0] : 0.0, su = do_update ? cu[kMaxAssets] : 0.0;
+  const double mm = cm[0], sm = cm[kMaxAssets];
+  double dn[DEG + 1];
+#pragma unroll
+  for (int m = 0; m <= DEG; ++m) dn[m] = do_update ? dense_u[m] : 0.0;
+  LsmArgs au = a, am = a;
+  au.t_idx = du.t_idx;
+  am.t_idx = dm.t_idx;
+  const int64_t p0 = (int64_t)blockIdx.x * rows_per_block;
+  // kSmallRows rows per thread and round, every load of a round issued
+  // before its arithmetic
+  for (int64_t base = 0; base < rows_per_block; base += 256 * kSmallRows) {
+    int64_t p[kSmallRows];
+    double cs[kSmallRows], vl[kSmallRows], xu[kSmallRows], xm[kSmallRows],
+        xs[kSmallRows];
+#pragma unroll
+    for (int i = 0; i < kSmallRows; ++i) {
+      p[i] = p0 + base + 256 * i + threadIdx.x;
+      const bool ok = base + 256 * i + threadIdx.x < rows_per_block &&
+                      p[i] < n;
+      if (!ok) p[i] = -1;
+      cs[i] = ok ? cash[p[i]] : 0.0;
+      vl[i] = ok ? vals[p[i]] : 0.0;
+      xu[i] = ok && do_update ? du.col[p[i] * sp] : 1.0;
+      xm[i] = ok && p[i] < n_calib ? dm.col[p[i] * sp] : 1.0;
+      xs[i] = ok && do_stats ? ds.col[p[i] * sp] : 1.0;
+    }
+#pragma unroll
+    for (int i = 0; i < kSmallRows; ++i) {
+      if (p[i] < 0) continue;
+      if (do_update) {
+        const double ex = itm1(au, xu[i], p[i]) ? payoff1(au, xu[i], p[i])
+                                                : 0.0;
+        const double c = (basis_input(a, xu[i]) - mu) * su;
+        double h = dn[DEG];
+#pragma unroll
+        for (int m = DEG - 1; m >= 0; --m) h = fma(h, c, dn[m]);
+        const double cont = h > 0 ? h : 0;
+        if (ex > cont) {
+          vl[i] = 0;
+          cs[i] = ex;
+        } else {
+          vl[i] = small_ratio(du, p[i]) * (cs[i] + vl[i]);
+          cs[i] = 0;
+        }
+        cash[p[i]] = cs[i];
+        vals[p[i]] = vl[i];
+      }
+      if (p[i] < n_calib && itm1(am, xm[i], p[i])) {
+        // in-the-money calibration rows only
+        const double c = (basis_input(a, xm[i]) - mm) * sm;
+        double v = 1.0, u = small_ratio(dm, p[i]) * (cs[i] + vl[i]);
+#pragma unroll
+        for (int m = 0; m < NX; ++m) {
+          acc[m] += v;
+          v *= c;
+        }
+#pragma unroll
+        for (int m = NX; m < NM; ++m) {
+          acc[m] += u;
+          u *= c;
+        }
+      }
+      if (do_stats) {
+        const double b = basis_input(a, xs[i]);
+        acc[NM] += b;
+        acc[NM + 1] += b * b;
+      }
+    }
+  }
+  // fixed-order workgroup sums: wave butterflies, then the waves in order
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < W; ++m) {
+    double v = acc[m];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave][m] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < W)
+    partial[(int64_t)blockIdx.x * W + threadIdx.x] =
+        ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
+        red[3][threadIdx.x];
+}
+
+// Solve r: column sums of the pass's n_rows partial rows (thread t adds
+// rows t, t + 256, ...; then wave butterflies and the four wave sums in
+// wave order), then, on one thread, the extended Gram
+// G[k][l] = sum c^(e_k + e_l), G[k][K] = sum y c^e_k and the steps of
+// lsm_solve_kernel: column scaling by 1 / sqrt(G_kk), pivots at or below
+// rtol * max scaled diagonal dropped, L^T w = z, beta = D w.  dense[m] is
+// the sum of the beta_k with exponent m, in basis order.  do_solve = 0 only
+// reduces the centring sums (into mean_next[0] and mean_next[kMaxAssets]).
+template <int DEG>
+__global__ void __launch_bounds__(256)
+lsm_small_solve_kernel(const double* __restrict__ partial, int n_rows,
+                       int do_solve, int do_stats, int64_t n_paths,
+                       const uint8_t* __restrict__ expo_g, int K, double rtol,
+                       double* __restrict__ dense, int* __restrict__ rank_out,
+                       double* __restrict__ mean_next) {
+  constexpr int NX = 2 * DEG + 1, NM = NX + DEG + 1, W = NM + 2;
+  __shared__ double red[4][W];
+  __shared__ double A[(kSmallMaxK + 1) * (kSmallMaxK + 1)];
+  __shared__ int expo[kSmallMaxK];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (t < K) expo[t] = expo_g[t];
+  double acc[W];
+#pragma unroll
+  for (int c = 0; c < W; ++c) acc[c] = 0.0;
+  // rows t + 256 i, eight per round with every load of a round issued
+  // before its sums (the rows are added in increasing order)
+  constexpr int kR = 8;
+  for (int r0 = t; r0 < n_rows; r0 += 256 * kR) {
+    double v[kR][W];
+#pragma unroll
+    for (int i = 0; i < kR; ++i) {
+      const int r = r0 + 256 * i;
+#pragma unroll
+      for (int c = 0; c < W; ++c)
+        v[i][c] = r < n_rows ? partial[(int64_t)r * W + c] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < kR; ++i)
+#pragma unroll
+      for (int c = 0; c < W; ++c) acc[c] += v[i][c];
+  }
+#pragma unroll
+  for (int c = 0; c < W; ++c) {
+    double v = acc[c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wave][c] = v;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  double M[W];
+#pragma unroll
+  for (int c = 0; c < W; ++c)
+    M[c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+  if (do_stats) {
+    const double nn = (double)n_paths;
+    const double m = M[NM] / nn;
+    const double var = M[NM + 1] / nn - m * m;
+    mean_next[0] = m;
+    mean_next[kMaxAssets] =
+        var > 1e-300 && sqrt(var) > 1e-12 * fabs(m) ? 1.0 / sqrt(var) : 1.0;
+  }
+  if (!do_solve) return;
+  // the sums into LDS (indexed by exponent sums below)
+  double* Ms = &red[0][0];
+  for (int c = 0; c < W; ++c) Ms[c] = M[c];
+  const int n1 = K + 1;
+  double dsc[kSmallMaxK + 1];
+  double maxdiag = 0.0;
+  for (int k = 0; k < K; ++k) {
+    const double gkk = Ms[2 * expo[k]];
+    dsc[k] = gkk > 0 ? 1.0 / sqrt(gkk) : 0.0;
+    maxdiag = fmax(maxdiag, gkk * (dsc[k] * dsc[k]));
+  }
+  dsc[K] = 1.0;
+  // scaled lower triangle of the extended matrix (row K: the rhs sums)
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double gij = 0.0;
+      if (i < K) gij = Ms[expo[i] + expo[j]];
+      else if (j < K) gij = Ms[NX + expo[j]];
+      A[i * n1 + j] = gij * (dsc[i] * dsc[j]);
+    }
+  const double thr = rtol * maxdiag;
+  int rank = 0;
+  for (int k = 0; k < K; ++k) {
+    const double d = A[k * n1 + k];
+    const bool keep = d > thr && d > 0;
+    const double root = keep ? sqrt(d) : 0.0;  // truncated: column dropped
+    rank += keep ? 1 : 0;
+    if (keep) {
+      const double inv = 1.0 / d;
+      for (int j = k + 1; j < n1; ++j) {
+        const double cj = A[j * n1 + k];
+        for (int i = j; i < n1; ++i) A[i * n1 + j] -= A[i * n1 + k] * cj * inv;
+      }
+    }
+    A[k * n1 + k] = root;
+    for (int i = k + 1; i < n1; ++i)
+      A[i * n1 + k] = root > 0 ? A[i * n1 + k] / root : 0.0;
+  }
+  if (rank_out) *rank_out = rank;
+  double r[kSmallMaxK], beta[kSmallMaxK];
+  for (int j = 0; j < K; ++j) r[j] = A[K * n1 + j];
+  for (int i = K - 1; i >= 0; --i) {
+    const double li = A[i * n1 + i];
+    const double w = li > 0 ? r[i] / li : 0.0;
+    beta[i] = w * dsc[i];
+    for (int j = 0; j < i; ++j) r[j] -= A[i * n1 + j] * w;
+  }
+  for (int m = 0; m <= DEG; ++m) {
+    double v = 0.0;
+    for (int k = 0; k < K; ++k)
+      if (expo[k] == m) v += beta[k];
+    dense[m] = v;
+  }
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 LsmArgs make_args(const tqf_lsm* l, int t_idx) {
@@ -1141,6 +1437,7 @@ LsmArgs make_args(const tqf_lsm* l, int t_idx) {
   a.K = l->n_basis;
   a.deg = l->degree;
   a.basis_log = l->basis_log;
+  a.log_strike = l->strike > 0 ? log(l->strike) : -HUGE_VAL;
   a.table = l->payoff_table;
   a.dnum = nullptr;
   a.dden = nullptr;
@@ -1163,6 +1460,8 @@ int validate(const tqf_lsm* l) {
   TQF_REQUIRE(l->payoff != TQF_PAYOFF_TABLE || l->payoff_table,
               "the table payoff needs payoff_table");
   TQF_REQUIRE(l->paths && l->exponents, "null LSM arrays");
+  TQF_REQUIRE(l->basis_log >= 0 && l->basis_log <= 2,
+              "basis_log must be 0, 1 or 2, got %d", l->basis_log);
   return TQF_OK;
 }
 
@@ -1181,7 +1480,69 @@ extern "C" int64_t tqf_lsm_workspace_size(int64_t n_paths, int32_t n_basis) {
   bytes += kext * kext * 8;                     // G
   bytes += 4 * kext * 8 + 64;                   // beta, rank (with slack)
   bytes += pairs * 8 + 4 * kMaxAssets * 8;      // pairs, mean, scale
+  bytes += 1024;                                // one-asset chain state
   return bytes + 4096;
+}
+
+// The one-asset chain of lsm_small_pass_kernel / lsm_small_solve_kernel
+// for the regressed dates r = n_ex - 2 .. 0 (exercise index ex_idx[r],
+// discount ratio df[r + 2] / df[r + 1]), after the terminal payoff of date
+// n_ex - 1 is in cash / vals.  slots holds three rotating centrings (mean
+// at [0], scale at [kMaxAssets]; date r uses slot r % 3); dense the
+// coefficients of the date solved last.
+template <int DEG>
+int small_date_chain(const tqf_lsm* l, const int32_t* ex_idx, int n_ex,
+                     const double* df, int64_t n_calib, double* partial,
+                     double* slots, double* dense, double* cash,
+                     double* vals, int32_t* ranks, hipStream_t s) {
+  const int64_t n = l->n_paths;
+  const LsmArgs a = make_args(l, 0);
+  // rows per workgroup: a multiple of one round (256 kSmallRows rows)
+  constexpr int64_t kRound = 256 * kSmallRows;
+  int64_t rpb = (n + kSmallMaxRows - 1) / kSmallMaxRows;
+  rpb = (rpb + kRound - 1) / kRound * kRound;
+  const unsigned grid = (unsigned)((n + rpb - 1) / rpb);
+  auto date = [&](int r) {
+    SmallDate d{};
+    d.col = a.paths + (int64_t)ex_idx[r] * a.st;
+    d.t_idx = ex_idx[r];
+    d.ratio = df[r + 2] / df[r + 1];
+    if (l->path_discounts) {
+      d.dnum = l->path_discounts + (int64_t)(r + 2) * n;
+      d.dden = l->path_discounts + (int64_t)(r + 1) * n;
+    }
+    return d;
+  };
+  auto slot = [&](int r) { return slots + (r % 3) * 2 * kMaxAssets; };
+  auto pass = [&](int r_upd, int r_mom, int r_stat) {
+    const int r_any = r_upd >= 0 ? r_upd : (r_mom >= 0 ? r_mom : 0);
+    const SmallDate du = date(r_upd >= 0 ? r_upd : r_any);
+    const SmallDate dm = date(r_mom >= 0 ? r_mom : r_any);
+    const SmallDate dst = date(r_stat >= 0 ? r_stat : r_any);
+    hipLaunchKernelGGL(lsm_small_pass_kernel<DEG>, grid, 256, 0, s, a, du,
+                       slot(r_upd >= 0 ? r_upd : r_any), dense,
+                       r_upd >= 0 ? 1 : 0, dm,
+                       slot(r_mom >= 0 ? r_mom : r_any),
+                       r_mom >= 0 ? n_calib : (int64_t)0, dst,
+                       r_stat >= 0 ? 1 : 0, rpb, cash, vals, partial);
+  };
+  auto solve = [&](int r, int r_stat) {
+    hipLaunchKernelGGL(lsm_small_solve_kernel<DEG>, 1, 256, 0, s, partial,
+                       (int)grid, r >= 0 ? 1 : 0, r_stat >= 0 ? 1 : 0, n,
+                       l->exponents, l->n_basis, l->rcond, dense,
+                       (ranks && r >= 0) ? ranks + r : nullptr,
+                       slot(r_stat >= 0 ? r_stat : 0));
+  };
+  // centring of the first regressed date
+  const int r0 = n_ex - 2;
+  pass(-1, -1, r0);
+  solve(-1, r0);
+  for (int r = r0; r >= 0; --r) {
+    pass(r < r0 ? r + 1 : -1, r, r >= 1 ? r - 1 : -1);
+    solve(r, r >= 1 ? r - 1 : -1);
+  }
+  pass(0, -1, -1);  // the exercise decision of the earliest date
+  return check_launch("tqf_lsm_price (one-asset chain)");
 }
 
 extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
@@ -1228,6 +1589,8 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
   int2* pairs = (int2*)take(n_pairs * 8);
   double* mean = (double*)take(kMaxAssets * 8);
   double* scale = (double*)take(kMaxAssets * 8);
+  // one-asset chain: three (mean, scale) slots and the dense coefficients
+  double* chain = (double*)take((6 * kMaxAssets + kSmallMaxDeg + 1) * 8);
 
   // pair table (host -> device, tiny)
   int2 hp[64];
@@ -1269,7 +1632,29 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
   LsmArgs a = make_args(&l2, ex_idx[n_ex - 1]);
   hipLaunchKernelGGL(lsm_terminal_kernel, blocks_for(n), 256, 0, s, a, cash,
                      vals);
-  for (int e = n_ex - 1; e > 0; --e) {
+  // one asset with the full polynomial basis: the two-launch date chain
+  const bool small_chain = l->dim == 1 && use_moments && mp.small &&
+                           ndense == l->degree + 1 &&
+                           l->degree <= kSmallMaxDeg && K <= kSmallMaxK &&
+                           n_ex >= 2;
+  if (small_chain) {
+    double* dense = chain + 6 * kMaxAssets;
+    switch (l->degree) {
+#define TQF_SMALL_CHAIN(DEG)                                               \
+  case DEG:                                                                \
+    rc = small_date_chain<DEG>(l, ex_idx, n_ex, df, n_calib, partial,      \
+                               chain, dense, cash, vals, ranks, s);        \
+    break;
+      TQF_SMALL_CHAIN(0)
+      TQF_SMALL_CHAIN(1)
+      TQF_SMALL_CHAIN(2)
+      TQF_SMALL_CHAIN(3)
+      TQF_SMALL_CHAIN(4)
+#undef TQF_SMALL_CHAIN
+    }
+    if (rc) return rc;
+  }
+  for (int e = small_chain ? 0 : n_ex - 1; e > 0; --e) {
     a = make_args(&l2, ex_idx[e - 1]);
     a.mean = mean;
     a.scale = scale;

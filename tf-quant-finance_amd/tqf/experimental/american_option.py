@@ -12,9 +12,9 @@ degree-2 polynomial basis on ln S and the call/put payoff on S
 
 Here the unit paths come from the exact GBM kernel (tqf_gbm_exact) with
 the engine's antithetic Philox stream (the reference's
-STATELESS_ANTITHETIC), the per-option mapping is one elementwise pass on
-the GPU, and the regression runs in the FP64 LSM kernels (tqf_lsm_price,
-basis on ln S).  A user basis_fn falls back to the torch LSM with the same
+STATELESS_ANTITHETIC), the per-option mapping to ln(S / S0) is one
+elementwise pass on the GPU, and the regression runs in the FP64 LSM
+kernels (tqf_lsm_price on log prices: basis on ln S, payoff on S).  A user basis_fn falls back to the torch LSM with the same
 log-path / payoff split.
 """
 import numpy as np
@@ -74,15 +74,17 @@ def bs_lsm_price(spots, expiry_times, strikes, volatility, discount_factors,
     dfs = np.exp(-rate[b] * times * expiry[b])
     kind = lsm_lib.PAYOFF_MAX_CALL if call[b] else lsm_lib.PAYOFF_BASKET_PUT
     if bspec is not None and bspec[0] == 'poly':
-      log_s_t = ((vol[b] * np.sqrt(expiry[b])) * w_t +
-                 (rate[b] - 0.5 * vol[b]**2) * (t_t * expiry[b]))
+      # ln(S / S0) in one pass: the LSM kernels take log prices
+      # (basis on ln S, payoff on exp)
+      log_s_t = torch.add((rate[b] - 0.5 * vol[b]**2) * (t_t * expiry[b]),
+                          w_t, alpha=float(vol[b] * np.sqrt(expiry[b])))
       # [N, T, 1] view of [T][1][N] memory
-      paths = torch.exp(log_s_t).unsqueeze(1).permute(2, 0, 1)
+      paths = log_s_t.unsqueeze(1).permute(2, 0, 1)
       p = lsm_lib._native_lsm(
           paths, ex, (kind, np.array([strike[b]])), bspec[1],
           np.concatenate([[1.0], dfs]).reshape(-1, 1, 1),
           num_calibration_samples,
-          torch.float64, dev, basis_log=True)
+          torch.float64, dev, basis_log=2)
       prices.append(float(p[0]))
     else:
       log_s = ((vol[b] * np.sqrt(expiry[b])) * w +

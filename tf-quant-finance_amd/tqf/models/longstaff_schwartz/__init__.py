@@ -177,7 +177,10 @@ def step_major_f64(paths):
 def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
                 return_details=False, basis_log=False, exponents=None,
                 path_values=None):
-  """exponents: an optional [K, d] table of monomial exponents (each at
+  """basis_log: False / 0 regresses on the path values, True / 1 on their
+  logarithms, 2 takes paths that hold log prices (basis on them, payoff on
+  their exp).
+  exponents: an optional [K, d] table of monomial exponents (each at
   most `degree`) in place of the tensor grid of make_polynomial_basis.
   path_values: an optional float64 device tensor [N] that receives, for the
   last payoff batch entry, every path's value at the first exercise date
@@ -185,6 +188,8 @@ def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
   df[0]): the per-path terms whose mean over the pricing rows is the
   price."""
   kind, strikes = pspec
+  if int(basis_log) not in (0, 1, 2):
+    raise ValueError('basis_log must be 0, 1 or 2')
   paths_b = x.unsqueeze(0) if x.dim() == 3 else x  # [Bp, N, T, d]
   bp = paths_b.shape[0]
   table = None
@@ -224,7 +229,7 @@ def _native_lsm(x, ex, pspec, degree, df, num_calib, dtype, dev,
     l.payoff = kind
     l.degree = degree
     l.n_basis = k
-    l.basis_log = int(bool(basis_log))
+    l.basis_log = int(basis_log)
     l.exponents = expo_t.data_ptr()
     l.strike = float(strikes[b if strikes.shape[0] > 1 else 0])
     l.num_calibration = int(num_calib or 0)
