@@ -718,44 +718,74 @@ bool moment_plan(int d, int deg, int n_pairs, MomentPlan* out) {
 }
 
 // Column-scaled Cholesky with pivot truncation and the two triangular
-// solves, one workgroup of 1024 threads, float64.  beta[K] out.
+// solves, one workgroup of 512 threads, float64.  beta[K] out.
 //
 // The factorization runs on the extended matrix [[G, b], [b^T, .]] of order
 // n = K + 1 <= 256 (b = X^T W y is column K of the Gram buffer): its last
 // row of L is z = L^-1 b, so the forward solve comes out of the
-// factorization.  Thread (ti, tj) of a 32 x 32 grid keeps the lower-triangle
-// entries (32a + ti, 32b + tj), b <= a < 8, of the scaled matrix in
-// registers (36 slots) for the whole factorization.  Column k is published
-// through LDS before its phase (double-buffered, one barrier per column);
-// in the phase the first n threads write the finished column of L to the
-// Gram buffer, and all update the entries right of it.  The back solve
-// L^T w = z runs on one wave from the Gram buffer.
-constexpr int kSolveThreads = 1024;
+// factorization.  Columns are scaled by 1 / sqrt(G_kk); a pivot at or below
+// rtol * (largest scaled diagonal) drops its column (L column zero), which
+// leaves the fitted values of the pseudo-inverse solution unchanged.
+//
+// Blocked right-looking form on 16 x 16 tiles (T = ceil(n / 16) tile rows):
+// the lower tiles stay in registers for the whole solve, in the float64
+// MFMA accumulator layout (lane l: column l % 16, rows l / 16 + 4 r); wave
+// w of eight keeps tile rows 15 - w and w (17 tiles, two waves per SIMD so
+// they fit its registers).  Per panel p:
+//   1. the owner of tile (p, p) factors it in LDS (16 sequential columns
+//      on one wave, no workgroup barrier) and forms Y = L_pp^-1 with zero
+//      rows and columns for dropped pivots;
+//   2. each panel tile (a, p) becomes X = A_ap Y^T (4 MFMAs), staged in LDS;
+//   3. each trailing tile (a, b), a >= b > p, takes -= X_ap X_bp^T (4 MFMAs).
+// Two workgroup barriers per panel in place of one per column.  The back
+// solve L^T w = z runs by blocks from the last: w_b = Y_bb^T r_b, then every
+// tile (b, c), c < b, subtracts L_bc^T w_b from r_c (one tile per c and
+// step, so the order of the sums is fixed).  beta = D w.
+constexpr int kSolveWaves = 8;
+constexpr int kSolveThreads = 64 * kSolveWaves;
+constexpr int kBT = 16;         // tile edge
+constexpr int kBLd = 17;        // LDS row stride of a staged tile (doubles)
+constexpr int kBTiles = 17;     // two tile rows per wave: 136 / 8
 
-__device__ __forceinline__ constexpr int solve_slot(int a, int b) {
-  return a > b ? a * (a - 1) / 2 + b : 28 + a;
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
 __global__ void __launch_bounds__(kSolveThreads)
-lsm_solve_kernel(double* __restrict__ g, int K, double rtol,
+lsm_solve_kernel(const double* __restrict__ g, int K, double rtol,
                  double* __restrict__ beta, int* __restrict__ rank_out) {
+  __shared__ double dsc[256];
+  __shared__ double D[kBT * kBLd];           // diagonal tile being factored
+  __shared__ double Ys[16][kBT * kBLd];      // Y_pp of every panel
+  __shared__ double P[16][kBT * kBLd];       // panel tiles X_ap
+  __shared__ double rv[256], wv[256];
+  __shared__ double rinvs[kBT];             // 1 / L_kk of the tile (0: dropped)
+  __shared__ double maxdiag_s;
+  __shared__ int rank_s;
   const int n = K + 1, kext = K + 1;
-  __shared__ double dsc[256];      // column scales (1 for the rhs row)
-  __shared__ double colb[2][256];  // column k of the working matrix
-  __shared__ double maxdiag;
-  const int t = threadIdx.x, ti = t >> 5, tj = t & 31;
+  const int T = (n + kBT - 1) / kBT;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int cl = lane & 15, rl = lane >> 4;
   if (t < 256) {
-    double s = 0.0;
+    double sc = 0.0;
     if (t < K) {
       const double d = g[t * kext + t];
-      s = d > 0 ? 1.0 / sqrt(d) : 0.0;
+      sc = d > 0 ? 1.0 / sqrt(d) : 0.0;
     } else if (t == K) {
-      s = 1.0;
+      sc = 1.0;
     }
-    dsc[t] = s;
-    colb[0][t] = 0.0;
-    colb[1][t] = 0.0;
+    dsc[t] = sc;
+    rv[t] = 0.0;
+    wv[t] = 0.0;
   }
+  if (t == 0) rank_s = 0;
   __syncthreads();
   // the largest scaled diagonal entry (wave 0)
   if (t < 64) {
@@ -767,109 +797,232 @@ lsm_solve_kernel(double* __restrict__ g, int K, double rtol,
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-    if (t == 0) maxdiag = m;
+    if (t == 0) maxdiag_s = m;
   }
-  double v[36];
+  // this wave's tiles q: (a, b) = (tile_a(q), tile_b(q)), a = -1 past the
+  // last tile (wave-uniform scalars), scaled
+  // wave w owns tile rows 15 - w (slots 0 .. 15 - w) and w (slots
+  // 16 - w .. 16): 17 tiles each; rows at or past T hold nothing
+  // (wo is the wave index passed through an empty asm statement inside the
+  // loops, so the 34 tile coordinates and their LDS addresses are rebuilt
+  // where used rather than hoisted out of the panel loop, which would
+  // exhaust the registers)
+  int wo = wave;
+  auto tile_a = [&](int q) {
+    const int a = q < 16 - wo ? 15 - wo : wo;
+    return a < T ? a : -1;
+  };
+  auto tile_b = [&](int q) { return q < 16 - wo ? q : q - 16 + wo; };
+  auto refresh = [&]() {
+    wo = wave;
+    asm volatile("" : "+s"(wo));
+  };
+  f64x4 tile[kBTiles];
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
+  for (int q = 0; q < kBTiles; ++q) {
+    const int a = tile_a(q), b = tile_b(q);
+    f64x4 v = {0.0, 0.0, 0.0, 0.0};
+    if (a >= 0) {
 #pragma unroll
-    for (int b = 0; b <= a; ++b) {
-      const int i = 32 * a + ti, j = 32 * b + tj;
-      v[solve_slot(a, b)] = (i < n && j <= i)
-          ? g[i * kext + j] * (dsc[i] * dsc[j]) : 0.0;
-    }
-  if (tj == 0) {
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      const int i = 32 * a + ti;
-      if (i < n) colb[0][i] = v[solve_slot(a, 0)];
-    }
-  }
-  __syncthreads();
-  const double thr = rtol * maxdiag;
-  int rank = 0;
-  for (int k = 0; k < K; ++k) {
-    const double* cb = colb[k & 1];
-    const double d = cb[k];
-    const bool keep = d > thr && d > 0;
-    const double root = keep ? sqrt(d) : 0.0;  // truncated: column dropped
-    rank += keep ? 1 : 0;
-    // column k of L (rows < K) and of z (row K): the pivot's root on the
-    // diagonal, the entries below divided by it
-    if (t >= k && t < n)
-      g[t * kext + k] = t == k ? root : (root > 0 ? cb[t] / root : 0.0);
-    // trailing update of the entries right of column k
-    if (keep) {
-      const double inv = 1.0 / d;
-      double cj[8];
-#pragma unroll
-      for (int b = 0; b < 8; ++b) cj[b] = cb[32 * b + tj];
-#pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        if (32 * a + 31 <= k) continue;
-        const double ci = cb[32 * a + ti];
-#pragma unroll
-        for (int b = 0; b <= a; ++b) {
-          if (32 * b + 31 <= k) continue;
-          if (32 * b + tj > k) v[solve_slot(a, b)] -= ci * cj[b] * inv;
-        }
+      for (int r = 0; r < 4; ++r) {
+        const int row = kBT * a + rl + 4 * r, col = kBT * b + cl;
+        v[r] = (row < n && col < n)
+                   ? g[row * kext + col] * (dsc[row] * dsc[col]) : 0.0;
       }
     }
-    // publish column k + 1 (its last update was the one above)
-    const int k1 = k + 1;
-    if (k1 < K && tj == (k1 & 31)) {
-      const int kb1 = k1 >> 5;
+    tile[q] = v;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  __syncthreads();
+  const double thr = rtol * maxdiag_s;
+  // 1. factor diagonal tile (p, p) on its owner (wave p or 15 - p, slot 16
+  // or p), with Y_pp = L_pp^-1 (zero rows and columns where a pivot is
+  // dropped).  Column k: the pivot's reciprocal root from v_rsq_f64 and two
+  // Newton steps, L_ik = D_ik rinv (i > k) and the rank-one update
+  // D_ij -= L_ik L_jk (k < j <= i), in one LDS round per column (lane l
+  // holds entries l, l + 64, ... of the 16 x 16 square).
+  auto factor_diag = [&](int pd) {
+    const int qd = pd < 8 ? 16 : pd;
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        if (b != kb1) continue;
+    for (int q = 0; q < kBTiles; ++q)
+      if (q == qd)
 #pragma unroll
-        for (int a = b; a < 8; ++a) {
-          const int i = 32 * a + ti;
-          if (i >= k1 && i < n) colb[k1 & 1][i] = v[solve_slot(a, b)];
+        for (int r = 0; r < 4; ++r) D[(rl + 4 * r) * kBLd + cl] = tile[q][r];
+    wave_lds_sync();
+    int kept = 0;
+    for (int k = 0; k < kBT; ++k) {
+      const double dk = D[k * kBLd + k];
+      double dik[4], djk[4], dij[4];
+#pragma unroll
+      for (int e4 = 0; e4 < 4; ++e4) {
+        const int e = 64 * e4 + lane, i = e >> 4, j = e & 15;
+        dik[e4] = D[i * kBLd + k];
+        djk[e4] = D[j * kBLd + k];
+        dij[e4] = D[i * kBLd + j];
+      }
+      const bool keep = kBT * pd + k < K && dk > thr && dk > 0;
+      double rinv = 0.0;
+      if (keep) {
+        double y = __builtin_amdgcn_rsq(dk);
+        y = y * fma(-0.5 * dk * y, y, 1.5);
+        y = y * fma(-0.5 * dk * y, y, 1.5);
+        rinv = y;
+      }
+      const double root = dk * rinv;  // 0 for a dropped pivot
+      kept += keep ? 1 : 0;
+      wave_lds_sync();
+      // every entry written back, changed or not (no branches; rinv = 0
+      // leaves the update a no-op and zeroes the column)
+#pragma unroll
+      for (int e4 = 0; e4 < 4; ++e4) {
+        const int e = 64 * e4 + lane, i = e >> 4, j = e & 15;
+        const double lik = dik[e4] * rinv, ljk = djk[e4] * rinv;
+        double v = dij[e4];
+        v = (j > k && i >= j) ? v - lik * ljk : v;
+        v = (j == k && i > k) ? lik : v;
+        v = (j == k && i == k) ? root : v;
+        D[i * kBLd + j] = v;
+      }
+      rinvs[k] = rinv;
+      wave_lds_sync();
+    }
+    // Y column `lane`: y_i = (delta_i,lane - sum_m<i L_im y_m) / L_ii
+    if (lane < kBT) {
+      double y[kBT];
+#pragma unroll
+      for (int i = 0; i < kBT; ++i) {
+        double acc = i == lane ? 1.0 : 0.0;
+#pragma unroll
+        for (int m = 0; m < i; ++m) acc -= D[i * kBLd + m] * y[m];
+        y[i] = acc * rinvs[i];
+      }
+#pragma unroll
+      for (int i = 0; i < kBT; ++i) Ys[pd][i * kBLd + lane] = y[i];
+    }
+    // L_pp (lower, zero above) back into the register tile
+#pragma unroll
+    for (int q = 0; q < kBTiles; ++q)
+      if (q == qd)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rl + 4 * r;
+          tile[q][r] = row >= cl ? D[row * kBLd + cl] : 0.0;
         }
+    if (lane == 0) rank_s += kept;
+  };
+  // p = -1 only factors tile (0, 0)
+  for (int p = -1; p < T; ++p) {
+   if (p >= 0) {
+    // Y_pp is ready (and the reads of the previous panel are done)
+    __syncthreads();
+    // 2. panel tiles X_ap = A_ap Y^T: A operand (row l % 16, k = 4 s + l / 16)
+    // from the staged tile, B operand Y^T (k, column l % 16) = Y[l % 16][k]
+    refresh();
+#pragma unroll
+    for (int q = 0; q < kBTiles; ++q) {
+      const int a = tile_a(q), b = tile_b(q);
+      if (a > p && b == p) {
+        double* pa = P[a];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pa[(rl + 4 * r) * kBLd + cl] = tile[q][r];
+        wave_lds_sync();
+        f64x4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < kBT; s4 += 4)
+          x = mfma_f64(pa[cl * kBLd + s4 + rl], Ys[p][cl * kBLd + s4 + rl], x);
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pa[(rl + 4 * r) * kBLd + cl] = x[r];
+        tile[q] = x;
+      }
+      // one tile's loads at a time (hoisting all of them would exhaust
+      // the registers)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+   }
+    // 3. trailing tiles (a, b), a >= b > p: C -= X_ap X_bp^T.  The owner
+    // of tile (p + 1, p + 1) updates it first and factors it at once
+    // (lookahead), then goes on with its other tiles.
+    refresh();
+    const int pn = p + 1;
+    const bool next_owner = pn < T && wave == (pn < 8 ? pn : 15 - pn);
+    auto trail = [&](int q) {
+      const int a = tile_a(q), b = tile_b(q);
+      const double* pa = P[a];
+      const double* pb = P[b];
+      f64x4 c = tile[q];
+#pragma unroll
+      for (int s4 = 0; s4 < kBT; s4 += 4)
+        c = mfma_f64(-pa[cl * kBLd + s4 + rl], pb[cl * kBLd + s4 + rl], c);
+      tile[q] = c;
+    };
+    if (next_owner) {
+      const int qn = pn < 8 ? 16 : pn;
+      if (p >= 0)
+#pragma unroll
+        for (int q = 0; q < kBTiles; ++q)
+          if (q == qn) trail(q);
+      factor_diag(pn);
+      refresh();
+    }
+    if (p >= 0)
+#pragma unroll
+      for (int q = 0; q < kBTiles; ++q) {
+        const int a = tile_a(q), b = tile_b(q);
+        if (a >= 0 && b > p && !(next_owner && a == pn && b == pn)) trail(q);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    // (the barrier at the top of the next panel orders these reads of P
+    // before its writes)
+  }
+  __syncthreads();
+  // r = z = row K of L
+  const int bk = K / kBT, rk = K % kBT;
+#pragma unroll
+  for (int q = 0; q < kBTiles; ++q)
+    if (tile_a(q) == bk)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = kBT * tile_b(q) + cl;
+        if (rl + 4 * r == rk && col < K) rv[col] = tile[q][r];
+      }
+  __syncthreads();
+  // one barrier per block: the owner of tile row b (wave b or 15 - b)
+  // forms w_b = Y_bb^T r_b and subtracts its row's L_bc^T w_b from every
+  // r_c, c < b, before the next block starts
+  for (int b = T - 1; b >= 0; --b) {
+    if (wave == (b < 8 ? b : 15 - b)) {
+      refresh();
+      if (lane < kBT) {
+        double w = 0.0;
+#pragma unroll
+        for (int j = 0; j < kBT; ++j)
+          w += Ys[b][j * kBLd + lane] * rv[kBT * b + j];
+        wv[kBT * b + lane] = kBT * b + lane < K ? w : 0.0;
+      }
+      wave_lds_sync();
+      double wb[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) wb[r] = wv[kBT * b + rl + 4 * r];
+#pragma unroll
+      for (int q = 0; q < kBTiles; ++q) {
+        const int c = tile_b(q);
+        if (tile_a(q) == b && c < b) {
+          double part = 0.0;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) part += tile[q][r] * wb[r];
+          part += __shfl_xor(part, 16, 64);
+          part += __shfl_xor(part, 32, 64);
+          if (rl == 0) rv[kBT * c + cl] -= part;
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
     __syncthreads();
   }
-  if (t == 0) *rank_out = rank;
-  if (t >= 64) return;
-  // L^T w = z, column-oriented on one wave: lane l keeps the remainders of
-  // entries l, l + 64, l + 128, l + 192; eight rows of L are loaded ahead of
-  // their use.  beta = D w.  (The last barrier above ordered every write of
-  // L before these reads.)
-  double r[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int j = t + 64 * q;
-    r[q] = j < K ? g[K * kext + j] : 0.0;
-  }
-  for (int i0 = K - 1; i0 >= 0; i0 -= 8) {
-    double lrow[8][4], ldiag[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = i0 - u;
-      ldiag[u] = i >= 0 ? g[i * kext + i] : 0.0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int j = t + 64 * q;
-        lrow[u][q] = (i >= 0 && j < i) ? g[i * kext + j] : 0.0;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = i0 - u;
-      if (i < 0) break;
-      const int qi = i >> 6;
-      double own = r[0];
-#pragma unroll
-      for (int q = 1; q < 4; ++q) own = qi == q ? r[q] : own;
-      own = __shfl(own, i & 63, 64);
-      const double w = ldiag[u] > 0 ? own / ldiag[u] : 0.0;
-      if (t == 0) beta[i] = w * dsc[i];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) r[q] -= lrow[u][q] * w;
-    }
-  }
+  if (t < K) beta[t] = wv[t] * dsc[t];
+  if (t == 0) *rank_out = rank_s;
 }
 
 // continuation = relu(X beta); exercise iff ex > continuation.  The
