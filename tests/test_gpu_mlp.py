@@ -240,3 +240,31 @@ def test_mlp_lsm_five_assets_gain_over_the_polynomial_judge():
   np.testing.assert_allclose(res['poly_price'], want[0], rtol=2e-6)
   _check_learned_lsm(res, df[0], 2**16)
   assert abs(res['gain']) < 0.03 * res['poly_price']
+
+
+def test_mlp_date_training_graph_replays_match_eager():
+  # one date's training captured as a hipGraph (Adam reset by fill kernels,
+  # then the steps) and replayed three times, with eager work on the
+  # inputs and a synchronisation between the replays, against the same
+  # steps launched one by one: parameters, Adam state and loss bit for bit
+  # after every date
+  from tqf.engine.mlp_lsm import MlpModel
+  rs = np.random.RandomState(3)
+  n = 50000
+  x = np.zeros((n, 8), np.float32)
+  x[:, :6] = rs.standard_normal((n, 6))
+  t = (np.tanh(x[:, 0]) + 0.3 * x[:, 1] * x[:, 2]).astype(np.float32)
+  w = (rs.uniform(size=n) > 0.3).astype(np.float32)
+  xt, tt, wt = (torch.as_tensor(a, device='cuda') for a in (x, t, w))
+  a = MlpModel(seed=1)
+  b = MlpModel(seed=1)
+  for date in range(3):
+    a.train_date(xt, tt, wt, 40000, 8192, 25, 1e-3, graph=True)
+    b.train_date(xt, tt, wt, 40000, 8192, 25, 1e-3, graph=False)
+    torch.cuda.synchronize()
+    for u, v in ((a.params, b.params), (a.m, b.m), (a.v, b.v),
+                 (a.loss, b.loss)):
+      assert torch.equal(u, v), date
+    tt.mul_(0.9).add_(0.05)
+    torch.cuda.synchronize()
+  assert len(a._graphs) == 1

@@ -60,6 +60,16 @@ class MlpModel:
     ws = _native.lib().tqf_mlp_workspace_size(self.n_blocks)
     self._ws = torch.empty(ws // 4 + 1, dtype=torch.float32, device=self.dev)
     self.loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
+    self._graphs = {}
+
+  def reset(self, seed=0, params=None):
+    """Fresh parameters (init_params(seed) or the given vector) and Adam
+    state, in place: captured graphs stay valid."""
+    p = init_params(seed) if params is None else np.asarray(params, np.float32)
+    self.params.copy_(torch.as_tensor(p, device=self.dev))
+    self.m.zero_()
+    self.v.zero_()
+    self.step = 0
 
   def forward(self, x, out=None):
     n = x.shape[0]
@@ -81,6 +91,37 @@ class MlpModel:
   def train_step(self, x, t, w, first, n, inv_norm, lr):
     self.step += 1
     self._call(x, t, w, first, n, inv_norm, lr, self.step, None)
+
+  def train_date(self, x, t, w, n_cal, batch, steps, lr, graph=True):
+    """One exercise date's training: the Adam state reset, then `steps`
+    Adam steps over the windows first = k * batch mod (n_cal - batch + 1)
+    of the inputs.  With graph=True the whole sequence (two fill kernels
+    and 2 * steps launches) is captured once per (inputs, sizes) as a
+    hipGraph and replayed: the dates of one pricing repeat the same
+    launches on the same buffers, and a replay takes none of the
+    per-launch host work or the gaps between dependent launches."""
+    if not graph:
+      self._date_steps(x, t, w, n_cal, batch, steps, lr)
+      return
+    key = (x.data_ptr(), t.data_ptr(), w.data_ptr(), int(n_cal), int(batch),
+           int(steps), float(lr))
+    g = self._graphs.get(key)
+    if g is None:
+      g = torch.cuda.CUDAGraph()
+      with torch.cuda.graph(g):
+        self._date_steps(x, t, w, n_cal, batch, steps, lr)
+      self._graphs[key] = g
+    g.replay()
+    self.step = int(steps)
+
+  def _date_steps(self, x, t, w, n_cal, batch, steps, lr):
+    # zero_ is a fill kernel (no memset node in a captured graph)
+    self.m.zero_()
+    self.v.zero_()
+    for k in range(int(steps)):
+      first = (k * batch) % max(n_cal - batch + 1, 1)
+      self._call(x, t, w, first, batch, 1.0 / batch, lr, k + 1, None)
+    self.step = int(steps)
 
   def _call(self, x, t, w, first, n, inv_norm, lr, step, g):
     _native.call('tqf_mlp_train_step', ctypes.c_void_p(self.params.data_ptr()),
@@ -116,10 +157,46 @@ def _lsm_struct(paths, payoff, strike, degree=2):
   return l, expo_t
 
 
+# The buffers and the model of the last pricing size, kept so that the
+# per-date training graphs captured by MlpModel.train_date (which hold their
+# device addresses) are replayed by later calls instead of captured again.
+# One entry: a call with another device or path count replaces it.
+_STATE = {}
+
+
+def _state(dev, n):
+  key = (str(dev), int(n))
+  st = _STATE.get(key)
+  if st is None:
+    _STATE.clear()
+    f64 = dict(dtype=torch.float64, device=dev)
+    st = {
+        'cash': torch.empty(n, **f64), 'vals': torch.empty(n, **f64),
+        'exv': torch.empty(n, **f64), 'y': torch.empty(n, **f64),
+        'mom': torch.empty(_native.lib().tqf_lsm_moments_size(n) // 8 + 1,
+                           **f64),
+        'x': torch.empty((n, 8), dtype=torch.float32, device=dev),
+        'tgt': torch.empty(n, dtype=torch.float32, device=dev),
+        'w': torch.empty(n, dtype=torch.float32, device=dev),
+        'cont': torch.empty(n, dtype=torch.float32, device=dev),
+        'fin': torch.empty((n + 255) // 256 * 2 + 2, **f64),
+        'res': torch.empty(2, **f64),
+        'model': MlpModel(dev=dev),
+    }
+    _STATE[key] = st
+  return st
+
+
+def clear_cache():
+  """Drops the cached pricing buffers, model and training graphs."""
+  _STATE.clear()
+
+
 def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
                   num_calibration, payoff=lsm_lib.PAYOFF_MAX_CALL,
                   steps_per_date=200, batch=65536, lr=1e-3, seed=0,
-                  poly_degree=2, judge=True, return_path_values=False):
+                  poly_degree=2, judge=True, return_path_values=False,
+                  graph=True):
   """Prices with the learned continuation value (and the polynomial judge).
 
   paths: float64 device tensor [N, T, d], rows [0, num_calibration) train
@@ -128,7 +205,9 @@ def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
   gain mlp - poly and its standard error, the accepted price and the
   per-date final training losses; with return_path_values the per-path
   values of both policies at the first exercise date (float64 [N], before
-  the final discount).
+  the final discount).  graph: each date's training replays one captured
+  hipGraph (MlpModel.train_date); the first call of a size pays the
+  capture (about 0.1 s), later calls of that size replay it.
   """
   dev = paths.device
   paths = lsm_lib.step_major_f64(paths)
@@ -138,18 +217,12 @@ def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
   l, keep = _lsm_struct(paths, payoff, strike, poly_degree)
   s = _native.stream_ptr(dev)
   f64 = dict(dtype=torch.float64, device=dev)
-  cash = torch.empty(n, **f64)
-  vals = torch.empty(n, **f64)
-  exv = torch.empty(n, **f64)
-  y = torch.empty(n, **f64)
-  mom = torch.empty(_native.lib().tqf_lsm_moments_size(n) // 8 + 1, **f64)
-  x = torch.empty((n, 8), dtype=torch.float32, device=dev)
-  tgt = torch.empty(n, dtype=torch.float32, device=dev)
-  w = torch.empty(n, dtype=torch.float32, device=dev)
-  cont = torch.empty(n, dtype=torch.float32, device=dev)
-  fin = torch.empty((n + 255) // 256 * 2 + 2, **f64)
-  res = torch.empty(2, **f64)
-  model = MlpModel(seed=seed, dev=dev)
+  st = _state(dev, n)
+  cash, vals, exv, y = st['cash'], st['vals'], st['exv'], st['y']
+  mom, x, tgt, w = st['mom'], st['x'], st['tgt'], st['w']
+  cont, fin, res = st['cont'], st['fin'], st['res']
+  model = st['model']
+  model.reset(seed)
   losses = []
   _native.call('tqf_lsm_terminal', ctypes.byref(l), ctypes.c_int32(ex[-1]),
                ctypes.c_void_p(cash.data_ptr()),
@@ -168,12 +241,8 @@ def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
                  ctypes.c_void_p(x.data_ptr()),
                  ctypes.c_void_p(tgt.data_ptr()),
                  ctypes.c_void_p(w.data_ptr()), s)
-    model.m.zero_()
-    model.v.zero_()
-    model.step = 0
-    for k in range(int(steps_per_date)):
-      first = (k * batch) % max(n_cal - batch + 1, 1)
-      model.train_step(x, tgt, w, first, batch, 1.0 / batch, lr)
+    model.train_date(x, tgt, w, n_cal, batch, steps_per_date, lr,
+                     graph=graph)
     losses.append(model.loss.clone())
     model.forward(x, cont)
     _native.call('tqf_lsm_update_cont', ctypes.c_int64(n),
