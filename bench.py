@@ -712,8 +712,29 @@ def _paths_and_points(out, timed, engine):
   res['pathmajor_heston_ft'] = {
       'value': nbytes / t / 1e9, 'unit': 'GB/s', 'seconds': t,
       'bytes_written': nbytes, 'frac_hbm_peak': nbytes / t / HBM_PEAK,
-      'path_steps_per_s': n * 365 / t}
+      'path_steps_per_s': n * 365 / t,
+      'note': 'contiguous [N][k][d] kernel mode; euler_sampling.sample '
+              'returns the [N, k, d] view of the coalesced store '
+              '(sample_call_heston_ft)'}
   del lp
+  torch.cuda.empty_cache()
+  # the drop-in call itself: euler_sampling.sample with HestonModel's
+  # full-truncation coefficients (fused route), 2^20 paths, all 366 dates,
+  # shifted Sobol draws; the whole call is timed (grid, tables, output
+  # allocation, launch), and returns an [N, 366, 2] view of [366][2][N]
+  hfn = tqf.models.HestonModel(2.0, 0.04, 0.3, -0.7).euler_ft_fns(0.03)
+  sc = lambda: tqf.models.euler_sampling.sample(
+      2, hfn[0], hfn[1], all_times[1:], num_time_steps=365, num_samples=n,
+      initial_state=np.array([np.log(100.0), 0.04], np.float32),
+      random_type=tqf.math.random.RandomType.SOBOL, seed=7, times_grid=None,
+      dtype=torch.float32)
+  sc()
+  t, _ = timed(sc, 3)
+  nb = 365 * 2 * n * 4
+  res['sample_call_heston_ft'] = {
+      'value': nb / t / 1e9, 'unit': 'GB/s (output of the whole call)',
+      'seconds': t, 'bytes_written': nb, 'frac_hbm_peak': nb / t / HBM_PEAK,
+      'path_steps_per_s': n * 365 / t}
   torch.cuda.empty_cache()
   # sobol.sample(730, 2^20) float32 (the C2 draw set, 3.06 GB)
   tsobol.sample(730, n)

@@ -271,6 +271,40 @@ def test_euler_payoff_mode_matches_recorded_paths():
   np.testing.assert_array_equal(_np(store).transpose(2, 0, 1), _np(paths))
 
 
+def test_sampling_apis_return_the_store_as_a_path_major_view():
+  # euler_sampling.sample, HestonModel.sample_paths and the GBM sample_paths
+  # hand back [N, k, d] views of the coalesced [k, d, N] store: same shape
+  # and, bit for bit, the values of the contiguous path-major mode
+  from tqf.models import geometric_brownian_motion as gbm_lib
+  n = 30000
+  spec = ('heston', 0.03, 2.0, 0.04, 0.3, -0.7, 0.0)
+  all_times, keep = pricing.heston_grid_for([0.25, 0.5, 1.0], 120,
+                                            'reference', torch.float32)
+  x0 = np.array([np.log(100.), 0.04])
+  shift = pricing.sobol_digital_shift(240, 3)
+  for rng in (engine.RngSpec('philox', seed=9),
+              engine.RngSpec('sobol_shifted', shift=shift, first_index=1)):
+    a = engine.run(spec, all_times, keep, 3, x0, n, rng, torch.float32,
+                   mode='paths')
+    b = engine.run(spec, all_times, keep, 3, x0, n, rng, torch.float32,
+                   mode='view')
+    assert b.shape == a.shape and b.stride(0) == 1 and a.is_contiguous()
+    assert torch.equal(a, b)
+  times = np.linspace(0.1, 1.0, 10)
+  rng = engine.RngSpec('philox_antithetic', seed=4)
+  for dt in (torch.float32, torch.float64):
+    a = gbm_lib.run_exact(times, [0.03], [0.2], None, 100.0, n, rng, dt,
+                          mode='paths')
+    b = gbm_lib.run_exact(times, [0.03], [0.2], None, 100.0, n, rng, dt,
+                          mode='view')
+    assert b.shape == a.shape == (n, 10, 1) and b.stride(0) == 1
+    assert torch.equal(a, b)
+  p = tqf.models.GeometricBrownianMotion(0.05, 0.2).sample_paths(
+      times, initial_state=100.0, num_samples=1000, seed=(1, 2),
+      random_type=mvn.RandomType.STATELESS_ANTITHETIC)
+  assert p.shape == (1000, 10, 1) and p.stride(0) == 1
+
+
 def test_gbm_euler_prices_within_3se_of_black_scholes():
   # Plain method: 1-asset GBM Euler, float64, 1e5 paths, 100 steps
   strikes = np.linspace(80, 120, 8)

@@ -293,8 +293,11 @@ def prepare(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
             initial_written=None, dev=None):
   """Builds a Launch of the fused kernel.
 
-  mode 'paths'  -> [N, k, d] (reference layout)
+  mode 'paths'  -> [N, k, d] (reference layout, contiguous)
   mode 'store'  -> [k, d, N] (coalesced)
+  mode 'view'   -> [N, k, d] view of the coalesced [k, d, N] store (the
+                   reference's shape and values, written at the path-store
+                   rate; .contiguous() gives the reference layout)
   mode 'payoff' -> dict(prices [k, n_strikes], stderrs, sums [k, K, 2])
   """
   dev = dev or _native.device()
@@ -386,16 +389,21 @@ def prepare(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
 
   o = _native.Output()
   result = None
+  # slots that no executed step records stay zero; when every slot is
+  # written the output needs no zero-fill pass
+  alloc = torch.empty if len(filled) == int(num_requested) else torch.zeros
   if mode == 'paths':
     o.mode = _native.OUT_PATH_MAJOR
-    result = torch.zeros((num_samples, int(num_requested), d), dtype=dtype,
-                         device=dev)
+    result = alloc((num_samples, int(num_requested), d), dtype=dtype,
+                   device=dev)
     o.paths = result.data_ptr()
-  elif mode == 'store':
+  elif mode in ('store', 'view'):
     o.mode = _native.OUT_STEP_MAJOR
-    result = torch.zeros((int(num_requested), d, num_samples), dtype=dtype,
-                         device=dev)
+    result = alloc((int(num_requested), d, num_samples), dtype=dtype,
+                   device=dev)
     o.paths = result.data_ptr()
+    if mode == 'view':
+      result = result.permute(2, 0, 1)
   elif mode == 'payoff':
     if len(filled) != int(num_requested):
       raise ValueError('every requested time must be reached for payoffs')

@@ -161,8 +161,9 @@ def _fused(dim, spec, all_times, keep_mask, k, x0, num_samples, random_type,
   if watch_params is not None:
     return None
   rng = engine.RngSpec.from_random_type(random_type, seed, skip, draws)
+  # the [N, k, d] result is a view of the coalesced [k, d, N] path store
   return engine.run(spec, all_times, keep_mask, k, x0, num_samples, rng,
-                    dtype, mode='paths', dev=dev)
+                    dtype, mode='view', dev=dev)
 
 
 def _generic_loop(dim, drift_fn, volatility_fn, all_times, keep_mask, k, x0,

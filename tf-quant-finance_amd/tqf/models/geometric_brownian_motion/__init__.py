@@ -41,7 +41,8 @@ def _dtype_of(dtype, *vals):
 def run_exact(times, drift, vol, chol, s0, num_samples, rng, dtype,
               keep_mask=None, mode='paths', dev=None):
   """Runs tqf_gbm_exact over the grid [0] + times (every time recorded
-  unless keep_mask says otherwise).  Returns [N, k, d] or [k, d, N]."""
+  unless keep_mask says otherwise).  mode 'paths': [N, k, d] contiguous;
+  'store': [k, d, N]; 'view': the [N, k, d] view of the [k, d, N] store."""
   dev = dev or _native.device()
   npd = utils.np_dtype(dtype)
   times = np.asarray(times, npd).reshape(-1)
@@ -114,17 +115,25 @@ def run_exact(times, drift, vol, chol, s0, num_samples, rng, dtype,
       sh = dv(np.asarray(rng.shift, np.uint64).astype(np.uint32).view(np.int32),
               torch.int32)
       r.sobol_shift = sh.data_ptr()
+  # every slot is written unless a requested time lies past the executed
+  # steps; only then does the output need zeros first
+  written = set(int(v) for v in slots[:executed] if v >= 0)
+  if init_slot >= 0:
+    written.add(int(init_slot))
+  alloc = torch.empty if len(written) == k else torch.zeros
   if mode == 'paths':
-    out = torch.zeros((num_samples, k, d), dtype=dtype, device=dev)
+    out = alloc((num_samples, k, d), dtype=dtype, device=dev)
     code = _native.OUT_PATH_MAJOR
   else:
-    out = torch.zeros((k, d, num_samples), dtype=dtype, device=dev)
+    out = alloc((k, d, num_samples), dtype=dtype, device=dev)
     code = _native.OUT_STEP_MAJOR
   _native.call('tqf_gbm_exact', ctypes.byref(g), ctypes.byref(grid),
                ctypes.byref(r), ctypes.c_int32(code),
                ctypes.c_void_p(out.data_ptr()), ctypes.c_int64(num_samples),
                ctypes.c_int32(_native.dtype_code(dtype)),
                _native.stream_ptr(dev))
+  if mode == 'view':
+    out = out.permute(2, 0, 1)
   out._tqf_keep = keep
   return out
 
@@ -171,7 +180,8 @@ class GeometricBrownianMotion:
   def sample_paths(self, times, initial_state=None, num_samples=1,
                    random_type=None, seed=None, skip=0, normal_draws=None,
                    name=None):
-    """Exact log-normal paths [num_samples, k, 1] at `times`."""
+    """Exact log-normal paths [num_samples, k, 1] at `times` (a view of
+    the coalesced [k, 1, num_samples] store)."""
     del name
     dev = _native.device()
     times_np = utils.to_numpy(times, utils.np_dtype(self._dtype)).reshape(-1)
@@ -193,7 +203,7 @@ class GeometricBrownianMotion:
     def run():
       return run_exact(times_np, [mu - sig * sig / 2], [sig], None,
                        s0.detach(), int(num_samples), rng, self._dtype,
-                       dev=dev)
+                       mode='view', dev=dev)
     params = [self._mean, self._volatility]
     if torch.is_grad_enabled() and (s0.requires_grad or any(
         torch.is_tensor(p) and p.requires_grad for p in params)):
@@ -312,4 +322,4 @@ class MultivariateGeometricBrownianMotion:
     rng = engine.RngSpec.from_random_type(random_type, seed, skip, draws)
     drift = self._means - self._vols**2 / 2
     return run_exact(times_np, drift, self._vols, self.cholesky(), s0,
-                     int(num_samples), rng, self._dtype, dev=dev)
+                     int(num_samples), rng, self._dtype, mode='view', dev=dev)

@@ -188,7 +188,7 @@ class HestonModel:
     rng = engine.RngSpec.from_random_type(random_type, seed, skip, draws)
     spec = ('heston_qe', coef)
     return engine.run(spec, all_times, keep_mask, times_np.shape[0], x0,
-                      int(num_samples), rng, dtype, mode='paths',
+                      int(num_samples), rng, dtype, mode='view',
                       initial_written=0, dev=dev)
 
   def expected_total_variance(self, future_times, initial_var, name=None):
