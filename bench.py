@@ -495,13 +495,20 @@ def _batch(out, timed, batch_lib, pool):
   # 1/sqrt(v) factors that can overflow float32
   feller = np.array([c['kind'] == batch_lib.KIND_GBM or
                      2 * c['kappa'] * c['theta'] >= c['xi']**2 for c in cs])
+  # relative differences where both are finite and the scale is positive
+  # (a zero price and Greek, e.g. a contract with no path in the money,
+  # is compared absolutely)
+  ok = fin & (scale > 0)
+  rel = np.zeros_like(means)
+  rel[ok] = np.abs(b[:, :8][ok] - means[ok]) / scale[ok]
+  zero = fin & (scale == 0)
   acc['greeks_%s_vs_%s' % (other, cfg['greeks'])] = {
       'contracts': len(cs),
-      'max_rel_diff': float(np.max(np.where(fin, np.abs(b[:, :8] - means) /
-                                            scale, 0.0))),
-      'max_rel_diff_feller_contracts': float(np.max(np.where(
-          fin[feller], np.abs(b[feller, :8] - means[feller]) / scale[feller],
-          0.0))),
+      'entries_compared': int(ok.sum()),
+      'max_rel_diff': float(rel.max()),
+      'max_rel_diff_feller_contracts': float(rel[feller].max()),
+      'max_abs_diff_zero_scale': float(np.max(
+          np.abs(b[:, :8][zero] - means[zero]), initial=0.0)),
       'contracts_with_nonfinite_greeks': {
           cfg['greeks']: int(np.sum(~np.all(np.isfinite(means), 1))),
           other: int(np.sum(~np.all(np.isfinite(b[:, :8]), 1)))},
