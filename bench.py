@@ -565,12 +565,19 @@ def _learned_lsm(out, pool):
   paths, df = mlp_lsm.bermudan_max_call_paths(2 * n_cal)
   mlp_lsm.mlp_lsm_price(paths, ex, 100.0, df, n_cal, judge=False)
   del paths
-  torch.cuda.synchronize()
-  t0 = time.perf_counter()
-  paths, df = mlp_lsm.bermudan_max_call_paths(2 * n_cal)
-  res = mlp_lsm.mlp_lsm_price(paths, ex, 100.0, df, n_cal)
-  torch.cuda.synchronize()
-  total = time.perf_counter() - t0
+  # the whole pipeline (paths, learned policy, polynomial judge, paired
+  # statistic): median of three runs, and the path simulation alone
+  totals, sims = [], []
+  for _ in range(3):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    paths, df = mlp_lsm.bermudan_max_call_paths(2 * n_cal)
+    torch.cuda.synchronize()
+    sims.append(time.perf_counter() - t0)
+    res = mlp_lsm.mlp_lsm_price(paths, ex, 100.0, df, n_cal)
+    torch.cuda.synchronize()
+    totals.append(time.perf_counter() - t0)
+  total = float(np.median(totals))
   dfp = lsm_lib._padded_discounts(df, len(ex), torch.float64)
   payoff = (lsm_lib.PAYOFF_MAX_CALL, np.array([100.0]))
 
@@ -590,6 +597,8 @@ def _learned_lsm(out, pool):
   reg_flop = FLOP_LSM_DATE * (len(ex) - 1)
   line = {
       'seconds': total,
+      'seconds_runs': [round(x, 4) for x in totals],
+      'paths_seconds': float(np.median(sims)),
       'mlp_price': res['mlp_price'], 'mlp_se': res['mlp_se'],
       'poly_price': res['poly_price'], 'poly_se': res['poly_se'],
       'gain': res['gain'], 'gain_se_paired': res['gain_se'],
