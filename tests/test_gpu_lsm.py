@@ -379,3 +379,29 @@ def test_log_price_paths_equal_the_log_basis_on_prices(dim):
                                discount_factors=dfs,
                                num_calibration_samples=n // 2)[0]
     np.testing.assert_allclose(float(_np(got)[0]), ref, rtol=2e-6)
+
+
+@pytest.mark.parametrize('degree', [0, 1, 2, 3, 4])
+def test_one_asset_chain_every_degree_matches_oracle(degree):
+  # the two-launch one-asset date chain (degree <= 4, full basis) against
+  # the oracle's least_square_mc on the same paths, with calibration rows
+  # and per-date discount factors
+  from tqf.models import longstaff_schwartz as lsm_lib
+  rs = np.random.RandomState(20 + degree)
+  n, t = 60000, 7
+  z = rs.standard_normal((n, t, 1))
+  times = np.linspace(0.15, 1.05, t)
+  paths = ogbm.sample_paths(times, 0.04, 0.3, 1.0, z)
+  dfs = np.exp(-0.04 * times)
+  dev = torch.device('cuda')
+  got = lsm_lib._native_lsm(
+      torch.as_tensor(paths, device=dev), list(range(t)),
+      (lsm_lib.PAYOFF_BASKET_PUT, np.array([1.05])), degree,
+      np.concatenate([[1.0], dfs]).reshape(-1, 1, 1), n // 3,
+      torch.float64, dev)
+  want = olsm.least_square_mc(
+      paths, list(range(t)),
+      lambda x, ti: np.maximum(1.05 - x[:, ti, :1], 0),
+      olsm.make_polynomial_basis(degree, True), discount_factors=dfs,
+      num_calibration_samples=n // 3)[0]
+  np.testing.assert_allclose(float(_np(got)[0]), want, rtol=2e-6)
