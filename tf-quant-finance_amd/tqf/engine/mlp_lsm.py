@@ -160,7 +160,9 @@ def _lsm_struct(paths, payoff, strike, degree=2):
 # The buffers and the model of the last pricing size, kept so that the
 # per-date training graphs captured by MlpModel.train_date (which hold their
 # device addresses) are replayed by later calls instead of captured again.
-# One entry: a call with another device or path count replaces it.
+# One entry: a call with another device or path count replaces it.  Calls
+# share these buffers, so mlp_lsm_price is not for concurrent use from
+# several host threads.
 _STATE = {}
 
 
