@@ -34,6 +34,18 @@ constexpr int kSubPaths = 32;   // paths staged per LDS round
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// LDS written by one lane of a wave is visible to the others after this
+// (no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 struct LsmArgs {
   const double* paths;      // float64, element (p, t, j) at
                             // paths[p sp + t st + j sa]
@@ -549,6 +561,286 @@ lsm_moment_kernel(LsmArgs a, MomentPlan mp, const double* __restrict__ exv,
   }
 }
 
+// The moment product specialised on its plan (D assets, degree DEG, DA
+// dimensions in the row group, DA <= 2 and at most 3 column dimensions).
+// Each wave owns every 16 x 16 output block and a private run of 64-path
+// sub-batches (sub-batches wave, wave + W, ... of the chunk for W waves),
+// so the product runs without a workgroup barrier until the fixed-order sum
+// of the waves' blocks at the end.  Per sub-batch each lane writes its
+// path's record of factor tables to the wave's LDS slice:
+//   TA[alpha]  prod_{j < DA} c_j^alpha_j       (alpha < R^DA, R = 2 DEG + 1)
+//   TU[u]      prod of the column dimensions but the last, c^u_q
+//              (u < R^NU, NU = D - DA - 1 of them)
+//   TW[e]      w c_{D-1}^e       (w = 1 in the money, else 0; e < R)
+//   TY[e]      w y c_{D-1}^e     (e <= DEG)
+//   and a zero,
+// so for each group of 4 paths a lane's A operand (row alpha) is one entry
+// TA[alpha] and its B operand (column beta) the product TU[u] TW[e] of a
+// moment column or TU[u'] TY[e'] of an rhs column (digits of beta in radix
+// R, or in radix DEG + 1 re-encoded for TU); padding rows and columns read
+// the zero.  The offsets are fixed by the lane and the block.  The output
+// is the same [chunk][rap][cbp] partial as lsm_moment_kernel.
+// Plans of more than 8 blocks run one wave per SIMD (4 per workgroup), so
+// a wave has registers for its accumulators (8 per block), the operand
+// offsets and the next group's reads in flight; smaller ones run 8 waves.
+// The inputs of a wave's next sub-batch are loaded while it multiplies the
+// current one.
+constexpr int kFastSub = 64;    // paths per wave sub-batch
+constexpr int kFastMaxBlocks = 20;
+
+constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
+
+template <int D, int DEG, int DA>
+struct FastPlan {
+  static constexpr int R = 2 * DEG + 1, NBD = D - DA, NU = NBD - 1;
+  static constexpr int RA = cpow(R, DA), CBM = cpow(R, NBD);
+  static constexpr int CBY = cpow(DEG + 1, NBD);
+  static constexpr int RAP = (RA + 15) / 16 * 16;
+  static constexpr int CBP = (CBM + CBY + 15) / 16 * 16;
+  static constexpr int NBI = RAP / 16, NBJ = CBP / 16;
+  // record layout (doubles)
+  static constexpr int TA = 0, TU = RA, TW = TU + cpow(R, NU);
+  static constexpr int TY = TW + R, ZERO = TY + DEG + 1;
+  static constexpr int S = (ZERO + 1) | 1;  // odd record stride
+  static constexpr int WAVES = NBI * NBJ > 8 ? 4 : 8;
+  static constexpr int THREADS = 64 * WAVES;
+  static constexpr int REC = WAVES * kFastSub * S;
+  static constexpr int LDS = REC > RAP * CBP ? REC : RAP * CBP;
+  // operand reads per group of 4 paths, spread over half of its MFMAs
+  static constexpr int HALF = (NBI * NBJ + 1) / 2;
+  static constexpr int READS_PER_MFMA =
+      (NBI + NBJ * (NU > 0 ? 2 : 1) + HALF - 1) / HALF;
+  static_assert(DA >= 1 && DA <= 2 && NBD >= 1 && NU <= 2,
+                "factor tables of one or two dimensions");
+  static_assert(NBI * NBJ <= kFastMaxBlocks, "accumulators of one wave");
+  static_assert(LDS * 8 <= 160 * 1024, "LDS of one workgroup");
+};
+
+// The product of the powers of one or two dimensions: t[a + R b] =
+// p0[a] p1[b] (N = 2) or t[a] = p0[a] (N = 1).
+template <int N, int R>
+__device__ __forceinline__ void factor_table(double* t, const double* p0,
+                                             const double* p1) {
+  if constexpr (N == 1) {
+#pragma unroll
+    for (int e = 0; e < R; ++e) t[e] = p0[e];
+  } else if constexpr (N == 2) {
+#pragma unroll
+    for (int b = 0; b < R; ++b)
+#pragma unroll
+      for (int e = 0; e < R; ++e) t[e + R * b] = p0[e] * p1[b];
+  }
+}
+
+// LOG: the basis variable is ln of the stored value (basis_log 1).
+template <int D, int DEG, int DA, bool LOG>
+__global__ void __launch_bounds__((FastPlan<D, DEG, DA>::THREADS))
+lsm_moment_fast_kernel(LsmArgs a, const double* __restrict__ exv,
+                       const double* __restrict__ y, int64_t n_calib,
+                       int64_t chunk, double* __restrict__ partial) {
+  using P = FastPlan<D, DEG, DA>;
+  constexpr int R = P::R, NU = P::NU, NBI = P::NBI, NBJ = P::NBJ;
+  constexpr int S = P::S;
+  __shared__ double lds[P::LDS];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int g = lane >> 4, r = lane & 15;
+
+  // lane-constant record offsets of the operands
+  int aoff[NBI], uoff[NBJ], woff[NBJ];
+#pragma unroll
+  for (int bi = 0; bi < NBI; ++bi) {
+    const int alpha = bi * 16 + r;
+    aoff[bi] = alpha < P::RA ? P::TA + alpha : P::ZERO;
+  }
+#pragma unroll
+  for (int bj = 0; bj < NBJ; ++bj) {
+    const int beta = bj * 16 + r;
+    int u = 0, o = P::ZERO;
+    if (beta < P::CBM) {
+      u = beta % cpow(R, NU);
+      o = P::TW + beta / cpow(R, NU);
+    } else if (beta < P::CBM + P::CBY) {
+      int rem = beta - P::CBM;
+#pragma unroll
+      for (int q = 0, m = 1; q < NU; ++q, m *= R) {
+        u += (rem % (DEG + 1)) * m;
+        rem /= DEG + 1;
+      }
+      o = P::TY + rem;
+    }
+    uoff[bj] = P::TU + u;
+    woff[bj] = o;
+  }
+
+  f64x4 acc[NBI][NBJ];
+#pragma unroll
+  for (int bi = 0; bi < NBI; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < NBJ; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
+  // the loop-carried accumulators live in accumulator registers on entry
+  // to and around the sub-batch loop (no copies to and from VGPRs at its
+  // edges)
+  auto pin_acc = [&]() {
+#pragma unroll
+    for (int bi = 0; bi < NBI; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < NBJ; ++bj) asm volatile("" : "+a"(acc[bi][bj]));
+  };
+  pin_acc();
+
+  const int64_t p_begin = (int64_t)blockIdx.x * chunk;
+  int64_t p_end = p_begin + chunk;
+  if (p_end > n_calib) p_end = n_calib;
+  const int nsub = p_end > p_begin
+      ? (int)((p_end - p_begin + kFastSub - 1) / kFastSub) : 0;
+  double* const rec = lds + wave * (kFastSub * S);
+  // inputs of sub-batch i: the exercise value, y and the D stored values.
+  // The loop body has no branch (rows past the chunk read the chunk's first
+  // row and get w = 0), so the accumulators stay in place across it.
+  double in_e = 0.0, in_y = 0.0, in_x[D], mu[D], sc[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    mu[j] = a.mean[j];
+    sc[j] = a.scale[j];
+  }
+  auto load = [&](int i) {
+    int64_t p = p_begin + (int64_t)i * kFastSub + lane;
+    const bool ok = i < nsub && p < p_end;
+    p = ok ? p : p_begin;
+    in_e = exv[p];
+    in_y = y[p];
+#pragma unroll
+    for (int j = 0; j < D; ++j) in_x[j] = path_x(a, p, j);
+    in_e = ok ? in_e : 0.0;
+  };
+  if (nsub > 0) load(wave);
+  for (int i = wave; i < nsub; i += P::WAVES) {
+    // this lane's path record; rows past the chunk or out of the money
+    // have w = 0 and unit powers
+    const bool itm = in_e > 0;
+    double pw[D][R];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const double b = LOG ? log(in_x[j]) : in_x[j];
+      const double c = itm ? (b - mu[j]) * sc[j] : 0.0;
+      pw[j][0] = 1.0;
+#pragma unroll
+      for (int e = 1; e < R; ++e) pw[j][e] = pw[j][e - 1] * c;
+    }
+    const double w = itm ? 1.0 : 0.0, wy = itm ? in_y : 0.0;
+    load(i + P::WAVES);
+    double* mine = rec + lane * S;
+    factor_table<DA, R>(mine + P::TA, pw[0], pw[DA > 1 ? 1 : 0]);
+    factor_table<NU, R>(mine + P::TU, pw[DA], pw[NU > 1 ? DA + 1 : DA]);
+#pragma unroll
+    for (int e = 0; e < R; ++e) mine[P::TW + e] = w * pw[D - 1][e];
+#pragma unroll
+    for (int e = 0; e <= DEG; ++e) mine[P::TY + e] = wy * pw[D - 1][e];
+    mine[P::ZERO] = 0.0;
+    wave_lds_sync();
+    // operands of the group of paths 4 k0 .. 4 k0 + 3
+    auto fetch = [&](int k0, double (&av)[NBI], double (&bv)[NBJ]) {
+      const double* pr = rec + (k0 * 4 + g) * S;
+#pragma unroll
+      for (int bi = 0; bi < NBI; ++bi) av[bi] = pr[aoff[bi]];
+#pragma unroll
+      for (int bj = 0; bj < NBJ; ++bj)
+        bv[bj] = NU > 0 ? pr[uoff[bj]] * pr[woff[bj]] : pr[woff[bj]];
+    };
+    // software pipeline: the reads and products of group k0 + 1 issue
+    // between the MFMAs of group k0
+    double av[NBI], bv[NBJ];
+    fetch(0, av, bv);
+#pragma unroll
+    for (int k0 = 0; k0 < kFastSub / 4; ++k0) {
+      double an[NBI], bn[NBJ];
+      if (k0 + 1 < kFastSub / 4) fetch(k0 + 1, an, bn);
+#pragma unroll
+      for (int bi = 0; bi < NBI; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < NBJ; ++bj)
+          acc[bi][bj] = mfma_f64(av[bi], bv[bj], acc[bi][bj]);
+      if (k0 + 1 < kFastSub / 4) {
+        // first half of the MFMAs: the reads of group k0 + 1 in flight;
+        // second half: their products
+#pragma unroll
+        for (int m = 0; m < P::HALF; ++m) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          __builtin_amdgcn_sched_group_barrier(0x100, P::READS_PER_MFMA, 0);
+        }
+#pragma unroll
+        for (int m = P::HALF; m < NBI * NBJ; ++m) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);  // VALU
+        }
+#pragma unroll
+        for (int bi = 0; bi < NBI; ++bi) av[bi] = an[bi];
+#pragma unroll
+        for (int bj = 0; bj < NBJ; ++bj) bv[bj] = bn[bj];
+      }
+      __builtin_amdgcn_sched_barrier(0);  // groups do not mix
+    }
+    wave_lds_sync();  // the next sub-batch overwrites the records
+    pin_acc();
+  }
+
+  // fixed-order sum of the waves' blocks: accumulator entry q of lane l is
+  // row l / 16 + 4 q, column l % 16 of its block
+  __syncthreads();
+  double* const buf = lds;  // [rap][cbp]
+  for (int wv = 0; wv < P::WAVES; ++wv) {
+    if (wave == wv) {
+#pragma unroll
+      for (int bi = 0; bi < NBI; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < NBJ; ++bj)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            double* o = buf + (bi * 16 + g + 4 * q) * P::CBP + bj * 16 + r;
+            *o = wv == 0 ? acc[bi][bj][q] : *o + acc[bi][bj][q];
+          }
+    }
+    __syncthreads();
+  }
+  double* out = partial + (int64_t)blockIdx.x * (P::RAP * P::CBP);
+  for (int e = tid; e < P::RAP * P::CBP; e += P::THREADS) out[e] = buf[e];
+}
+
+// Launches the specialised moment product when one is built for the plan;
+// false otherwise (the generic kernel then runs).
+template <int D, int DEG, int DA>
+bool try_moment_fast(const MomentPlan& mp, int d, int deg, const LsmArgs& a,
+                     const double* exv, const double* y, int64_t n_calib,
+                     int64_t chunk, int n_chunks, double* partial,
+                     hipStream_t s) {
+  using P = FastPlan<D, DEG, DA>;
+  if (d != D || deg != DEG || mp.dA != DA || mp.rap != P::RAP ||
+      mp.cbp != P::CBP || mp.cbm != P::CBM)
+    return false;
+  if (a.basis_log == 1)
+    hipLaunchKernelGGL((lsm_moment_fast_kernel<D, DEG, DA, true>), n_chunks,
+                       P::THREADS, 0, s, a, exv, y, n_calib, chunk, partial);
+  else
+    hipLaunchKernelGGL((lsm_moment_fast_kernel<D, DEG, DA, false>), n_chunks,
+                       P::THREADS, 0, s, a, exv, y, n_calib, chunk, partial);
+  return true;
+}
+
+bool launch_moment_fast(const MomentPlan& mp, int d, int deg,
+                        const LsmArgs& a, const double* exv, const double* y,
+                        int64_t n_calib, int64_t chunk, int n_chunks,
+                        double* partial, hipStream_t s) {
+  return try_moment_fast<2, 2, 1>(mp, d, deg, a, exv, y, n_calib, chunk,
+                                  n_chunks, partial, s) ||
+         try_moment_fast<3, 2, 2>(mp, d, deg, a, exv, y, n_calib, chunk,
+                                  n_chunks, partial, s) ||
+         try_moment_fast<4, 2, 2>(mp, d, deg, a, exv, y, n_calib, chunk,
+                                  n_chunks, partial, s) ||
+         try_moment_fast<5, 2, 2>(mp, d, deg, a, exv, y, n_calib, chunk,
+                                  n_chunks, partial, s);
+}
+
 // One asset: the moments are c^0 .. c^(2 deg) and y c^0 .. y c^deg
 // (5 + 3 at degree 2).  Each thread accumulates them for its paths in
 // registers by running products; a fixed-order workgroup sum gives one row
@@ -746,16 +1038,6 @@ constexpr int kSolveThreads = 64 * kSolveWaves;
 constexpr int kBT = 16;         // tile edge
 constexpr int kBLd = 17;        // LDS row stride of a staged tile (doubles)
 constexpr int kBTiles = 17;     // two tile rows per wave: 136 / 8
-
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 
 __global__ void __launch_bounds__(kSolveThreads)
 lsm_solve_kernel(const double* __restrict__ g, int K, double rtol,
@@ -1832,7 +2114,8 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
       else if (mp.small)
         hipLaunchKernelGGL(lsm_moment_small_kernel<kSmallMoments>, n_chunks,
                            256, 0, s, a, mp, exv, y, n_calib, chunk, partial);
-      else
+      else if (!launch_moment_fast(mp, a.d, a.deg, a, exv, y, n_calib, chunk,
+                                   n_chunks, partial, s))
         hipLaunchKernelGGL(lsm_moment_kernel, n_chunks, kMomThreads, mp.lds,
                            s, a, mp, exv, y, n_calib, chunk, partial);
       hipLaunchKernelGGL(lsm_moment_reduce_kernel, (size + 255) / 256, 256, 0,
