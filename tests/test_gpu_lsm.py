@@ -344,12 +344,14 @@ def test_total_degree_table_takes_the_dense_gram_and_basis_order_update():
   np.testing.assert_allclose(_np(got), want, rtol=2e-6)
 
 
-@pytest.mark.parametrize('dim', [1, 2])
+@pytest.mark.parametrize('dim', [1, 2, 5])
 def test_log_price_paths_equal_the_log_basis_on_prices(dim):
   # basis_log = 2 (paths hold ln S: basis on them, payoff on exp, the
   # one-asset in-the-money test as ln S < ln K) against basis_log = 1 on
   # exp of the same paths, with half the rows as calibration samples; one
-  # asset runs the two-launch date chain, two the general one
+  # asset runs the two-launch date chain, two and five the general one
+  # (with the moment products specialised for 2 and 5 assets, ln taken in
+  # the product for basis_log = 1)
   from tqf.models import longstaff_schwartz as lsm_lib
   rs = np.random.RandomState(5 + dim)
   n, t = 40000, 9

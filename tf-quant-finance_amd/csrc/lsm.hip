@@ -1440,6 +1440,98 @@ lsm_update_dense_kernel(LsmArgs a, double df_ratio,
   }
 }
 
+// The dense Horner with the degree fixed: the nest unrolls to constant
+// coefficient offsets into the workgroup's LDS copy of the tensor (one
+// broadcast read per coefficient, no scalar load and wait per coefficient)
+// and a thread evaluates NP paths, so each read feeds NP FMAs.  The
+// operations of each path are DenseHorner's, in the same order.
+template <int J, int DEG, int NP>
+struct DenseHornerFixed {
+  __device__ __forceinline__ static void eval(const double* __restrict__ t,
+                                              const double (*c)[NP],
+                                              double (&h)[NP]) {
+    constexpr int ST = cpow(DEG + 1, J);
+    DenseHornerFixed<J - 1, DEG, NP>::eval(t + DEG * ST, c, h);
+    if constexpr (J >= 3) {
+      // dimensions above the third loop at run time: only a sub-nest of
+      // (DEG + 1)^3 coefficients is unrolled, so its reads are the ones in
+      // flight (not the whole tensor's)
+#pragma unroll 1
+      for (int m = DEG - 1; m >= 0; --m) {
+        double g[NP];
+        DenseHornerFixed<J - 1, DEG, NP>::eval(t + m * ST, c, g);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) h[q] = fma(h[q], c[J][q], g[q]);
+      }
+    } else {
+#pragma unroll
+      for (int m = DEG - 1; m >= 0; --m) {
+        double g[NP];
+        DenseHornerFixed<J - 1, DEG, NP>::eval(t + m * ST, c, g);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) h[q] = fma(h[q], c[J][q], g[q]);
+      }
+    }
+  }
+};
+template <int DEG, int NP>
+struct DenseHornerFixed<0, DEG, NP> {
+  __device__ __forceinline__ static void eval(const double* __restrict__ t,
+                                              const double (*c)[NP],
+                                              double (&h)[NP]) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) h[q] = t[DEG];
+#pragma unroll
+    for (int m = DEG - 1; m >= 0; --m)
+#pragma unroll
+      for (int q = 0; q < NP; ++q) h[q] = fma(h[q], c[0][q], t[m]);
+  }
+};
+
+constexpr int kDenseP = 4;  // paths per thread of the fixed-degree update
+
+// lsm_update_dense_kernel at degree DEG: thread t of block b takes paths
+// b 256 NP + t + 256 q, q < NP (coalesced per q).
+template <int D, int DEG>
+__global__ void __launch_bounds__(256)
+lsm_update_dense_fixed_kernel(LsmArgs a, double df_ratio,
+                              const double* __restrict__ exv,
+                              const double* __restrict__ dense,
+                              double* __restrict__ cash,
+                              double* __restrict__ vals) {
+  constexpr int NT = cpow(DEG + 1, D);
+  __shared__ double tl[NT];
+  for (int e = threadIdx.x; e < NT; e += 256) tl[e] = dense[e];
+  const int64_t base = (int64_t)blockIdx.x * (256 * kDenseP) + threadIdx.x;
+  const int64_t n = a.n_paths;
+  double c[D][kDenseP];
+#pragma unroll
+  for (int q = 0; q < kDenseP; ++q) {
+    int64_t p = base + 256 * q;
+    p = p < n ? p : n - 1;  // rows past the end compute and are not stored
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      c[j][q] = (basis_input(a, path_x(a, p, j)) - a.mean[j]) * a.scale[j];
+  }
+  __syncthreads();
+  double h[kDenseP];
+  DenseHornerFixed<D - 1, DEG, kDenseP>::eval(tl, c, h);
+#pragma unroll
+  for (int q = 0; q < kDenseP; ++q) {
+    const int64_t p = base + 256 * q;
+    if (p >= n) break;
+    const double cont = h[q] > 0 ? h[q] : 0;
+    const double ex = exv[p];
+    if (ex > cont) {
+      vals[p] = 0;
+      cash[p] = ex;
+    } else {
+      vals[p] = disc_ratio(a, df_ratio, p) * (cash[p] + vals[p]);
+      cash[p] = 0;
+    }
+  }
+}
+
 // Dense coefficients of (d, deg), or 0 when the dense update does not apply:
 // more than kMaxDense of them, or more work than the basis-order sum.
 int dense_size(int d, int deg, int K) {
@@ -2141,21 +2233,41 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
       double* dense = partial;
       hipLaunchKernelGGL(lsm_dense_beta_kernel, 1, kMaxDense, 0, s, beta,
                          a.expo, K, a.d, a.deg, ndense, dense);
-      switch (a.d) {
+      // degree 2 (the tensor basis of up to 6 assets): the unrolled nest
+      if (a.deg == 2 && a.d <= 6) {
+        const unsigned fb =
+            (unsigned)((n + 256 * kDenseP - 1) / (256 * kDenseP));
+        switch (a.d) {
+#define TQF_DENSE_UPDATE_DEG2(D)                                             \
+  case D:                                                                    \
+    hipLaunchKernelGGL((lsm_update_dense_fixed_kernel<D, 2>), fb, 256, 0, s, \
+                       a, ratio, exv, dense, cash, vals);                    \
+    break;
+          TQF_DENSE_UPDATE_DEG2(1)
+          TQF_DENSE_UPDATE_DEG2(2)
+          TQF_DENSE_UPDATE_DEG2(3)
+          TQF_DENSE_UPDATE_DEG2(4)
+          TQF_DENSE_UPDATE_DEG2(5)
+          TQF_DENSE_UPDATE_DEG2(6)
+#undef TQF_DENSE_UPDATE_DEG2
+        }
+      } else {
+        switch (a.d) {
 #define TQF_DENSE_UPDATE(D)                                                  \
   case D:                                                                    \
     hipLaunchKernelGGL(lsm_update_dense_kernel<D>, blocks_for(n), 256, 0, s, \
                        a, ratio, exv, dense, cash, vals);                    \
     break;
-        TQF_DENSE_UPDATE(1)
-        TQF_DENSE_UPDATE(2)
-        TQF_DENSE_UPDATE(3)
-        TQF_DENSE_UPDATE(4)
-        TQF_DENSE_UPDATE(5)
-        TQF_DENSE_UPDATE(6)
-        TQF_DENSE_UPDATE(7)
-        TQF_DENSE_UPDATE(8)
+          TQF_DENSE_UPDATE(1)
+          TQF_DENSE_UPDATE(2)
+          TQF_DENSE_UPDATE(3)
+          TQF_DENSE_UPDATE(4)
+          TQF_DENSE_UPDATE(5)
+          TQF_DENSE_UPDATE(6)
+          TQF_DENSE_UPDATE(7)
+          TQF_DENSE_UPDATE(8)
 #undef TQF_DENSE_UPDATE
+        }
       }
     } else {
       const int ut = update_threads(a);
