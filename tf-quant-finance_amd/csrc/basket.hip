@@ -33,21 +33,27 @@ basket_kernel(const float* __restrict__ prm, int n_steps, float t, float r,
   __shared__ double red[kBlockK / kWave][2 * NS];
   for (int e = threadIdx.x; e < 3 * D + D * D; e += kBlockK) sp[e] = prm[e];
   __syncthreads();
-  const float* s0 = sp;
-  const float* sig = sp + D;
-  const float* wt = sp + 2 * D;
-  const float* L = sp + 3 * D;
   const float dt = t / (float)n_steps;
   const float sdt = sqrtf(dt);
   const float disc = __expf(-r * t);
-  float acc[2 * NS];
-#pragma unroll
-  for (int q = 0; q < 2 * NS; ++q) acc[q] = 0.f;
-  for (int64_t p = (int64_t)blockIdx.x * kBlockK + threadIdx.x; p < n_paths;
-       p += (int64_t)gridDim.x * kBlockK) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nq = greeks ? NS : 1;
+  // per-wave sums in LDS (float64): each round of paths is summed over the
+  // wave and added there, so no per-thread accumulators stay live across
+  // the step loop (occupancy)
+  for (int e = threadIdx.x; e < (kBlockK / kWave) * 2 * NS; e += kBlockK)
+    (&red[0][0])[e] = 0.0;
+  __syncthreads();
+  // rounds are uniform over the block; lanes past n_paths compute a valid
+  // path and contribute zero
+  for (int64_t p0 = (int64_t)blockIdx.x * kBlockK; p0 < n_paths;
+       p0 += (int64_t)gridDim.x * kBlockK) {
+    const int64_t p = p0 + threadIdx.x;
+    const float valid = p < n_paths ? 1.f : 0.f;
     float zs[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) zs[i] = 0.f;
+#pragma unroll 1
     for (int st = 0; st < n_steps; ++st) {
       // D normals of this step: draw indices st*D .. st*D + D - 1
 #pragma unroll
@@ -65,7 +71,16 @@ basket_kernel(const float* __restrict__ prm, int n_steps, float t, float r,
         if (i + 3 < D) zs[i + 3] += z3;
       }
     }
-    float wv[D], s[D];
+    // the parameters are read here, after the steps: an opaque offset keeps
+    // their LDS loads from being hoisted out of the path loop, where the
+    // D (D + 7) / 2 values would stay live in VGPRs through every step
+    int o = 0;
+    asm volatile("" : "+s"(o));
+    const float* s0 = sp + o;
+    const float* sig = s0 + D;
+    const float* wt = s0 + 2 * D;
+    const float* L = s0 + 3 * D;
+    float wv[D], sv[D];
     float a = 0.f;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -73,34 +88,33 @@ basket_kernel(const float* __restrict__ prm, int n_steps, float t, float r,
 #pragma unroll
       for (int c = 0; c <= i; ++c) acc_w += L[i * D + c] * zs[c];
       wv[i] = acc_w * sdt;
-      s[i] = s0[i] * __expf((r - 0.5f * sig[i] * sig[i]) * t + sig[i] * wv[i]);
-      a += wt[i] * s[i];
+      sv[i] = s0[i] * __expf((r - 0.5f * sig[i] * sig[i]) * t + sig[i] * wv[i]);
+      a += wt[i] * sv[i];
     }
-    const float itm = a > strike ? 1.f : 0.f;
+    const float itm = a > strike ? valid : 0.f;
     const float pay = disc * itm * (a - strike);
     float g[NS];
     g[0] = pay;
     g[1] = -t * pay + disc * itm * t * a;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      g[2 + i] = disc * itm * wt[i] * s[i] / s0[i];
-      g[2 + D + i] = disc * itm * wt[i] * s[i] * (wv[i] - sig[i] * t);
+      g[2 + i] = disc * itm * wt[i] * sv[i] / s0[i];
+      g[2 + D + i] = disc * itm * wt[i] * sv[i] * (wv[i] - sig[i] * t);
     }
-    const int nq = greeks ? NS : 1;
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       if (q < nq) {
-        acc[q] += g[q];
-        acc[NS + q] += g[q] * g[q];
+        float v = g[q], v2 = g[q] * g[q];
+        for (int off = 32; off > 0; off >>= 1) {
+          v += __shfl_xor(v, off, 64);
+          v2 += __shfl_xor(v2, off, 64);
+        }
+        if (lane == 0) {
+          red[wave][q] += (double)v;
+          red[wave][NS + q] += (double)v2;
+        }
       }
     }
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < 2 * NS; ++q) {
-    float v = acc[q];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) red[wave][q] = (double)v;
   }
   __syncthreads();
   for (int q = threadIdx.x; q < 2 * NS; q += kBlockK) {
@@ -110,21 +124,32 @@ basket_kernel(const float* __restrict__ prm, int n_steps, float t, float r,
   }
 }
 
-__global__ void basket_finalize_kernel(const double* __restrict__ partial,
-                                       int n_blocks, int ns, int64_t n_paths,
-                                       double* __restrict__ out) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ns) return;
+// One workgroup per output: each thread sums a strided subset of the block
+// partials in order, then a fixed-shape LDS tree (deterministic).
+__global__ void __launch_bounds__(256)
+basket_finalize_kernel(const double* __restrict__ partial, int n_blocks,
+                       int ns, int64_t n_paths, double* __restrict__ out) {
+  __shared__ double r1[256], r2[256];
+  const int q = blockIdx.x;
   double s1 = 0, s2 = 0;
-  // unrolled: several blocks' loads in flight, the sums keep their order
-#pragma unroll 8
-  for (int b = 0; b < n_blocks; ++b) {
+  for (int b = threadIdx.x; b < n_blocks; b += 256) {
     s1 += partial[(int64_t)b * 2 * ns + q];
     s2 += partial[(int64_t)b * 2 * ns + ns + q];
   }
+  r1[threadIdx.x] = s1;
+  r2[threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
   const double n = (double)n_paths;
-  const double m = s1 / n;
-  double var = n > 1 ? (s2 - n * m * m) / (n - 1) : 0.0;
+  const double m = r1[0] / n;
+  double var = n > 1 ? (r2[0] - n * m * m) / (n - 1) : 0.0;
   out[q] = m;
   out[ns + q] = sqrt((var > 0 ? var : 0) / n);
 }
@@ -165,7 +190,7 @@ extern "C" int tqf_basket_call(const float* params, int32_t dim,
     TQF_BASKET(16)
   }
 #undef TQF_BASKET
-  hipLaunchKernelGGL(basket_finalize_kernel, (ns + 63) / 64, 64, 0, s, partial,
-                     n_blocks, ns, n_paths, out);
+  hipLaunchKernelGGL(basket_finalize_kernel, ns, 256, 0, s, partial, n_blocks,
+                     ns, n_paths, out);
   return check_launch("tqf_basket_call");
 }
