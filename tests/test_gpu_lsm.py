@@ -243,6 +243,29 @@ def test_bs_lsm_price_american_options():
   assert abs(got[2] - bs) <= 0.015 * bs
 
 
+def test_bs_lsm_price_batch_on_streams_equals_one_option_at_a_time():
+  # the options of a batch run their regressions on side streams (four,
+  # so five options reuse one): each price equals that option priced on
+  # its own, exactly, and a second batch call repeats the first
+  from tqf.experimental import american_option
+  spots = [90.0, 100.0, 110.0, 100.0, 95.0]
+  expiries = [1.0, 1.0, 0.5, 2.0, 1.5]
+  vols = [0.2, 0.25, 0.3, 0.2, 0.35]
+  calls = [False, False, True, False, True]
+  dfs = [float(np.exp(-0.05 * e)) for e in expiries]
+  kw = dict(num_samples=50000, num_exercise_times=30, seed=(5, 6))
+  batch = _np(american_option.bs_lsm_price(
+      spots, expiries, [100.0] * 5, vols, dfs, is_call_option=calls, **kw))
+  again = _np(american_option.bs_lsm_price(
+      spots, expiries, [100.0] * 5, vols, dfs, is_call_option=calls, **kw))
+  np.testing.assert_array_equal(batch, again)
+  for i in range(5):
+    one = _np(american_option.bs_lsm_price(
+        [spots[i]], [expiries[i]], [100.0], [vols[i]], [dfs[i]],
+        is_call_option=[calls[i]], **kw))[0]
+    assert batch[i] == one, (i, batch[i], one)
+
+
 def test_bs_lsm_price_matches_oracle_on_the_same_paths():
   # the GPU LSM (basis on ln S, payoff on S) against the oracle's
   # restatement of least_square_mc on the engine's own paths

@@ -1723,7 +1723,8 @@ __device__ __forceinline__ double small_ratio(const SmallDate& d,
 // = 0 skips the update (the first regressed date follows the terminal
 // payoff); n_calib = 0 skips the power sums; do_stats = 0 the centring
 // sums.  The centrings of dates r + 1 and r come from device slots
-// (mean_u / mean_m, written by the solves).
+// (mean_u / mean_m, written by the solves).  cash holds each path's whole
+// value (see small_date_chain).
 template <int DEG>
 __global__ void __launch_bounds__(256)
 lsm_small_pass_kernel(LsmArgs a, SmallDate du, const double* __restrict__ cu,
@@ -1731,7 +1732,7 @@ lsm_small_pass_kernel(LsmArgs a, SmallDate du, const double* __restrict__ cu,
                       SmallDate dm, const double* __restrict__ cm,
                       int64_t n_calib, SmallDate ds, int do_stats,
                       int64_t rows_per_block, double* __restrict__ cash,
-                      double* __restrict__ vals, double* __restrict__ partial) {
+                      double* __restrict__ partial) {
   constexpr int NX = 2 * DEG + 1, NM = NX + DEG + 1, W = NM + 2;
   __shared__ double red[4][W];
   double acc[W];
@@ -1751,8 +1752,7 @@ lsm_small_pass_kernel(LsmArgs a, SmallDate du, const double* __restrict__ cu,
   // before its arithmetic
   for (int64_t base = 0; base < rows_per_block; base += 256 * kSmallRows) {
     int64_t p[kSmallRows];
-    double cs[kSmallRows], vl[kSmallRows], xu[kSmallRows], xm[kSmallRows],
-        xs[kSmallRows];
+    double cs[kSmallRows], xu[kSmallRows], xm[kSmallRows], xs[kSmallRows];
 #pragma unroll
     for (int i = 0; i < kSmallRows; ++i) {
       p[i] = p0 + base + 256 * i + threadIdx.x;
@@ -1760,7 +1760,6 @@ lsm_small_pass_kernel(LsmArgs a, SmallDate du, const double* __restrict__ cu,
                       p[i] < n;
       if (!ok) p[i] = -1;
       cs[i] = ok ? cash[p[i]] : 0.0;
-      vl[i] = ok ? vals[p[i]] : 0.0;
       xu[i] = ok && do_update ? du.col[p[i] * sp] : 1.0;
       xm[i] = ok && p[i] < n_calib ? dm.col[p[i] * sp] : 1.0;
       xs[i] = ok && do_stats ? ds.col[p[i] * sp] : 1.0;
@@ -1776,20 +1775,13 @@ lsm_small_pass_kernel(LsmArgs a, SmallDate du, const double* __restrict__ cu,
 #pragma unroll
         for (int m = DEG - 1; m >= 0; --m) h = fma(h, c, dn[m]);
         const double cont = h > 0 ? h : 0;
-        if (ex > cont) {
-          vl[i] = 0;
-          cs[i] = ex;
-        } else {
-          vl[i] = small_ratio(du, p[i]) * (cs[i] + vl[i]);
-          cs[i] = 0;
-        }
+        cs[i] = ex > cont ? ex : small_ratio(du, p[i]) * cs[i];
         cash[p[i]] = cs[i];
-        vals[p[i]] = vl[i];
       }
       if (p[i] < n_calib && itm1(am, xm[i], p[i])) {
         // in-the-money calibration rows only
         const double c = (basis_input(a, xm[i]) - mm) * sm;
-        double v = 1.0, u = small_ratio(dm, p[i]) * (cs[i] + vl[i]);
+        double v = 1.0, u = small_ratio(dm, p[i]) * cs[i];
 #pragma unroll
         for (int m = 0; m < NX; ++m) {
           acc[m] += v;
@@ -2014,14 +2006,19 @@ extern "C" int64_t tqf_lsm_workspace_size(int64_t n_paths, int32_t n_basis) {
 // The one-asset chain of lsm_small_pass_kernel / lsm_small_solve_kernel
 // for the regressed dates r = n_ex - 2 .. 0 (exercise index ex_idx[r],
 // discount ratio df[r + 2] / df[r + 1]), after the terminal payoff of date
-// n_ex - 1 is in cash / vals.  slots holds three rotating centrings (mean
+// n_ex - 1 is in cash (vals zero).  The chain keeps each path's whole value
+// in cash: the reference's pair (cashflow, discounted later values) always
+// has one member zero after an update (exercise: (payoff, 0); hold:
+// (0, ratio (cashflow + values))), so their sum is exact and is all the
+// regression and the price read; vals is left at zero, and the final
+// cash + vals is the same number.  slots holds three rotating centrings (mean
 // at [0], scale at [kMaxAssets]; date r uses slot r % 3); dense the
 // coefficients of the date solved last.
 template <int DEG>
 int small_date_chain(const tqf_lsm* l, const int32_t* ex_idx, int n_ex,
                      const double* df, int64_t n_calib, double* partial,
                      double* slots, double* dense, double* cash,
-                     double* vals, int32_t* ranks, hipStream_t s) {
+                     int32_t* ranks, hipStream_t s) {
   const int64_t n = l->n_paths;
   const LsmArgs a = make_args(l, 0);
   // rows per workgroup: a multiple of one round (256 kSmallRows rows)
@@ -2051,7 +2048,7 @@ int small_date_chain(const tqf_lsm* l, const int32_t* ex_idx, int n_ex,
                        r_upd >= 0 ? 1 : 0, dm,
                        slot(r_mom >= 0 ? r_mom : r_any),
                        r_mom >= 0 ? n_calib : (int64_t)0, dst,
-                       r_stat >= 0 ? 1 : 0, rpb, cash, vals, partial);
+                       r_stat >= 0 ? 1 : 0, rpb, cash, partial);
   };
   auto solve = [&](int r, int r_stat) {
     hipLaunchKernelGGL(lsm_small_solve_kernel<DEG>, 1, 256, 0, s, partial,
@@ -2170,7 +2167,7 @@ extern "C" int tqf_lsm_price(const tqf_lsm* l, const int32_t* ex_idx,
 #define TQF_SMALL_CHAIN(DEG)                                               \
   case DEG:                                                                \
     rc = small_date_chain<DEG>(l, ex_idx, n_ex, df, n_calib, partial,      \
-                               chain, dense, cash, vals, ranks, s);        \
+                               chain, dense, cash, ranks, s);              \
     break;
       TQF_SMALL_CHAIN(0)
       TQF_SMALL_CHAIN(1)

@@ -14,7 +14,11 @@ Here the unit paths come from the exact GBM kernel (tqf_gbm_exact) with
 the engine's antithetic Philox stream (the reference's
 STATELESS_ANTITHETIC), the per-option mapping to ln(S / S0) is one
 elementwise pass on the GPU, and the regression runs in the FP64 LSM
-kernels (tqf_lsm_price on log prices: basis on ln S, payoff on S).  A user basis_fn falls back to the torch LSM with the same
+kernels (tqf_lsm_price on log prices: basis on ln S, payoff on S).  The
+options' regressions are independent: each runs on its own stream (up to
+four, default priority), so one option's single-workgroup solves overlap
+the other options' bandwidth-bound date passes, and the prices come back
+in one copy.  A user basis_fn falls back to the torch LSM with the same
 log-path / payoff split.
 """
 import numpy as np
@@ -27,6 +31,19 @@ from tqf.models import longstaff_schwartz as lsm_lib
 from tqf.models import utils
 
 __all__ = ['bs_lsm_price']
+
+
+_STREAMS = {}
+_MAX_STREAMS = 4
+
+
+def _streams(dev, n):
+  """Up to _MAX_STREAMS side streams of `dev` (default priority), cached."""
+  key = str(dev)
+  pool = _STREAMS.setdefault(key, [])
+  while len(pool) < min(n, _MAX_STREAMS):
+    pool.append(torch.cuda.Stream(device=dev))
+  return pool[:min(n, _MAX_STREAMS)]
 
 
 def _vec(v, n, dtype):
@@ -69,36 +86,46 @@ def bs_lsm_price(spots, expiry_times, strikes, volatility, discount_factors,
   if basis_fn is None:
     basis_fn = lsm_lib.make_polynomial_basis(2)
   bspec = getattr(basis_fn, '_tqf_basis', None)
+  if bspec is not None and bspec[0] == 'poly':
+    main = torch.cuda.current_stream(dev)
+    streams = _streams(dev, n_opt)
+    outs = []
+    for b in range(n_opt):
+      st = streams[b % len(streams)]
+      st.wait_stream(main)  # the unit paths are written on the caller's
+      with torch.cuda.stream(st):
+        dfs = np.exp(-rate[b] * times * expiry[b])
+        kind = (lsm_lib.PAYOFF_MAX_CALL if call[b]
+                else lsm_lib.PAYOFF_BASKET_PUT)
+        # ln(S / S0) in one pass: the LSM kernels take log prices
+        # (basis on ln S, payoff on exp)
+        log_s_t = torch.add((rate[b] - 0.5 * vol[b]**2) * (t_t * expiry[b]),
+                            w_t, alpha=float(vol[b] * np.sqrt(expiry[b])))
+        # [N, T, 1] view of [T][1][N] memory
+        paths = log_s_t.unsqueeze(1).permute(2, 0, 1)
+        outs.append(lsm_lib._native_lsm(
+            paths, ex, (kind, np.array([strike[b]])), bspec[1],
+            np.concatenate([[1.0], dfs]).reshape(-1, 1, 1),
+            num_calibration_samples, torch.float64, dev, basis_log=2))
+    for st in streams:
+      main.wait_stream(st)
+    prices = torch.cat(outs).cpu().numpy()
+    return torch.as_tensor(prices * spots_np, dtype=dtype, device=dev)
   prices = []
   for b in range(n_opt):
     dfs = np.exp(-rate[b] * times * expiry[b])
-    kind = lsm_lib.PAYOFF_MAX_CALL if call[b] else lsm_lib.PAYOFF_BASKET_PUT
-    if bspec is not None and bspec[0] == 'poly':
-      # ln(S / S0) in one pass: the LSM kernels take log prices
-      # (basis on ln S, payoff on exp)
-      log_s_t = torch.add((rate[b] - 0.5 * vol[b]**2) * (t_t * expiry[b]),
-                          w_t, alpha=float(vol[b] * np.sqrt(expiry[b])))
-      # [N, T, 1] view of [T][1][N] memory
-      paths = log_s_t.unsqueeze(1).permute(2, 0, 1)
-      p = lsm_lib._native_lsm(
-          paths, ex, (kind, np.array([strike[b]])), bspec[1],
-          np.concatenate([[1.0], dfs]).reshape(-1, 1, 1),
-          num_calibration_samples,
-          torch.float64, dev, basis_log=2)
-      prices.append(float(p[0]))
-    else:
-      log_s = ((vol[b] * np.sqrt(expiry[b])) * w +
-               (rate[b] - 0.5 * vol[b]**2) * (t * expiry[b]))
-      k, is_call = strike[b], bool(call[b])
+    log_s = ((vol[b] * np.sqrt(expiry[b])) * w +
+             (rate[b] - 0.5 * vol[b]**2) * (t * expiry[b]))
+    k, is_call = strike[b], bool(call[b])
 
-      def payoff_fn(log_paths, time_index, k=k, is_call=is_call):
-        s = torch.exp(log_paths[:, time_index, 0])
-        v = torch.relu(s - k) if is_call else torch.relu(k - s)
-        return v.unsqueeze(-1)
-      p = lsm_lib.least_square_mc(log_s.unsqueeze(-1), ex, payoff_fn,
-                                  basis_fn, discount_factors=dfs,
-                                  num_calibration_samples=num_calibration_samples,
-                                  dtype=torch.float64)
-      prices.append(float(p.reshape(-1)[0]))
+    def payoff_fn(log_paths, time_index, k=k, is_call=is_call):
+      s = torch.exp(log_paths[:, time_index, 0])
+      v = torch.relu(s - k) if is_call else torch.relu(k - s)
+      return v.unsqueeze(-1)
+    p = lsm_lib.least_square_mc(log_s.unsqueeze(-1), ex, payoff_fn,
+                                basis_fn, discount_factors=dfs,
+                                num_calibration_samples=num_calibration_samples,
+                                dtype=torch.float64)
+    prices.append(float(p.reshape(-1)[0]))
   out = torch.as_tensor(np.asarray(prices) * spots_np, dtype=dtype, device=dev)
   return out
