@@ -57,6 +57,33 @@ def test_sobol_integers_bit_exact(dim, n, skip):
                                 want)
 
 
+# whole-row tiles run on a persistent grid (three workgroups per CU), each
+# workgroup advancing its item starts from tile to tile: sizes with more
+# tiles than workgroups (16-row tiles of 730 dimensions, 4096-row tiles of
+# one), every row compared
+@pytest.mark.parametrize('dim,n,skip', [(730, 2**15, 7), (1, 2**23, 3),
+                                        (37, 2**18, 1000)])
+def test_sobol_tiles_advanced_across_a_workgroup_bit_exact(dim, n, skip):
+  print('sobol tiles dim=%d n=%d skip=%d' % (dim, n, skip), flush=True)
+  got, _ = tsobol.sample_int(dim, n, skip)
+  want, _ = osobol.sample_int(dim, n, skip)
+  np.testing.assert_array_equal(_np(got).view(np.uint32).astype(np.uint64),
+                                want)
+
+
+def test_sobol_sample_full_size_rows_match_the_oracle():
+  # the C2 draw set, sobol.sample(730, 2^20) float32 (about 86 tiles per
+  # workgroup): rows at tile boundaries and spread over the range against
+  # the oracle's row of the same index
+  got = _np(tqf.math.random.sobol.sample(730, 2**20))
+  rows = [0, 1, 14, 15, 16, 17, 31, 32, 1023, 1024, 2**19 - 1, 2**19,
+          2**20 - 17, 2**20 - 16, 2**20 - 1]
+  rows += list(np.random.RandomState(4).randint(0, 2**20, 16))
+  for r in rows:
+    want = osobol.sample(730, 1, int(r))[0]
+    np.testing.assert_array_equal(got[r], want, err_msg='row %d' % r)
+
+
 def test_sobol_sample_goldens_and_dtypes():
   g = load_golden('sobol_golden.json')
   e = g['random_ops_dim2_first5']

@@ -29,6 +29,17 @@ int check_launch(const char* what);
 
 constexpr int kWave = 64;
 
+// Workgroup barrier that orders LDS only: the fences are limited to the
+// local address space, so the wave's outstanding global stores are not
+// waited for (__syncthreads waits for them).  For loops that stage data in
+// LDS and stream it out: the next stage may overwrite the LDS buffer while
+// the previous stage's stores are still in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ------------------------------------------------------------ Philox4x32 --
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of
 // (hi, lo) = mulhilo(M, c) with a Weyl key schedule.
