@@ -133,6 +133,29 @@ def test_record_plan_replays_reference_loop(mask, k):
   assert got == want
 
 
+def test_record_plan_random_masks_replay_reference_loop():
+  # the plan is computed from prefix sums of the mask: random masks, counts
+  # (including none and more than the grid reaches) and starting counts
+  # against the replayed loop, slot by slot, and the steps it executes
+  rs = np.random.RandomState(11)
+  for _ in range(3000):
+    steps = rs.randint(0, 30)
+    mask = rs.rand(steps + 1) < rs.rand()
+    k = rs.randint(1, steps + 4)
+    iw = None if rs.rand() < 0.5 else int(rs.randint(0, 3))
+    slots, init_slot, executed, filled = utils.record_plan(mask, k, iw)
+    want, n_exec = _simulate_records(mask, k, iw)
+    assert executed == n_exec
+    got = [None] * k
+    if init_slot >= 0:
+      got[init_slot] = -1
+    for i, s in enumerate(slots):
+      if s >= 0:
+        got[s] = i
+    assert got == want, (mask, k, iw)
+    assert filled == sorted(j for j in range(k) if want[j] is not None)
+
+
 def test_qe_step_coefficients_reproduce_oracle_moments():
   t = np.array([0.0, 0.1, 0.25, 0.25, 0.6])
   c = qe_step_coefficients(t, 1.5, 0.05, 0.6, -0.4, 1e-6, torch.float64)

@@ -141,7 +141,78 @@ _RNG_CODES = {'draws': _native.RNG_DRAWS, 'philox': _native.RNG_PHILOX,
               'sobol_shifted': _native.RNG_SOBOL_SHIFTED}
 
 
-def _model_struct(spec, dtype, dev, keep, step_times=None, step_slots=None):
+_NP_OF = {torch.float32: np.float32, torch.float64: np.float64,
+          torch.int32: np.int32}
+
+
+class Staging:
+  """The small per-call host arrays of a launch (grid, step coefficients,
+  shift, strikes, discounts) uploaded together: one pinned staging buffer
+  per device and stream (rewritten once its previous upload has completed)
+  and one asynchronous copy into one device buffer, each array a view of
+  it.  `add` records an array and the struct fields that take its device
+  address; `flush` uploads and sets them."""
+
+  _host = {}
+
+  def __init__(self, dev):
+    self.dev = dev
+    self.items = []
+
+  def add(self, arr, tdtype, patches=()):
+    a = np.ascontiguousarray(np.asarray(arr).astype(_NP_OF[tdtype],
+                                                    copy=False))
+    self.items.append((a, tdtype, list(patches)))
+
+  def flush(self, keep):
+    if not self.items:
+      return []
+    offs, total = [], 0
+    for a, _, _ in self.items:
+      total = (total + 255) & ~255
+      offs.append(total)
+      total += a.nbytes
+    total = max(total, 1)
+    stream = torch.cuda.current_stream(self.dev)
+    key = (str(self.dev), stream.cuda_stream)
+    ent = Staging._host.get(key)
+    if ent is None or ent['buf'].numel() < total:
+      cap = 1 << max(16, (total - 1).bit_length())
+      ent = {'buf': torch.empty(cap, dtype=torch.uint8, pin_memory=True),
+             'done': None}
+      Staging._host[key] = ent
+    if ent['done'] is not None:
+      ent['done'].synchronize()
+    hb = ent['buf'].numpy()
+    for (a, _, _), o in zip(self.items, offs):
+      hb[o:o + a.nbytes] = a.reshape(-1).view(np.uint8)
+    dbuf = torch.empty(total, dtype=torch.uint8, device=self.dev)
+    dbuf.copy_(ent['buf'][:total], non_blocking=True)
+    ent['done'] = torch.cuda.Event()
+    ent['done'].record(stream)
+    keep.append(dbuf)
+    outs = []
+    for (a, td, patches), o in zip(self.items, offs):
+      t = dbuf[o:o + a.nbytes].view(td)
+      outs.append(t)
+      for obj, field in patches:
+        setattr(obj, field, t.data_ptr())
+    self.items = []
+    return outs
+
+
+def _model_struct(spec, dtype, dev, keep, step_times=None, step_slots=None,
+                  stage=None):
+  """The model descriptor; its device arrays go through `stage` when one is
+  given (their addresses are set by stage.flush), else are uploaded here."""
+  def upload(arr, tdtype, field):
+    if stage is not None:
+      stage.add(arr, tdtype, [(m, field)])
+      return
+    t = torch.as_tensor(np.ascontiguousarray(arr), device=dev).to(tdtype)
+    keep.append(t)
+    setattr(m, field, t.data_ptr())
+
   m = _native.Model()
   d = model_dim(spec)
   m.dim = d
@@ -175,15 +246,10 @@ def _model_struct(spec, dtype, dev, keep, step_times=None, step_slots=None):
       bits = coef[:, 5].view(ib)
       bits[:] = -1
       bits[:n] = np.asarray(step_slots[:n], ib)
-      ct = torch.from_numpy(np.ascontiguousarray(coef)).to(dev)
-      keep.append(ct)
-      m.step_coef = ct.data_ptr()
+      upload(coef, dtype, 'step_coef')
   elif spec[0] == 'heston_qe':
     m.kind = _native.MODEL_HESTON_QE
-    coef = torch.as_tensor(np.ascontiguousarray(spec[1]), dtype=dtype,
-                           device=dev).contiguous()
-    keep.append(coef)
-    m.step_coef = coef.data_ptr()
+    upload(utils.to_numpy(spec[1]), dtype, 'step_coef')
   elif spec[0] == 'hull_white':
     # spec[1](step_times) -> (decay, mean, sd, f(0, t_{i+1}), f(0, t_0)) of
     # the exact steps between consecutive grid times
@@ -203,19 +269,15 @@ def _model_struct(spec, dtype, dev, keep, step_times=None, step_slots=None):
     bits = coef[:, 5].view(ib)
     bits[:] = -1
     bits[:n] = np.asarray(step_slots[:n], ib)
-    ct = torch.from_numpy(np.ascontiguousarray(coef)).to(dev)
-    keep.append(ct)
-    m.step_coef = ct.data_ptr()
+    upload(coef, dtype, 'step_coef')
   elif spec[0] == 'local_vol':
     tab = spec[1]
     m.kind = _native.MODEL_LOCAL_VOL
     m.p[0] = tab.drift_rate
     m.p[1] = 0.0 if tab.outside is None else 1.0
     rows = tab.device_rows()
-    vol = torch.as_tensor(rows, dtype=dtype, device=dev).contiguous()
-    tt = torch.as_tensor(tab.times, dtype=dtype, device=dev).contiguous()
-    keep.extend([vol, tt])
-    m.lv_vol, m.lv_t = vol.data_ptr(), tt.data_ptr()
+    upload(rows, dtype, 'lv_vol')
+    upload(utils.to_numpy(tab.times), dtype, 'lv_t')
     m.lv_x0, m.lv_dx = tab.x0, tab.dx
     m.lv_nt, m.lv_nx = rows.shape
   else:
@@ -311,24 +373,23 @@ def prepare(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
   n_rng_steps = dt.shape[0]
   keep = []
 
-  def dev_t(a, dt_=dtype):
-    t = torch.as_tensor(np.ascontiguousarray(a), device=dev).to(dt_)
-    keep.append(t)
-    return t
+  # the small arrays go up in one staged copy (addresses set by flush)
+  stage = Staging(dev)
 
   g = _native.Grid()
-  g.dt = dev_t(dt[:max(executed, 1)]).data_ptr()
-  g.sqrt_dt = dev_t(sqrt_dt[:max(executed, 1)]).data_ptr()
-  g.t_next = dev_t(all_times[1:][:max(executed, 1)]).data_ptr()
-  g.slot = dev_t(np.concatenate([slots, [-1]]).astype(np.int32),
-                 torch.int32).data_ptr()
+  stage.add(dt[:max(executed, 1)], dtype, [(g, 'dt')])
+  stage.add(sqrt_dt[:max(executed, 1)], dtype, [(g, 'sqrt_dt')])
+  stage.add(all_times[1:][:max(executed, 1)], dtype, [(g, 't_next')])
+  stage.add(np.concatenate([slots, [-1]]).astype(np.int32), torch.int32,
+            [(g, 'slot')])
   g.n_steps = executed
   g.n_slots = int(num_requested)
   g.init_slot = init_slot
   g.n_rng_steps = max(n_rng_steps, executed)
 
   m = _model_struct(spec, dtype, dev, keep,
-                    step_times=all_times[:executed + 1], step_slots=slots)
+                    step_times=all_times[:executed + 1], step_slots=slots,
+                    stage=stage)
 
   r = _native.Rng()
   r.kind = _RNG_CODES[rng.kind]
@@ -364,28 +425,37 @@ def prepare(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
     lo = sobol_lib.device_lo_table(ndims, nd, group=d, dev=dev)
     r.sobol_v, r.sobol_lo = v.data_ptr(), lo.data_ptr()
     if rng.shift is not None:
-      sh = dev_t(np.asarray(rng.shift, np.uint64).astype(np.uint32).view(
-          np.int32), torch.int32)
-      if sh.numel() < ndims:
+      sh = np.asarray(rng.shift, np.uint64).astype(np.uint32).view(np.int32)
+      if sh.size < ndims:
         raise ValueError('digital shift has %d entries, need %d' %
-                         (sh.numel(), ndims))
-      r.sobol_shift = sh.data_ptr()
+                         (sh.size, ndims))
+      stage.add(sh, torch.int32, [(r, 'sobol_shift')])
     r.sobol_first_index = first
     r.sobol_nd = nd
     r.sobol_ndims = ndims
 
-  x0_t = torch.as_tensor(x0, dtype=dtype, device=dev)
-  if x0_t.dim() == 0:
-    x0_t = x0_t.reshape(1)
-  if x0_t.dim() == 1:
-    if x0_t.shape[0] != d:
-      x0_t = x0_t.expand(d) if x0_t.shape[0] == 1 else x0_t
+  x0_t = None
+  if not torch.is_tensor(x0) and np.ndim(x0) <= 1:
+    # one state for every path: staged with the other small arrays
+    x0_h = np.asarray(utils.to_numpy(x0), npd).reshape(-1)
+    if x0_h.shape[0] == 1 and d != 1:
+      x0_h = np.broadcast_to(x0_h, (d,))
     x0_stride = 0
+    x0_at = len(stage.items)
+    stage.add(x0_h, dtype)
   else:
-    x0_t = x0_t.expand(num_samples, d)
-    x0_stride = d
-  x0_t = x0_t.contiguous()
-  keep.append(x0_t)
+    x0_t = torch.as_tensor(x0, dtype=dtype, device=dev)
+    if x0_t.dim() == 0:
+      x0_t = x0_t.reshape(1)
+    if x0_t.dim() == 1:
+      if x0_t.shape[0] != d:
+        x0_t = x0_t.expand(d) if x0_t.shape[0] == 1 else x0_t
+      x0_stride = 0
+    else:
+      x0_t = x0_t.expand(num_samples, d)
+      x0_stride = d
+    x0_t = x0_t.contiguous()
+    keep.append(x0_t)
 
   o = _native.Output()
   result = None
@@ -408,23 +478,21 @@ def prepare(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
     if len(filled) != int(num_requested):
       raise ValueError('every requested time must be reached for payoffs')
     o.mode = _native.OUT_PAYOFF
-    k = torch.as_tensor(np.asarray(strikes, np.float64), dtype=torch.float64,
-                        device=dev).contiguous()
-    keep.append(k)
-    o.n_strikes = k.numel()
-    o.strikes = k.data_ptr()
+    kn = np.asarray(utils.to_numpy(strikes), np.float64).reshape(-1)
+    stage.add(kn, torch.float64, [(o, 'strikes')])
+    o.n_strikes = kn.size
     disc = np.ones(int(num_requested)) if discount is None else discount
-    dd = dev_t(np.asarray(disc, np.float64), torch.float64)
-    o.discount = dd.data_ptr()
+    stage.add(np.asarray(utils.to_numpy(disc), np.float64), torch.float64,
+              [(o, 'discount')])
     ws_bytes = _native.lib().tqf_euler_workspace_size(
-        num_samples, int(num_requested), k.numel())
+        num_samples, int(num_requested), kn.size)
     ws = torch.empty(max(ws_bytes // 8, 1), dtype=torch.float64, device=dev)
     keep.append(ws)
     o.workspace = ws.data_ptr()
-    prices = torch.empty((int(num_requested), k.numel()), dtype=torch.float64,
+    prices = torch.empty((int(num_requested), kn.size), dtype=torch.float64,
                          device=dev)
     ses = torch.empty_like(prices)
-    sums = torch.empty((int(num_requested), k.numel(), 2),
+    sums = torch.empty((int(num_requested), kn.size, 2),
                        dtype=torch.float64, device=dev)
     o.prices, o.stderrs, o.sums = (prices.data_ptr(), ses.data_ptr(),
                                    sums.data_ptr())
@@ -432,6 +500,9 @@ def prepare(spec, all_times, keep_mask, num_requested, x0, num_samples, rng,
   else:
     raise ValueError('unknown mode %r' % mode)
 
+  staged = stage.flush(keep)
+  if x0_t is None:
+    x0_t = staged[x0_at]
   if isinstance(result, dict):
     result['_keep'] = keep
   else:

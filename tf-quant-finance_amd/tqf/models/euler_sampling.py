@@ -96,11 +96,19 @@ def sample(dim, drift_fn, volatility_fn, times, time_step=None,
   num_samples = int(num_samples)
 
   if initial_state is None:
-    initial_state = torch.zeros(dim, dtype=dtype, device=dev)
-  x0 = utils.device_tensor(initial_state, dtype, dev)
+    initial_state = np.zeros(dim, npd)
+  if torch.is_tensor(initial_state):
+    x0 = utils.device_tensor(initial_state, dtype, dev)
+  else:
+    # host values stay on the host: the fused launch uploads them with its
+    # other small arrays in one asynchronous copy
+    x0 = np.asarray(utils.to_numpy(initial_state), npd)
+
+  def on_dev(a):
+    return a if torch.is_tensor(a) else torch.as_tensor(a, device=dev)
   # batch_shape = shape(initial_state)[:-2] (euler_sampling.py:251), or the
   # leading axes of batched normal_draws
-  if x0.dim() > 2:
+  if x0.ndim > 2:
     batch = tuple(torch.broadcast_shapes(tuple(x0.shape[:-2]), batch))
 
   spec = engine.spec_of(drift_fn, volatility_fn)
@@ -109,8 +117,8 @@ def sample(dim, drift_fn, volatility_fn, times, time_step=None,
     # a batch of initial states (or of draws) for one model: the batch
     # members are consecutive blocks of paths of a single launch
     nb = int(np.prod(batch))
-    xb = (x0 + torch.zeros(batch + (num_samples, dim), dtype=dtype,
-                           device=dev)).reshape(nb * num_samples, dim)
+    xb = (on_dev(x0) + torch.zeros(batch + (num_samples, dim), dtype=dtype,
+                                   device=dev)).reshape(nb * num_samples, dim)
     if draws is not None:
       draws = draws.expand(batch + tuple(draws.shape[-3:])).reshape(
           (nb * num_samples,) + tuple(draws.shape[-2:]))
@@ -126,8 +134,8 @@ def sample(dim, drift_fn, volatility_fn, times, time_step=None,
     if out is not None:
       return out
   return _generic_loop(dim, drift_fn, volatility_fn, all_times, keep_mask, k,
-                       x0, num_samples, random_type, seed, skip, draws, dtype,
-                       dev, batch)
+                       on_dev(x0), num_samples, random_type, seed, skip, draws,
+                       dtype, dev, batch)
 
 
 def _fused(dim, spec, all_times, keep_mask, k, x0, num_samples, random_type,
@@ -156,6 +164,8 @@ def _fused(dim, spec, all_times, keep_mask, k, x0, num_samples, random_type,
           num_sample_paths=num_samples, random_type=random_type, seed=seed,
           skip=skip, dtype=dtype).permute(1, 0, 2)
     rng = engine.RngSpec.from_random_type(random_type, seed, skip, draws)
+    if not torch.is_tensor(x0):
+      x0 = torch.as_tensor(x0, device=dev)
     return differentiable.sample_paths(spec, all_times, keep_mask, k, x0,
                                        num_samples, rng, dtype, dev)
   if watch_params is not None:

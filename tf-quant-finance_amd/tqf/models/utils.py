@@ -121,23 +121,21 @@ def record_plan(keep_mask, num_requested, initial_written=None):
   steps = keep.shape[0] - 1
   k = int(num_requested)
   record = k != 1
-  written = int(keep[0]) if initial_written is None else int(initial_written)
-  writer = {0: -1}  # slot 0 starts as the initial state either way
-  i = 0
-  while i < steps and written < k:
-    slot = written if record else 0
-    writer[slot] = i
-    written += int(keep[i + 1])
-    i += 1
-  executed = i
-  slots = np.full(max(executed, 0), -1, np.int32)
-  init_slot = -1
-  for s, w in writer.items():
-    if w >= 0:
-      slots[w] = s
-    else:
-      init_slot = s
-  filled = sorted(writer)
+  written0 = int(keep[0]) if initial_written is None else int(initial_written)
+  # the count at the top of step i: written0 + keep[1] + ... + keep[i]
+  cnt = written0 + np.concatenate(
+      [[0], np.cumsum(keep[1:max(steps, 1)], dtype=np.int64)])[:steps]
+  stop = np.flatnonzero(cnt >= k)
+  executed = int(stop[0]) if stop.size else steps
+  slot = cnt[:executed] if record else np.zeros(executed, np.int64)
+  # the count never decreases, so a slot's writes are consecutive steps and
+  # its final writer is the last of them
+  last = np.ones(executed, bool)
+  last[:-1] = slot[1:] != slot[:-1]
+  slots = np.where(last, slot, -1).astype(np.int32)
+  # slot 0 starts as the initial state unless a step writes it
+  init_slot = -1 if executed and slot[0] == 0 else 0
+  filled = sorted({0} | set(slot.tolist()))
   return slots, init_slot, executed, filled
 
 
