@@ -55,34 +55,40 @@ def run_exact(times, drift, vol, chol, s0, num_samples, rng, dtype,
   slots, init_slot, executed, _ = utils.record_plan(keep_mask, k)
   d = int(np.asarray(drift).shape[0])
   keep = []
-
-  def dv(a, t=dtype):
-    x = torch.as_tensor(np.ascontiguousarray(a), device=dev).to(t)
-    keep.append(x)
-    return x
+  # the small arrays go up in one staged copy (addresses set by flush)
+  stage = engine.Staging(dev)
 
   g = _native.Gbm()
   g.dim = d
-  g.drift = dv(np.asarray(drift, np.float64)).data_ptr()
-  g.vol = dv(np.asarray(vol, np.float64)).data_ptr()
-  g.chol = 0 if chol is None else dv(np.asarray(chol, np.float64)).data_ptr()
-  s0_t = utils.device_tensor(s0, dtype, dev)
-  if s0_t.dim() == 0:
-    s0_t = s0_t.reshape(1)
-  if s0_t.dim() == 1:
-    s0_t = s0_t.expand(d).contiguous()
+  stage.add(utils.to_numpy(drift, np.float64), dtype, [(g, 'drift')])
+  stage.add(utils.to_numpy(vol, np.float64), dtype, [(g, 'vol')])
+  if chol is None:
+    g.chol = 0
+  else:
+    stage.add(utils.to_numpy(chol, np.float64), dtype, [(g, 'chol')])
+  if not torch.is_tensor(s0) and np.ndim(s0) <= 1:
+    # one state for every path
+    s0_h = np.asarray(utils.to_numpy(s0), utils.np_dtype(dtype)).reshape(-1)
+    stage.add(np.broadcast_to(s0_h, (d,)), dtype, [(g, 's0')])
     g.s0_stride_path = 0
   else:
-    s0_t = s0_t.expand(num_samples, d).contiguous()
-    g.s0_stride_path = d
-  keep.append(s0_t)
-  g.s0 = s0_t.data_ptr()
+    s0_t = utils.device_tensor(s0, dtype, dev)
+    if s0_t.dim() == 0:
+      s0_t = s0_t.reshape(1)
+    if s0_t.dim() == 1:
+      s0_t = s0_t.expand(d).contiguous()
+      g.s0_stride_path = 0
+    else:
+      s0_t = s0_t.expand(num_samples, d).contiguous()
+      g.s0_stride_path = d
+    keep.append(s0_t)
+    g.s0 = s0_t.data_ptr()
 
   grid = _native.Grid()
-  grid.dt = dv(dt[:max(executed, 1)]).data_ptr()
-  grid.sqrt_dt = dv(sqrt_dt[:max(executed, 1)]).data_ptr()
-  grid.slot = dv(np.concatenate([slots, [-1]]).astype(np.int32),
-                 torch.int32).data_ptr()
+  stage.add(dt[:max(executed, 1)], dtype, [(grid, 'dt')])
+  stage.add(sqrt_dt[:max(executed, 1)], dtype, [(grid, 'sqrt_dt')])
+  stage.add(np.concatenate([slots, [-1]]).astype(np.int32), torch.int32,
+            [(grid, 'slot')])
   grid.n_steps, grid.n_slots, grid.init_slot = executed, k, init_slot
   grid.n_rng_steps = dt.shape[0]
 
@@ -112,9 +118,8 @@ def run_exact(times, drift, vol, chol, s0, num_samples, rng, dtype,
     r.sobol_first_index = rng.skip + 1 if rng.kind == 'sobol' else rng.first_index
     r.sobol_nd, r.sobol_ndims = nd, ndims
     if rng.shift is not None:
-      sh = dv(np.asarray(rng.shift, np.uint64).astype(np.uint32).view(np.int32),
-              torch.int32)
-      r.sobol_shift = sh.data_ptr()
+      stage.add(np.asarray(rng.shift, np.uint64).astype(np.uint32).view(
+          np.int32), torch.int32, [(r, 'sobol_shift')])
   # every slot is written unless a requested time lies past the executed
   # steps; only then does the output need zeros first
   written = set(int(v) for v in slots[:executed] if v >= 0)
@@ -127,6 +132,7 @@ def run_exact(times, drift, vol, chol, s0, num_samples, rng, dtype,
   else:
     out = alloc((k, d, num_samples), dtype=dtype, device=dev)
     code = _native.OUT_STEP_MAJOR
+  stage.flush(keep)
   _native.call('tqf_gbm_exact', ctypes.byref(g), ctypes.byref(grid),
                ctypes.byref(r), ctypes.c_int32(code),
                ctypes.c_void_p(out.data_ptr()), ctypes.c_int64(num_samples),
