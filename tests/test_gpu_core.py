@@ -756,6 +756,21 @@ def test_halton_matches_oracle(dtype, rtol):
   np.testing.assert_array_equal(_np(part), _np(got)[1000:])
 
 
+def test_halton_full_size_rows_match_the_oracle():
+  # halton.sample(100, 2^20) float32 (64-point tiles, one axis per wave):
+  # rows at tile boundaries and spread over the range against the oracle
+  from oracle import halton as ohalton
+  from tqf.math.random import halton
+  got, _ = halton.sample(100, num_results=2**20, randomized=False,
+                         dtype=torch.float32)
+  got = _np(got)
+  rows = np.array([0, 1, 62, 63, 64, 65, 127, 128, 2**19 - 1, 2**19,
+                   2**20 - 65, 2**20 - 64, 2**20 - 1] +
+                  list(np.random.RandomState(8).randint(0, 2**20, 24)))
+  want = ohalton.sample(100, sequence_indices=rows, dtype=np.float32)
+  np.testing.assert_allclose(got[rows], want, rtol=1e-6, atol=1e-30)
+
+
 def test_halton_randomized_integration_and_normal_draws():
   from oracle import halton as ohalton
   from tqf.math.random import halton

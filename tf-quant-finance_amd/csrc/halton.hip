@@ -7,8 +7,10 @@
 // :386-408).  Randomised (Owen 2017, :252-285, :288-330): digit j of axis d
 // goes through perms[j * radix_sum + offset[d] + digit] -- the zeros past
 // the integer's length included, up to max_digits[d] -- and
-// zero_correction[d] is added.  One thread per (point, axis); the value is
-// accumulated in float64 and rounded once to the output type.
+// zero_correction[d] is added.  The value is accumulated in float64 and
+// rounded once to the output type; rows of up to 128 axes are built as
+// 64-point tiles with one axis per wave (halton_tile_kernel), others one
+// thread per (point, axis).
 #include "common.h"
 
 using namespace tqf;
@@ -45,23 +47,11 @@ __device__ __forceinline__ int next_digit(uint64_t& k, uint32_t b) {
   return c;
 }
 
-__global__ void __launch_bounds__(256) halton_kernel(HaltonArgs a) {
-  // (point, axis) of this thread: 32-bit arithmetic when the element
-  // count fits
-  int64_t i;
-  int d;
-  if (a.n * (int64_t)a.dim <= 0xffffffffll - 256) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if ((int64_t)g >= a.n * (int64_t)a.dim) return;
-    const uint32_t i32 = g / (uint32_t)a.dim;
-    i = i32;
-    d = (int)(g - i32 * (uint32_t)a.dim);
-  } else {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.n * (int64_t)a.dim) return;
-    i = g / a.dim;
-    d = (int)(g - i * a.dim);
-  }
+// Element (point i, axis d) as float64: the radical inverse of the 1-based
+// index in base b_d (digit-permuted when randomised), plus the axis's zero
+// correction.
+__device__ __forceinline__ double halton_value(const HaltonArgs& a, int64_t i,
+                                              int d) {
   uint64_t k = (uint64_t)((a.seq ? a.seq[i] : a.first + i) + 1);
   const uint32_t b = (uint32_t)a.bases[d];
   const int nd = a.max_digits[d];
@@ -108,18 +98,73 @@ __global__ void __launch_bounds__(256) halton_kernel(HaltonArgs a) {
     }
   }
   if (a.zero_correction) acc += a.zero_correction[d];
-  const int64_t o = i * a.stride + d;
-  if (a.kind == 1) {
-    ((float*)a.out)[o] = (float)acc;
-  } else if (a.kind == 2) {
-    ((double*)a.out)[o] = acc;
-  } else if (a.kind == 3) {
-    // the reference's transform in float32: erfinv((u - 0.5) * 2) sqrt 2
+  return acc;
+}
+
+// The output value of kind 1 / 3 (float32 uniform / normal) or 2 / 4
+// (float64): the reference's transform for normals, erfinv((u - .5) 2) sqrt 2.
+template <typename T>
+__device__ __forceinline__ T halton_out(int kind, double acc) {
+  if constexpr (sizeof(T) == sizeof(float)) {
     const float u = (float)acc;
-    ((float*)a.out)[o] = erfinv_f32((u - 0.5f) * 2.0f) * 1.41421356f;
+    return kind == 1 ? u : erfinv_f32((u - 0.5f) * 2.0f) * 1.41421356f;
   } else {
-    ((double*)a.out)[o] = erfinv((acc - 0.5) * 2.0) * 1.4142135623730951;
+    return kind == 2 ? acc : erfinv((acc - 0.5) * 2.0) * 1.4142135623730951;
   }
+}
+
+__global__ void __launch_bounds__(256) halton_kernel(HaltonArgs a) {
+  // (point, axis) of this thread: 32-bit arithmetic when the element
+  // count fits
+  int64_t i;
+  int d;
+  if (a.n * (int64_t)a.dim <= 0xffffffffll - 256) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int64_t)g >= a.n * (int64_t)a.dim) return;
+    const uint32_t i32 = g / (uint32_t)a.dim;
+    i = i32;
+    d = (int)(g - i32 * (uint32_t)a.dim);
+  } else {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.n * (int64_t)a.dim) return;
+    i = g / a.dim;
+    d = (int)(g - i * a.dim);
+  }
+  const double acc = halton_value(a, i, d);
+  const int64_t o = i * a.stride + d;
+  if (a.kind == 1 || a.kind == 3)
+    ((float*)a.out)[o] = halton_out<float>(a.kind, acc);
+  else
+    ((double*)a.out)[o] = halton_out<double>(a.kind, acc);
+}
+
+// Tiles of 64 consecutive points x every axis (dim <= kHaltonTileDim, rows
+// contiguous): lane p of a wave takes point p of one axis at a time, so a
+// wave's 64 elements share the base and the digit loop runs the same count
+// on every lane (one thread per element mixed bases in a wave and ran the
+// smallest base's count, about 20 digits at 2^20 points); the tile goes
+// through LDS and out as one contiguous stretch of 64 rows.  The element
+// values are halton_value's.
+constexpr int kHaltonTileDim = 128;
+
+template <typename T>
+__global__ void __launch_bounds__(256) halton_tile_kernel(HaltonArgs a) {
+  extern __shared__ unsigned char smem[];
+  T* tile = (T*)smem;  // [64][dim]
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t i0 = (int64_t)blockIdx.x * 64;
+  const int64_t i = i0 + lane;
+  const bool valid = i < a.n;
+  for (int d = wave; d < a.dim; d += 4)
+    tile[lane * a.dim + d] =
+        valid ? halton_out<T>(a.kind, halton_value(a, i, d)) : (T)0;
+  __syncthreads();
+  int64_t np = a.n - i0;
+  if (np > 64) np = 64;
+  const int cnt = (int)np * a.dim;
+  T* out = (T*)a.out + i0 * a.dim;  // rows contiguous (stride == dim)
+  for (int e = threadIdx.x; e < cnt; e += 256) out[e] = tile[e];
 }
 
 }  // namespace
@@ -146,7 +191,17 @@ extern "C" int tqf_halton_points(const int32_t* bases,
   HaltonArgs a{bases, max_digits, offsets, dim, n, first_index, seq_indices,
                perms, radix_sum, zero_correction, out_kind, out, out_stride};
   const int64_t total = n * (int64_t)dim;
-  hipLaunchKernelGGL(halton_kernel, dim3((unsigned)((total + 255) / 256)),
-                     dim3(256), 0, (hipStream_t)stream, a);
+  if (out_stride == dim && dim <= kHaltonTileDim) {
+    const unsigned blocks = (unsigned)((n + 63) / 64);
+    if (out_kind == 1 || out_kind == 3)
+      hipLaunchKernelGGL(halton_tile_kernel<float>, dim3(blocks), dim3(256),
+                         64 * dim * sizeof(float), (hipStream_t)stream, a);
+    else
+      hipLaunchKernelGGL(halton_tile_kernel<double>, dim3(blocks), dim3(256),
+                         64 * dim * sizeof(double), (hipStream_t)stream, a);
+  } else {
+    hipLaunchKernelGGL(halton_kernel, dim3((unsigned)((total + 255) / 256)),
+                       dim3(256), 0, (hipStream_t)stream, a);
+  }
   return check_launch("tqf_halton_points");
 }
