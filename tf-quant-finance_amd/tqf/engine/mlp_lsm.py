@@ -194,6 +194,17 @@ def clear_cache():
   _STATE.clear()
 
 
+_JUDGE_STREAMS = {}
+
+
+def _judge_stream(dev):
+  """One cached side stream per device (default priority)."""
+  key = str(dev)
+  if key not in _JUDGE_STREAMS:
+    _JUDGE_STREAMS[key] = torch.cuda.Stream(device=dev)
+  return _JUDGE_STREAMS[key]
+
+
 def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
                   num_calibration, payoff=lsm_lib.PAYOFF_MAX_CALL,
                   steps_per_date=200, batch=65536, lr=1e-3, seed=0,
@@ -226,10 +237,24 @@ def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
   model = st['model']
   model.reset(seed)
   losses = []
+  n_cal = int(num_calibration)
+  if judge:
+    # the polynomial judge depends on the paths only: it runs on a side
+    # stream (default priority) while the network trains on this one
+    # (the training kernels hold one wave per SIMD); the two meet at the
+    # paired statistic
+    main = torch.cuda.current_stream(dev)
+    side = _judge_stream(dev)
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+      poly_values = torch.empty(n, **f64)
+      poly, poly_se, _ = lsm_lib._native_lsm(
+          paths, ex, (payoff, np.array([float(strike)])), poly_degree,
+          np.asarray(df).reshape(-1, 1, 1), n_cal, torch.float64, dev,
+          return_details=True, path_values=poly_values)
   _native.call('tqf_lsm_terminal', ctypes.byref(l), ctypes.c_int32(ex[-1]),
                ctypes.c_void_p(cash.data_ptr()),
                ctypes.c_void_p(vals.data_ptr()), s)
-  n_cal = int(num_calibration)
   batch = min(int(batch), n_cal)
   for e in range(len(ex) - 1, 0, -1):
     ratio = df[e + 1] / df[e]
@@ -262,12 +287,12 @@ def mlp_lsm_price(paths, exercise_idx, strike, discount_factors,
   out = {'mlp_price': mlp_price, 'mlp_se': mlp_se,
          'losses': [float(t.item()) for t in losses]}
   if judge:
+    main.wait_stream(side)
+    # read on this stream from here on: their blocks are not handed back
+    # to the side stream before these reads are done
+    for t in (poly_values, poly, poly_se):
+      t.record_stream(main)
     mlp_values = cash + vals
-    poly_values = torch.empty(n, **f64)
-    poly, poly_se, _ = lsm_lib._native_lsm(
-        paths, ex, (payoff, np.array([float(strike)])), poly_degree,
-        np.asarray(df).reshape(-1, 1, 1), n_cal, torch.float64, dev,
-        return_details=True, path_values=poly_values)
     poly_price = float(poly[0])
     poly_se = float(poly_se[0])
     # paired: mean and standard error of df0 (mlp_p - poly_p) over the
