@@ -1,8 +1,11 @@
 // Small MLP (6 -> 64 -> 64 -> 1, SiLU) for the Longstaff-Schwartz
 // continuation value: forward, backward and Adam as hand-written kernels.
 //
-// The 6->64 and 64x64 layers run on MFMA (v_mfma_f32_16x16x16_bf16: bf16
-// operands, float32 accumulation), the 64->1 layer on VALU.  Layer 1 is
+// The 6->64 and 64x64 layers run on MFMA (bf16 operands, float32
+// accumulation: v_mfma_f32_16x16x32_bf16 wherever two k-steps of 16 pair up,
+// i.e. layer 2, dH1, and dW1/dW2 over two groups of samples;
+// v_mfma_f32_16x16x16_bf16 for layer 1's single step), the 64->1 layer on
+// VALU.  Layer 1 is
 // computed transposed, Z1^T = W1^T X^T (inputs padded to k = 16, the bias
 // as the accumulator's initial value), so its output lands in the A-operand
 // layout layer 2 needs; dW1 = X^T dZ1 is one more MFMA per 16-column tile.
@@ -63,6 +66,21 @@ __device__ __forceinline__ float silu_grad_s(float z, float s) {
 
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Two k-steps of 16 in one gfx950 v_mfma_f32_16x16x32_bf16: the operands of
+// step 0 and step 1 concatenated per lane (A lane l holds k 8(l/16) + j of
+// the instruction, here step 0's 4(l/16) + j for j < 4 and step 1's for
+// j >= 4; B the same), so the product sums both steps' k.
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f32x4 mfma2(bf16x4 a0, bf16x4 a1, bf16x4 b0,
+                                       bf16x4 b1, f32x4 c) {
+  const s16x8 a = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+  const s16x8 b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+      __builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0,
+      0);
 }
 
 // Orders this wave's LDS writes and reads (the transpose buffers are private
@@ -137,7 +155,9 @@ __device__ __forceinline__ void forward_group(const float* __restrict__ P,
   for (int t = 0; t < 4; ++t) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc = mfma(f.h1a[c], w2b[t * 4 + c], acc);
+    for (int c = 0; c < 4; c += 2)
+      acc = mfma2(f.h1a[c], f.h1a[c + 1], w2b[t * 4 + c], w2b[t * 4 + c + 1],
+                  acc);
     const float bb = P[oB2 + t * 16 + r];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i] += bb;
@@ -438,13 +458,23 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
         dzb[u][t] = pack_bf16(dz2[u][t][0], dz2[u][t][1], dz2[u][t][2],
                               dz2[u][t][3]);
     }
+    // the groups in flight in pairs: one k = 32 step over both groups'
+    // samples
 #pragma unroll
-    for (int u = 0; u < kF; ++u)
+    for (int u = 0; u + 1 < kF; u += 2)
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
         for (int t = 0; t < 4; ++t)
-          dw2[mt * 4 + t] = mfma(h1t[u][mt], dzb[u][t], dw2[mt * 4 + t]);
+          dw2[mt * 4 + t] = mfma2(h1t[u][mt], h1t[u + 1][mt], dzb[u][t],
+                                  dzb[u + 1][t], dw2[mt * 4 + t]);
+    if (kF % 2)
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          dw2[mt * 4 + t] = mfma(h1t[kF - 1][mt], dzb[kF - 1][t],
+                                 dw2[mt * 4 + t]);
     // dH1 = dZ2 W2^T: A = dZ2 (rows = samples r, k = n c*16 + 4q + j) via LDS
     wave_lds_sync();
 #pragma unroll
@@ -475,13 +505,16 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
       for (int t = 0; t < 4; ++t) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-          acc = mfma(dza[u][c], w2t_s[(t * 4 + c) * 64 + lane], acc);
+        for (int c = 0; c < 4; c += 2)
+          acc = mfma2(dza[u][c], dza[u][c + 1],
+                      w2t_s[(t * 4 + c) * 64 + lane],
+                      w2t_s[(t * 4 + c + 1) * 64 + lane], acc);
         dh[u][t] = acc;
       }
     // dZ1 = dH1 SiLU'(z1) (C layout: rows samples 4q + i, column feature
     // t*16 + r) is the A operand of dW1^T (row feature, k sample): tile t
     // of dW1^T += dZ1^T X
+    bf16x4 dzp[kF][4];
 #pragma unroll
     for (int u = 0; u < kF; ++u) {
       const float* G1 = g1s[wave][u];
@@ -494,9 +527,20 @@ mlp_grad_kernel(const float* __restrict__ params, const float* __restrict__ x,
           d[i] = dh[u][t][i] * G1[(4 * q + i) * kG1Stride + fo];
           db1[t] += d[i];
         }
-        dw1t[t] = mfma(pack_bf16(d[0], d[1], d[2], d[3]), xtb[u], dw1t[t]);
+        dzp[u][t] = pack_bf16(d[0], d[1], d[2], d[3]);
       }
     }
+    // the groups in pairs, as dW2
+#pragma unroll
+    for (int u = 0; u + 1 < kF; u += 2)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        dw1t[t] = mfma2(dzp[u][t], dzp[u + 1][t], xtb[u], xtb[u + 1],
+                        dw1t[t]);
+    if (kF % 2)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        dw1t[t] = mfma(dzp[kF - 1][t], xtb[kF - 1], dw1t[t]);
   }
   // block reduction in a fixed order: each wave folds its four lane groups
   // (q = lane / 16, which hold the same features) with shuffles where
