@@ -46,6 +46,14 @@ __device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Lane `l`'s value of v in every lane (l wave-uniform).
+__device__ __forceinline__ double read_lane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 struct LsmArgs {
   const double* paths;      // float64, element (p, t, j) at
                             // paths[p sp + t st + j sa]
@@ -1024,15 +1032,17 @@ bool moment_plan(int d, int deg, int n_pairs, MomentPlan* out) {
 // MFMA accumulator layout (lane l: column l % 16, rows l / 16 + 4 r); wave
 // w of eight keeps tile rows 15 - w and w (17 tiles, two waves per SIMD so
 // they fit its registers).  Per panel p:
-//   1. the owner of tile (p, p) factors it in LDS (16 sequential columns
-//      on one wave, no workgroup barrier) and forms Y = L_pp^-1 with zero
-//      rows and columns for dropped pivots;
+//   1. the owner of tile (p, p) factors it in registers (16 sequential
+//      columns on one wave, operands exchanged between lanes, no LDS round
+//      and no workgroup barrier) and forms Y = L_pp^-1 in the same column
+//      steps, with zero rows and columns for dropped pivots;
 //   2. each panel tile (a, p) becomes X = A_ap Y^T (4 MFMAs), staged in LDS;
 //   3. each trailing tile (a, b), a >= b > p, takes -= X_ap X_bp^T (4 MFMAs).
 // Two workgroup barriers per panel in place of one per column.  The back
 // solve L^T w = z runs by blocks from the last: w_b = Y_bb^T r_b, then every
 // tile (b, c), c < b, subtracts L_bc^T w_b from r_c (one tile per c and
-// step, so the order of the sums is fixed).  beta = D w.
+// step, so the order of the sums is fixed; the tiles' row sums meet in LDS
+// and are added all at once).  beta = D w.
 constexpr int kSolveWaves = 8;
 constexpr int kSolveThreads = 64 * kSolveWaves;
 constexpr int kBT = 16;         // tile edge
@@ -1043,11 +1053,9 @@ __global__ void __launch_bounds__(kSolveThreads)
 lsm_solve_kernel(const double* __restrict__ g, int K, double rtol,
                  double* __restrict__ beta, int* __restrict__ rank_out) {
   __shared__ double dsc[256];
-  __shared__ double D[kBT * kBLd];           // diagonal tile being factored
   __shared__ double Ys[16][kBT * kBLd];      // Y_pp of every panel
   __shared__ double P[16][kBT * kBLd];       // panel tiles X_ap
   __shared__ double rv[256], wv[256];
-  __shared__ double rinvs[kBT];             // 1 / L_kk of the tile (0: dropped)
   __shared__ double maxdiag_s;
   __shared__ int rank_s;
   const int n = K + 1, kext = K + 1;
@@ -1118,30 +1126,36 @@ lsm_solve_kernel(const double* __restrict__ g, int K, double rtol,
   __syncthreads();
   const double thr = rtol * maxdiag_s;
   // 1. factor diagonal tile (p, p) on its owner (wave p or 15 - p, slot 16
-  // or p), with Y_pp = L_pp^-1 (zero rows and columns where a pivot is
-  // dropped).  Column k: the pivot's reciprocal root from v_rsq_f64 and two
-  // Newton steps, L_ik = D_ik rinv (i > k) and the rank-one update
-  // D_ij -= L_ik L_jk (k < j <= i), in one LDS round per column (lane l
-  // holds entries l, l + 64, ... of the 16 x 16 square).
+  // or p) in registers, in the tile's own layout (lane l: column l % 16,
+  // rows l / 16 + 4 r), together with Y_pp = L_pp^-1 (zero rows and columns
+  // where a pivot is dropped).  The tile is symmetric and both triangles are
+  // updated alike, so row k of the trailing square equals its column k.
+  // Column k: the pivot from its lane (v_readlane), its reciprocal root from
+  // v_rsq_f64 and two Newton steps, L_ik from lane 16 (l / 16) + k (the same
+  // row), L_jk = D_kj and the row k of R = I - (sum of the steps so far)
+  // from lane 16 (k % 4) + l % 16 (the same column); then
+  // D_ij -= L_ik L_jk (i, j > k) and R_ij -= L_ik Y_kj (i > k) with
+  // Y_kj = R_kj / L_kk, the forward substitution L Y = I done column by
+  // column alongside the factorization.
   auto factor_diag = [&](int pd) {
     const int qd = pd < 8 ? 16 : pd;
+    f64x4 dv = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < kBTiles; ++q)
-      if (q == qd)
+      if (q == qd) dv = tile[q];
+    f64x4 yv;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) D[(rl + 4 * r) * kBLd + cl] = tile[q][r];
-    wave_lds_sync();
+    for (int r = 0; r < 4; ++r) yv[r] = rl + 4 * r == cl ? 1.0 : 0.0;
     int kept = 0;
-    for (int k = 0; k < kBT; ++k) {
-      const double dk = D[k * kBLd + k];
-      double dik[4], djk[4], dij[4];
 #pragma unroll
-      for (int e4 = 0; e4 < 4; ++e4) {
-        const int e = 64 * e4 + lane, i = e >> 4, j = e & 15;
-        dik[e4] = D[i * kBLd + k];
-        djk[e4] = D[j * kBLd + k];
-        dij[e4] = D[i * kBLd + j];
-      }
+    for (int k = 0; k < kBT; ++k) {
+      const int rk = k >> 2, gk = k & 3, src = 16 * gk + cl;
+      const double dk = read_lane_f64(dv[rk], 16 * gk + k);
+      double dik[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dik[r] = __shfl(dv[r], (lane & 48) + k, 64);
+      const double dkj = __shfl(dv[rk], src, 64);
+      const double rkj = __shfl(yv[rk], src, 64);
       const bool keep = kBT * pd + k < K && dk > thr && dk > 0;
       double rinv = 0.0;
       if (keep) {
@@ -1152,44 +1166,33 @@ lsm_solve_kernel(const double* __restrict__ g, int K, double rtol,
       }
       const double root = dk * rinv;  // 0 for a dropped pivot
       kept += keep ? 1 : 0;
-      wave_lds_sync();
-      // every entry written back, changed or not (no branches; rinv = 0
-      // leaves the update a no-op and zeroes the column)
+      // branch-free: rinv = 0 leaves the updates no-ops and zeroes row and
+      // column k of L and Y
+      const double ljk = dkj * rinv, ykj = rkj * rinv;
 #pragma unroll
-      for (int e4 = 0; e4 < 4; ++e4) {
-        const int e = 64 * e4 + lane, i = e >> 4, j = e & 15;
-        const double lik = dik[e4] * rinv, ljk = djk[e4] * rinv;
-        double v = dij[e4];
-        v = (j > k && i >= j) ? v - lik * ljk : v;
-        v = (j == k && i > k) ? lik : v;
-        v = (j == k && i == k) ? root : v;
-        D[i * kBLd + j] = v;
+      for (int r = 0; r < 4; ++r) {
+        const int i = rl + 4 * r;
+        const double lik = dik[r] * rinv;
+        double v = dv[r];
+        v = (i > k && cl > k) ? v - lik * ljk : v;
+        v = (cl == k && i > k) ? lik : v;
+        v = (cl == k && i == k) ? root : v;
+        dv[r] = v;
+        double y = yv[r];
+        y = i > k ? y - lik * ykj : y;
+        y = i == k ? ykj : y;
+        yv[r] = y;
       }
-      rinvs[k] = rinv;
-      wave_lds_sync();
     }
-    // Y column `lane`: y_i = (delta_i,lane - sum_m<i L_im y_m) / L_ii
-    if (lane < kBT) {
-      double y[kBT];
 #pragma unroll
-      for (int i = 0; i < kBT; ++i) {
-        double acc = i == lane ? 1.0 : 0.0;
-#pragma unroll
-        for (int m = 0; m < i; ++m) acc -= D[i * kBLd + m] * y[m];
-        y[i] = acc * rinvs[i];
-      }
-#pragma unroll
-      for (int i = 0; i < kBT; ++i) Ys[pd][i * kBLd + lane] = y[i];
-    }
+    for (int r = 0; r < 4; ++r) Ys[pd][(rl + 4 * r) * kBLd + cl] = yv[r];
     // L_pp (lower, zero above) back into the register tile
 #pragma unroll
     for (int q = 0; q < kBTiles; ++q)
       if (q == qd)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = rl + 4 * r;
-          tile[q][r] = row >= cl ? D[row * kBLd + cl] : 0.0;
-        }
+        for (int r = 0; r < 4; ++r)
+          tile[q][r] = rl + 4 * r >= cl ? dv[r] : 0.0;
     if (lane == 0) rank_s += kept;
   };
   // p = -1 only factors tile (0, 0)
@@ -1287,6 +1290,9 @@ lsm_solve_kernel(const double* __restrict__ g, int K, double rtol,
       double wb[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) wb[r] = wv[kBT * b + rl + 4 * r];
+      // every tile (b, c), c < b: each lane's sum over its four rows into
+      // P[c] (free after the factorization), then lane l adds the four row
+      // groups of column l % 16 of blocks c = l / 16 + 4 m into r_c
 #pragma unroll
       for (int q = 0; q < kBTiles; ++q) {
         const int c = tile_b(q);
@@ -1294,11 +1300,18 @@ lsm_solve_kernel(const double* __restrict__ g, int K, double rtol,
           double part = 0.0;
 #pragma unroll
           for (int r = 0; r < 4; ++r) part += tile[q][r] * wb[r];
-          part += __shfl_xor(part, 16, 64);
-          part += __shfl_xor(part, 32, 64);
-          if (rl == 0) rv[kBT * c + cl] -= part;
+          P[c][rl * kBT + cl] = part;
         }
-        __builtin_amdgcn_sched_barrier(0);
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int c = rl + 4 * m;
+        if (c < b) {
+          const double* pc = P[c];
+          rv[kBT * c + cl] -= (pc[cl] + pc[kBT + cl]) +
+                              (pc[2 * kBT + cl] + pc[3 * kBT + cl]);
+        }
       }
     }
     __syncthreads();
