@@ -10,7 +10,8 @@
 // zero_correction[d] is added.  The value is accumulated in float64 and
 // rounded once to the output type; rows of up to 128 axes are built as
 // 64-point tiles with one axis per wave (halton_tile_kernel), others one
-// thread per (point, axis).
+// thread per (point, axis).  Digits come from float64 division by the base
+// (exact quotients while the index fits in 32 bits; see BaseDiv).
 #include "common.h"
 
 using namespace tqf;
@@ -33,13 +34,31 @@ struct HaltonArgs {
   int64_t stride;
 };
 
-// One base-b digit of k: c = k mod b, k /= b.  A 32-bit division when k
-// fits (the common case; the 64-bit one is a long software sequence).
-__device__ __forceinline__ int next_digit(uint64_t& k, uint32_t b) {
+// Division by the axis's base in float64, for k < 2^32 and 2 <= b < 2^13
+// (the 1000th prime is 7919): with inv the double just above 1/b, the
+// product k * inv lies in [k/b, k/b + 2^-20) (two roundings of at most
+// 2^-52 relative on a value below 2^32 / b), and the fraction of k/b is 0
+// or at most 1 - 1/b, so the product stays below the next integer and its
+// truncation is the quotient; the remainder k - q b is exact.  Four
+// instructions in place of the integer division's software sequence.
+struct BaseDiv {
+  double b, inv;
+};
+
+__device__ __forceinline__ BaseDiv base_div(uint32_t b) {
+  const double bd = (double)b;
+  return {bd, __longlong_as_double(__double_as_longlong(1.0 / bd) + 1)};
+}
+
+// One base-b digit of k: c = k mod b, k /= b (float64 while k fits in 32
+// bits, the common case; the 64-bit division is a long software sequence).
+__device__ __forceinline__ int next_digit(uint64_t& k, uint32_t b,
+                                          const BaseDiv& dv) {
   if (k <= 0xffffffffull) {
-    const uint32_t k32 = (uint32_t)k, q = k32 / b;
-    k = q;
-    return (int)(k32 - q * b);
+    const double kd = (double)(uint32_t)k;
+    const double q = trunc(kd * dv.inv);
+    k = (uint64_t)(uint32_t)q;
+    return (int)fma(-q, dv.b, kd);
   }
   const uint64_t q = k / b;
   const int c = (int)(k - q * (uint64_t)b);
@@ -51,7 +70,7 @@ __device__ __forceinline__ int next_digit(uint64_t& k, uint32_t b) {
 // index in base b_d (digit-permuted when randomised), plus the axis's zero
 // correction.
 __device__ __forceinline__ double halton_value(const HaltonArgs& a, int64_t i,
-                                              int d) {
+                                              int d, const BaseDiv& dv) {
   uint64_t k = (uint64_t)((a.seq ? a.seq[i] : a.first + i) + 1);
   const uint32_t b = (uint32_t)a.bases[d];
   const int nd = a.max_digits[d];
@@ -68,7 +87,7 @@ __device__ __forceinline__ double halton_value(const HaltonArgs& a, int64_t i,
     if (nd * bits <= 63) {
       uint64_t R = 0, B = 1;
       for (int j = 0; j < nd; ++j) {
-        const int c = next_digit(k, b);
+        const int c = next_digit(k, b, dv);
         R = R * b + (uint64_t)pd[(int64_t)j * a.radix_sum + c];
         B *= b;
       }
@@ -76,23 +95,26 @@ __device__ __forceinline__ double halton_value(const HaltonArgs& a, int64_t i,
     } else {
       double w = 1.0;
       for (int j = 0; j < nd; ++j) {
-        const int c = next_digit(k, b);
+        const int c = next_digit(k, b, dv);
         w *= (double)b;
         acc += (double)pd[(int64_t)j * a.radix_sum + c] / w;
       }
     }
   } else if (k <= 0xffffffffull) {
-    uint64_t R = 0, B = 1;
-    for (int j = 0; j < nd && k; ++j) {
-      const int c = next_digit(k, b);
-      R = R * b + (uint64_t)c;
-      B *= b;
+    // R < B <= b k < 2^45: exact in float64, and R / B is the same
+    // correctly rounded quotient as from 64-bit integers
+    double kd = (double)k, R = 0.0, B = 1.0;
+    for (int j = 0; j < nd && kd != 0.0; ++j) {
+      const double q = trunc(kd * dv.inv);
+      R = fma(R, dv.b, fma(-q, dv.b, kd));
+      B *= dv.b;
+      kd = q;
     }
-    acc = (double)R / (double)B;
+    acc = R / B;
   } else {
     double w = 1.0;
     for (int j = 0; j < nd && k; ++j) {
-      const int c = next_digit(k, b);
+      const int c = next_digit(k, b, dv);
       w *= (double)b;
       acc += (double)c / w;
     }
@@ -130,7 +152,7 @@ __global__ void __launch_bounds__(256) halton_kernel(HaltonArgs a) {
     i = g / a.dim;
     d = (int)(g - i * a.dim);
   }
-  const double acc = halton_value(a, i, d);
+  const double acc = halton_value(a, i, d, base_div((uint32_t)a.bases[d]));
   const int64_t o = i * a.stride + d;
   if (a.kind == 1 || a.kind == 3)
     ((float*)a.out)[o] = halton_out<float>(a.kind, acc);
@@ -151,6 +173,14 @@ template <typename T>
 __global__ void __launch_bounds__(256) halton_tile_kernel(HaltonArgs a) {
   extern __shared__ unsigned char smem[];
   T* tile = (T*)smem;  // [64][dim]
+  // each axis's base and reciprocal, formed once per workgroup
+  __shared__ double sb[kHaltonTileDim], sinv[kHaltonTileDim];
+  if ((int)threadIdx.x < a.dim) {
+    const BaseDiv dv = base_div((uint32_t)a.bases[threadIdx.x]);
+    sb[threadIdx.x] = dv.b;
+    sinv[threadIdx.x] = dv.inv;
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t i0 = (int64_t)blockIdx.x * 64;
@@ -158,7 +188,8 @@ __global__ void __launch_bounds__(256) halton_tile_kernel(HaltonArgs a) {
   const bool valid = i < a.n;
   for (int d = wave; d < a.dim; d += 4)
     tile[lane * a.dim + d] =
-        valid ? halton_out<T>(a.kind, halton_value(a, i, d)) : (T)0;
+        valid ? halton_out<T>(a.kind, halton_value(a, i, d, {sb[d], sinv[d]}))
+              : (T)0;
   __syncthreads();
   int64_t np = a.n - i0;
   if (np > 64) np = 64;
