@@ -188,19 +188,22 @@ struct Stepper {
         const R v = x[1];
         const R m = c[8] + (v - c[8]) * c[0];
         const R s2 = v * c[1] + c[2];
-        const R psi = s2 / (m * m);
+        // float32 divisions by the reciprocal (fast_div): the lanes of a
+        // wave take both branches, and the correctly rounded sequence
+        // dominated the step
+        const R psi = fast_div(s2, m * m);
         R vn;
         if (psi < (R)1.5) {
-          const R pinv = (R)2 / psi;
+          const R pinv = fast_div((R)2, psi);
           const R b2 = pinv - (R)1 + fast_sqrt(pinv * (pinv - (R)1));
-          const R am = m / ((R)1 + b2);
+          const R am = fast_div(m, (R)1 + b2);
           const R t = fast_sqrt(b2) + z[0];
           vn = am * t * t;
         } else {
-          const R p = (psi - (R)1) / (psi + (R)1);
-          const R beta = ((R)1 - p) / m;
+          const R p = fast_div(psi - (R)1, psi + (R)1);
+          const R beta = fast_div((R)1 - p, m);
           const R u = (R)0.5 * ((R)1 + erf(z[0] * (R)0.70710678118654752));
-          vn = u > p ? (log((R)1 - p) - log((R)1 - u)) / beta : (R)0;
+          vn = u > p ? fast_div(log((R)1 - p) - log((R)1 - u), beta) : (R)0;
         }
         x[0] = x[0] + c[3] + c[4] * v + c[5] * vn +
                fast_sqrt(c[6] * v + c[7] * vn) * z[1];
