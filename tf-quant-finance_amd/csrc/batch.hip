@@ -189,7 +189,8 @@ __device__ void heston_path(const Contract& c, int64_t p, uint32_t stream,
         const float dw0 = z0s[u] * sdt, dw1 = z1s[u] * sdt;
         const float sqb = xb * dw0 + vb * (xr * dw0 + xs * dw1);
         float vpb = -0.5f * dt * xb - dt * c.kappa * vb;
-        vpb += sq > 0.f ? sqb * 0.5f / sq : 0.f;
+        // d sqrt(v+) / dv+ = 1 / (2 sqrt(v+)) by the hardware reciprocal
+        vpb += sq > 0.f ? sqb * 0.5f * __builtin_amdgcn_rcpf(sq) : 0.f;
         rb += xb * dt;
         kb += vb * dt * (c.theta - vp);
         tb += vb * dt * c.kappa;
@@ -246,7 +247,9 @@ __device__ void heston_path_tangent(const Contract& c, int64_t p,
     const float sq = __builtin_amdgcn_sqrtf(vp);
     const float dw0 = z0 * sdt, dw1 = z1 * sdt;
     const float cc = xr * dw0 + xs * dw1;
-    const float hs = sq > 0.f ? 0.5f / sq : 0.f;
+    // d sqrt(v+) / dv+ by the hardware reciprocal (one instruction in
+    // place of a correctly rounded division by sq)
+    const float hs = sq > 0.f ? 0.5f * __builtin_amdgcn_rcpf(sq) : 0.f;
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       const float tvp = pos ? tv[q] : 0.f;
