@@ -408,7 +408,7 @@ __global__ void __launch_bounds__(kBlock) euler_kernel(EulerArgs<R> a) {
         while (lv_row + 1 < a.lv_nt && a.lv_t[lv_row + 1] <= t) ++lv_row;
         if (lv_row + 1 < a.lv_nt) {
           const R t0 = a.lv_t[lv_row], t1 = a.lv_t[lv_row + 1];
-          lv_w = t <= t0 ? (R)0 : (t - t0) / (t1 - t0);
+          lv_w = t <= t0 ? (R)0 : fast_div(t - t0, t1 - t0);
         }
       }
       Stepper<R, MODEL>::step(a, x, zc, dt, sdt, tab, lv_row, lv_w, i, row);
