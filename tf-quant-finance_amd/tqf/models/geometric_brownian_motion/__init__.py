@@ -122,10 +122,12 @@ def run_exact(times, drift, vol, chol, s0, num_samples, rng, dtype,
           np.int32), torch.int32, [(r, 'sobol_shift')])
   # every slot is written unless a requested time lies past the executed
   # steps; only then does the output need zeros first
-  written = set(int(v) for v in slots[:executed] if v >= 0)
+  written = np.zeros(k, bool)
+  done = np.asarray(slots[:executed])
+  written[done[done >= 0]] = True
   if init_slot >= 0:
-    written.add(int(init_slot))
-  alloc = torch.empty if len(written) == k else torch.zeros
+    written[init_slot] = True
+  alloc = torch.empty if written.all() else torch.zeros
   if mode == 'paths':
     out = alloc((num_samples, k, d), dtype=dtype, device=dev)
     code = _native.OUT_PATH_MAJOR
