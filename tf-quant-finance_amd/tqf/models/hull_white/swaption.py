@@ -138,16 +138,22 @@ def _lsm_inputs(plan, bonds, r_t, pay, dcf, cpn, tdt):
   n = r_t.shape[0]
   sim_times = plan['sim_times']
   dt = np.concatenate([[0.0], sim_times[1:] - sim_times[:-1]])
-  disc = torch.cumprod(
-      torch.exp(-r_t[..., 0] * torch.as_tensor(dt, dtype=tdt, device=dev)),
-      dim=1)  # [N, k]
+  unique_ex = plan['unique_ex']
+  u_idx = np.searchsorted(sim_times, unique_ex)
+  # path discount factors at the exercise dates only, exp(-sum r dt) up to
+  # each date: one [N, k] x [k, n_u] product with the masked step lengths
+  # (the same values as the cumulative product of the per-step factors, up
+  # to rounding, without scanning all k dates of every path)
+  steps = np.where(np.arange(dt.shape[0])[:, None] <= u_idx[None, :],
+                   dt[:, None], 0.0)
+  disc_u = torch.exp(-(r_t[..., 0] @ torch.as_tensor(steps, dtype=tdt,
+                                                      device=dev)))
   c = torch.as_tensor(cpn * dcf, dtype=tdt, device=dev)
   fixed = (c * bonds).sum(-1)
   floating = 1.0 - bonds[..., -1]
   swap = floating - fixed  # [N, batch, n_ex]
   # payoffs on the union of exercise dates; a date that is not one of a
   # swaption's own dates pays nothing for it (swaption.py:876-935)
-  unique_ex = plan['unique_ex']
   n_u = unique_ex.shape[0]
   shape = plan['batch_shape']
   flat_b = int(np.prod(shape)) if shape else 1
@@ -161,8 +167,8 @@ def _lsm_inputs(plan, bonds, r_t, pay, dcf, cpn, tdt):
     is_ex[b, exi[b]] = True
   payoff_table = torch.where(is_ex[None], torch.relu(pay_u),
                              torch.zeros_like(pay_u))
-  u_idx = torch.as_tensor(np.searchsorted(sim_times, unique_ex), device=dev)
-  return r_t[:, u_idx], payoff_table, disc[:, u_idx]
+  return (r_t[:, torch.as_tensor(u_idx, device=dev)], payoff_table,
+          disc_u)
 
 
 def _price_on_paths(plan, bonds, r_t, pay, dcf, cpn, basis, tdt):
