@@ -112,10 +112,18 @@ def _exercise_bonds(model, plan, r_t, pay):
   (swaption.py:655-672) that the payoffs read, without materialising the
   whole [N, tenors, times] curve."""
   dev = r_t.device
-  sim_idx = np.searchsorted(plan['sim_times'], plan['ex_full'])
-  r_ex = r_t[:, torch.as_tensor(sim_idx, device=dev), 0]
+  # the short rate once per exercise date [N, batch, n_ex], and the bond
+  # as exp(C - r G) with the host constant C = ln(P(0,T)/P(0,t)) + f(0,t) G
+  # - y G^2 / 2: one fused multiply-add and one exponential over the
+  # [N, batch, n_ex, n_pay] tensor (the same bonds as model._bond_prices
+  # up to rounding)
+  ex = plan['ex_full'][..., 0]
+  sim_idx = np.searchsorted(plan['sim_times'], ex)
+  r_e = r_t[:, torch.as_tensor(sim_idx, device=dev), 0]
   f0, p0big, g, y = model._bond_constants(plan['ex_full'], pay)
-  return model._bond_prices(r_ex, f0, p0big, g, y)
+  c = np.log(p0big) + f0 * g - 0.5 * y * g * g
+  as_t = lambda a: torch.as_tensor(a, dtype=r_t.dtype, device=dev)
+  return torch.addcmul(as_t(c), r_e[..., None], as_t(-g)).exp_()
 
 
 def bonds_from_curve_paths(plan, p_t_tau, pay):
@@ -148,10 +156,18 @@ def _lsm_inputs(plan, bonds, r_t, pay, dcf, cpn, tdt):
                    dt[:, None], 0.0)
   disc_u = torch.exp(-(r_t[..., 0] @ torch.as_tensor(steps, dtype=tdt,
                                                       device=dev)))
-  c = torch.as_tensor(cpn * dcf, dtype=tdt, device=dev)
-  fixed = (c * bonds).sum(-1)
-  floating = 1.0 - bonds[..., -1]
-  swap = floating - fixed  # [N, batch, n_ex]
+  # swap = (1 - P_last) - sum_j c_j P_j = 1 - sum_j w_j P_j with w_j = c_j
+  # plus 1 at the last payment: one float64 product of the bonds
+  # [N, batch * n_ex * n_pay] with the block-diagonal weights
+  n_pay = bonds.shape[-1]
+  n_k = int(np.prod(bonds.shape[1:-1]))
+  w = np.zeros((n_k * n_pay, n_k))
+  cw = np.broadcast_to(cpn * dcf, bonds.shape[1:]).reshape(n_k, n_pay).copy()
+  cw[:, -1] += 1.0
+  for k in range(n_k):
+    w[k * n_pay:(k + 1) * n_pay, k] = cw[k]
+  swap = (1.0 - bonds.reshape(n, n_k * n_pay) @ torch.as_tensor(
+      w, dtype=tdt, device=dev)).reshape(bonds.shape[:-1])  # [N, batch, n_ex]
   # payoffs on the union of exercise dates; a date that is not one of a
   # swaption's own dates pays nothing for it (swaption.py:876-935)
   n_u = unique_ex.shape[0]
