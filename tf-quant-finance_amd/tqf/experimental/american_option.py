@@ -25,8 +25,8 @@ import numpy as np
 import torch
 
 from tqf import _native
+from tqf.engine import euler as engine
 from tqf.math.random.multivariate_normal import RandomType
-from tqf.models import GeometricBrownianMotion
 from tqf.models import longstaff_schwartz as lsm_lib
 from tqf.models import utils
 
@@ -71,16 +71,20 @@ def bs_lsm_price(spots, expiry_times, strikes, volatility, discount_factors,
                                     bool).reshape(-1), [n_opt])
   rate = -np.log(df) / expiry
   times = np.linspace(0.0, 1.0, int(num_exercise_times))
-  gbm = GeometricBrownianMotion(0.0, 1.0, dtype=torch.float64)
-  # unit paths exp(-t/2 + W(t)) on the exercise grid, antithetic
-  unit = gbm.sample_paths(times, initial_state=1.0,
-                          num_samples=int(num_samples), seed=seed,
-                          random_type=RandomType.STATELESS_ANTITHETIC)
+  # W(t) on the exercise grid, antithetic, in date-major memory [T, N]
+  # (the regression kernels read one date of every path at a time): the
+  # fused kernel's constant-coefficient model dX = 0 dt + 1 dW, which draws
+  # the same normals per path and step as the unit GBM paths
+  # exp(-t/2 + W(t)) and writes W without their exp and log
+  all_times = np.concatenate([[0.0], np.asarray(times, np.float64)])
+  keep = np.concatenate([[False], np.ones(len(times), bool)])
+  rng = engine.RngSpec.from_random_type(RandomType.STATELESS_ANTITHETIC,
+                                        seed, 0)
+  w_t = engine.run(('const', 0.0, 1.0), all_times, keep, len(times),
+                   np.zeros(1), int(num_samples), rng, torch.float64,
+                   mode='store', dev=dev)[:, 0, :]
+  w = w_t.t()  # [N, T]
   t = torch.as_tensor(times, dtype=torch.float64, device=dev).reshape(1, -1)
-  w = torch.log(unit[..., 0]) + 0.5 * t  # W(t), [N, T]
-  # the same in date-major memory [T, N]: the regression kernels read one
-  # date of every path at a time
-  w_t = w.t().contiguous()
   t_t = t.t()
   ex = list(range(int(num_exercise_times)))
   if basis_fn is None:
